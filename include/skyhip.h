@@ -1,0 +1,436 @@
+/*
+ * skyhip.h -- C ABI of the MI355X (gfx950) implementation of SkyRendering's
+ * data-parallel passes: the atmosphere LUT builds (transmittance,
+ * multiple scattering, sky view, aerial perspective), the volumetric-cloud
+ * raymarcher, the cloud-map / detail noise generators and the voxel-cloud path
+ * tracer, plus two additions that have no reference counterpart (a batched LUT
+ * builder and a learned sky-view surrogate).
+ *
+ * Conventions
+ *  - Every parameter block below is byte-identical to the std140 uniform block
+ *    the corresponding GLSL pass reads (sizes and offsets are static_asserted in
+ *    the implementation).  Units are km and radians, as in the shaders.
+ *  - Images are device pointers in GL texel order: texel (x, y, z) lives at
+ *    ((z * height) + y) * width + x, channels interleaved.  RGBA32F LUTs are
+ *    float[4] per texel.  RGBA16F images are IEEE binary16[4] per texel.
+ *  - Mipmapped UNORM8 textures are one device allocation holding every level
+ *    back to back (level 0 first); level n has max(1, size >> n) texels per axis.
+ *  - Every function returns SKYHIP_OK (0) or a negative SKYHIP_ERR_* code and
+ *    never throws.  skyhip_last_error() gives the message of the last failure
+ *    on the calling thread.
+ *  - Work is enqueued on the context's stream and is asynchronous unless a
+ *    function says otherwise.  A context is thread-compatible: calls on one
+ *    context must not overlap; different contexts may run concurrently.
+ */
+#ifndef SKYHIP_H
+#define SKYHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes                                                               */
+/* ------------------------------------------------------------------------- */
+#define SKYHIP_OK 0
+#define SKYHIP_ERR_INVALID_ARGUMENT (-1) /* null pointer, unknown enum, bad handle */
+#define SKYHIP_ERR_INVALID_SIZE (-2)     /* zero / negative / too large / mismatched sizes */
+#define SKYHIP_ERR_NON_FINITE (-3)       /* a parameter is NaN or infinite, or degenerate */
+#define SKYHIP_ERR_DEVICE (-4)           /* HIP runtime error (no device, launch failure) */
+#define SKYHIP_ERR_OUT_OF_MEMORY (-5)    /* device allocation failed */
+#define SKYHIP_ERR_NOT_READY (-6)        /* required state missing (e.g. surrogate untrained) */
+
+/* ------------------------------------------------------------------------- */
+/* Parameter blocks (std140 images of the reference's uniform blocks)          */
+/* ------------------------------------------------------------------------- */
+
+/* AtmosphereBufferData, 128 bytes: shaders/SkyRendering/Atmosphere.glsl:6-32,
+ * host struct src/SkyRendering/Atmosphere.cpp:21-47. */
+typedef struct SkyhipAtmosphereBufferData {
+    float solar_illuminance[3];
+    float sun_angular_radius; /* radians */
+    float rayleigh_scattering[3];
+    float inv_rayleigh_exponential_distribution;
+    float mie_scattering[3];
+    float inv_mie_exponential_distribution;
+    float mie_absorption[3];
+    float ozone_center_altitude;
+    float ozone_absorption[3];
+    float inv_ozone_width;
+    float ground_albedo[3];
+    float mie_phase_g;
+    float _atmosphere_padding[3];
+    float multiscattering_mask;
+    float bottom_radius;
+    float top_radius;
+    float transmittance_steps;
+    float multiscattering_steps;
+} SkyhipAtmosphereBufferData;
+
+/* AtmosphereParameters (the JSON-facing struct), src/SkyRendering/Atmosphere.h:10-38.
+ * Angles in degrees, lengths in km, scattering = colour * scale (per km). */
+typedef struct SkyhipAtmosphereParameters {
+    float solar_illuminance[3];
+    float sun_angular_radius; /* degrees */
+    float bottom_radius;
+    float thickness;
+    float ground_albedo[3];
+    float rayleigh_exponential_distribution;
+    float rayleigh_scattering_scale;
+    float rayleigh_scattering[3];
+    float mie_exponential_distribution;
+    float mie_phase_g;
+    float mie_scattering_scale;
+    float mie_scattering[3];
+    float mie_absorption_scale;
+    float mie_absorption[3];
+    float ozone_center_altitude;
+    float ozone_width;
+    float ozone_absorption_scale;
+    float ozone_absorption[3];
+    float transmittance_steps;
+    float multiscattering_steps;
+    float multiscattering_mask;
+} SkyhipAtmosphereParameters;
+
+/* AtmosphereRenderBufferData, 320 bytes: shaders/SkyRendering/AtmosphereRenderer.glsl:25-50,
+ * host struct src/SkyRendering/AtmosphereRenderer.cpp:25-50.  Matrices are
+ * column-major float[16] (GLSL / glm memory order). */
+typedef struct SkyhipAtmosphereRenderBufferData {
+    float sun_direction[3];
+    float star_luminance_scale;
+    float earth_center[3];
+    float camera_earth_center_distance;
+    float camera_position[3];
+    float raymarching_steps;
+    float up_direction[3];
+    float sky_view_lut_steps;
+    float right_direction[3];
+    float aerial_perspective_lut_steps;
+    float front_direction[3];
+    float aerial_perspective_lut_max_distance;
+    float moon_position[3];
+    float moon_radius;
+    float inv_view_projection[16];
+    float light_view_projection[16];
+    float padding00;
+    float uInvShadowFroxelMaxDistance;
+    float blocker_kernel_size_k;
+    float pcss_size_k;
+    float uCloudShadowMapMat[16];
+} SkyhipAtmosphereRenderBufferData;
+
+/* AtmosphereRenderParameters, src/SkyRendering/AtmosphereRenderer.h:12-35
+ * (the fields the sky-view / aerial-perspective passes read). */
+typedef struct SkyhipAtmosphereRenderParameters {
+    float sun_direction_theta; /* zenith angle, degrees */
+    float sun_direction_phi;   /* azimuth, degrees */
+    float star_luminance_scale;
+    float raymarching_steps;
+    float sky_view_lut_steps;
+    float aerial_perspective_lut_steps;
+    float aerial_perspective_lut_max_distance;
+    float camera_position[3];  /* world space, km, y up */
+    float view_projection[16]; /* column-major */
+} SkyhipAtmosphereRenderParameters;
+
+/* Camera, src/Base/include/Camera.h + src/Base/src/Camera.cpp:8-37. */
+typedef struct SkyhipCamera {
+    float position[3];
+    float yaw;   /* degrees */
+    float pitch; /* degrees */
+    float aspect;
+    float fovy;  /* degrees */
+    float z_near;
+    float z_far;
+} SkyhipCamera;
+
+/* VolumetricCloudCommonBufferData, 384 bytes:
+ * shaders/SkyRendering/VolumetricCloudCommon.glsl:6-28, src/SkyRendering/VolumetricCloud.cpp:11-33. */
+typedef struct SkyhipCloudCommonBufferData {
+    float uInvMVP[16];
+    float uReprojectMat[16];
+    float uLightVP[16];
+    float uInvLightVP[16];
+    float uShadowMapReprojectMat[16];
+    float uCameraPos[3];
+    uint32_t uBaseShadingIndex;
+    float uLinearDepthParam[2];
+    float uBottomAltitude;
+    float uTopAltitude;
+    float uSunDirection[3];
+    float uFrameID;
+    float uInvShadowFroxelMaxDistance;
+    float uAerialPerspectiveLutMaxDistance;
+    float uShadowFroxelMaxDistance;
+    float uEarthRadius;
+} SkyhipCloudCommonBufferData;
+
+/* VolumetricCloudBufferData, 64 bytes: shaders/SkyRendering/VolumetricCloudRender.comp:17-32,
+ * src/SkyRendering/VolumetricCloud.cpp:35-50. */
+typedef struct SkyhipCloudBufferData {
+    float uSunIlluminanceScale;
+    float uMaxRaymarchDistance;
+    float uMaxRaymarchSteps;
+    float uMaxVisibleDistance;
+    float uEnvColorScale[3];
+    float uShadowSteps;
+    float uSunMultiscatteringSigmaScale;
+    float uEnvMultiscatteringSigmaScale;
+    float uShadowDistance;
+    float uEnvBottomVisibility;
+    float padding_[3];
+    float uEnvSunHeightCurveExp;
+} SkyhipCloudBufferData;
+
+/* VolumetricCloud's serialised fields, src/SkyRendering/VolumetricCloud.h:20-35. */
+typedef struct SkyhipVolumetricCloudParameters {
+    float bottom_altitude;
+    float thickness;
+    float max_raymarch_distance;
+    float max_raymarch_steps;
+    float max_visible_distance;
+    float env_color[3];
+    float env_color_scale;
+    float sun_illuminance_scale;
+    float shadow_steps;
+    float shadow_distance;
+    float shadow_map_max_distance;
+    float shadow_froxel_max_distance;
+    float sun_multiscattering_sigma_scale;
+    float env_multiscattering_sigma_scale;
+    float env_bottom_visibility;
+    float env_sun_height_curve_exp;
+} SkyhipVolumetricCloudParameters;
+
+/* SampleInfo + VolumetricCloudMaterialCommonBufferData, 64 bytes:
+ * shaders/SkyRendering/VolumetricCloudDefaultMaterialCommon.glsl:5-18,
+ * src/SkyRendering/VolumetricCloudDefaultMaterial.cpp:9-22. */
+typedef struct SkyhipSampleInfo {
+    float bias[2];
+    float frequency;
+    float k_lod;
+} SkyhipSampleInfo;
+
+typedef struct SkyhipCloudMaterialCommonBufferData {
+    SkyhipSampleInfo uCloudMapSampleInfo;
+    SkyhipSampleInfo uDetailSampleInfo;
+    SkyhipSampleInfo uDisplacementSampleInfo;
+    float padding0[2];
+    float uLodBias;
+    float uDensity;
+} SkyhipCloudMaterialCommonBufferData;
+
+/* VolumetricCloudDefaultMaterial1 block, 32 bytes:
+ * shaders/SkyRendering/VolumetricCloudDefaultMaterial1.glsl:3-12,
+ * src/SkyRendering/VolumetricCloudDefaultMaterial.cpp:228-237. */
+typedef struct SkyhipCloudMaterial1BufferData {
+    float uBaseDensityThreshold;
+    float uBaseHeightHardness;
+    float uBaseEdgeHardness;
+    float uDetailBase;
+    float uDetailScale;
+    float uHeightCut; /* = 1 - height_cut_ (VolumetricCloudDefaultMaterial.cpp:253) */
+    float uEdgeCur;
+    float padding1;
+} SkyhipCloudMaterial1BufferData;
+
+/* VolumetricCloudDefaultMaterial0 block, 16 bytes:
+ * shaders/SkyRendering/VolumetricCloudDefaultMaterial0.glsl:3-7, VolumetricCloudDefaultMaterial.cpp:183-187. */
+typedef struct SkyhipCloudMaterial0BufferData {
+    float uDetailParam[2];
+    float uDisplacementScale;
+    float padding1;
+} SkyhipCloudMaterial0BufferData;
+
+/* Minimal material block, 16 bytes: shaders/SkyRendering/VolumetricCloudMaterialMinimal.glsl:1-4. */
+typedef struct SkyhipCloudMinimalBufferData {
+    float padding[3];
+    float uDensity;
+} SkyhipCloudMinimalBufferData;
+
+/* Voxel material block, 32 bytes: shaders/SkyRendering/VolumetricCloudMaterialVoxel.glsl:3-10,
+ * src/SkyRendering/VolumetricCloudVoxelMaterial.cpp:17-24. */
+typedef struct SkyhipCloudVoxelBufferData {
+    float uSampleFrequency[2];
+    float uLodBias;
+    float uDensity;
+    float uSampleBias[2];
+    float uSampleLodK;
+    float voxel_material_padding;
+} SkyhipCloudVoxelBufferData;
+
+/* The material plugin (IVolumetricCloudMaterial, src/SkyRendering/IVolumetricCloudMaterial.h:9-30)
+ * becomes a tagged parameter set: the kernels are specialised per kind, so
+ * there is no virtual dispatch on the device. */
+typedef enum SkyhipCloudMaterialKind {
+    SKYHIP_MATERIAL_DEFAULT0 = 0, /* VolumetricCloudDefaultMaterial0.glsl */
+    SKYHIP_MATERIAL_DEFAULT1 = 1, /* VolumetricCloudDefaultMaterial1.glsl */
+    SKYHIP_MATERIAL_MINIMAL = 2,  /* VolumetricCloudMaterialMinimal.glsl */
+    SKYHIP_MATERIAL_VOXEL = 3     /* VolumetricCloudMaterialVoxel.glsl (R8 3D texture with mips) */
+} SkyhipCloudMaterialKind;
+
+typedef struct SkyhipCloudMaterial {
+    int32_t kind; /* SkyhipCloudMaterialKind */
+    SkyhipCloudMaterialCommonBufferData common;  /* DEFAULT0 / DEFAULT1 */
+    SkyhipCloudMaterial1BufferData material1;    /* DEFAULT1 */
+    SkyhipCloudMaterial0BufferData material0;    /* DEFAULT0 */
+    SkyhipCloudMinimalBufferData minimal;        /* MINIMAL */
+    SkyhipCloudVoxelBufferData voxel;            /* VOXEL */
+    /* Device textures, all levels packed (see conventions).  Sampler state is
+     * fixed to the reference's: REPEAT + LINEAR mag + NEAREST_MIPMAP_NEAREST min
+     * for the default materials (VolumetricCloudDefaultMaterial.cpp:112-116),
+     * CLAMP_TO_BORDER(0) for the voxel material (VolumetricCloudVoxelMaterial.cpp:30-37). */
+    const uint8_t* cloud_map;    /* RG8 2D, cloud_map_size^2 */
+    int32_t cloud_map_size;
+    int32_t cloud_map_levels;
+    const uint8_t* detail;       /* R8 3D, detail_size^3 */
+    int32_t detail_size;
+    int32_t detail_levels;
+    const uint8_t* displacement; /* RGBA8 2D, displacement_size^2 (DEFAULT0 only) */
+    int32_t displacement_size;
+    int32_t displacement_levels;
+    const uint8_t* voxel_texture; /* R8 3D, voxel_dim[0] x voxel_dim[1] x voxel_dim[2] (VOXEL only) */
+    int32_t voxel_dim[3];
+    int32_t voxel_levels;
+} SkyhipCloudMaterial;
+
+/* NoiseCreateInfo, src/SkyRendering/VolumetricCloudDefaultMaterial.h:57-62 and
+ * shaders/SkyRendering/NoiseGen.comp:5-10. */
+typedef struct SkyhipNoiseCreateInfo {
+    uint32_t seed;
+    uint32_t base_frequency;
+    float remap_min;
+    float remap_max;
+} SkyhipNoiseCreateInfo;
+
+/* ------------------------------------------------------------------------- */
+/* Context, memory and timing helpers                                         */
+/* ------------------------------------------------------------------------- */
+typedef struct SkyhipContext SkyhipContext;
+
+/* Creates a context on HIP device `device` with its own non-blocking stream. */
+int skyhip_create(int device, SkyhipContext** out_ctx);
+int skyhip_destroy(SkyhipContext* ctx);
+/* Message of the last failed call on this thread ("" if none). */
+const char* skyhip_last_error(void);
+/* The context's hipStream_t, as an opaque pointer. */
+void* skyhip_stream(SkyhipContext* ctx);
+int skyhip_synchronize(SkyhipContext* ctx);
+
+int skyhip_malloc(SkyhipContext* ctx, size_t bytes, void** out_device_ptr);
+int skyhip_free(SkyhipContext* ctx, void* device_ptr);
+int skyhip_memcpy_h2d(SkyhipContext* ctx, void* dst_device, const void* src_host, size_t bytes);
+int skyhip_memcpy_d2h(SkyhipContext* ctx, void* dst_host, const void* src_device, size_t bytes);
+int skyhip_memcpy_d2d(SkyhipContext* ctx, void* dst_device, const void* src_device, size_t bytes);
+int skyhip_memset(SkyhipContext* ctx, void* dst_device, int value, size_t bytes);
+
+/* Stream-ordered timing: records event `slot` (0..63) on the context's stream. */
+int skyhip_event_record(SkyhipContext* ctx, int slot);
+/* Milliseconds between two recorded events (synchronises on `end_slot`). */
+int skyhip_event_elapsed_ms(SkyhipContext* ctx, int start_slot, int end_slot, float* out_ms);
+
+/* Device properties used to size launches (CU count, wave size, clock). */
+int skyhip_device_info(SkyhipContext* ctx, int32_t* cu_count, int32_t* wave_size, int32_t* clock_khz,
+                       char* name, int32_t name_len);
+
+/* ------------------------------------------------------------------------- */
+/* Host-side parameter conversion (no device work)                            */
+/* ------------------------------------------------------------------------- */
+
+/* AssignBufferData, src/SkyRendering/Atmosphere.cpp:49-70. */
+int skyhip_atmosphere_assign_buffer_data(const SkyhipAtmosphereParameters* parameters,
+                                         SkyhipAtmosphereBufferData* out);
+
+/* AssignBufferData, src/SkyRendering/AtmosphereRenderer.cpp:52-83, with
+ * Earth::center() = (0, -bottom_radius, 0) (src/SkyRendering/Earth.h:37).
+ * Moon, shadow-map and cloud-shadow fields are zeroed (those passes are not
+ * part of this library). */
+int skyhip_atmosphere_render_assign_buffer_data(const SkyhipAtmosphereRenderParameters* parameters,
+                                                float bottom_radius,
+                                                SkyhipAtmosphereRenderBufferData* out);
+
+/* Camera::ViewMatrix() * Camera::ProjectionMatrix() as one column-major
+ * view-projection matrix (src/Base/src/Camera.cpp:13-37, glm::lookAt and
+ * glm::perspective with GL clip conventions). Also returns the camera basis. */
+int skyhip_camera_view_projection(const SkyhipCamera* camera, float out_view_projection[16],
+                                  float out_front[3], float out_right[3], float out_up[3]);
+
+/* VolumetricCloud::Update, src/SkyRendering/VolumetricCloud.cpp:188-271: the
+ * local z-up frame at the ground point below the camera, uInvMVP, the local
+ * sun direction and camera position, and both std140 blocks.  frame_id is the
+ * reference's frame_id_ (0..255).  Shadow-map matrices are left identity. */
+int skyhip_cloud_assign_buffer_data(const SkyhipVolumetricCloudParameters* cloud,
+                                    const SkyhipCamera* camera, float bottom_radius,
+                                    const float sun_direction_world[3],
+                                    float aerial_perspective_lut_max_distance, int32_t frame_id,
+                                    SkyhipCloudCommonBufferData* out_common,
+                                    SkyhipCloudBufferData* out_cloud,
+                                    float out_model[16]);
+
+/* gen_sample_info + CalKLod, src/SkyRendering/VolumetricCloudDefaultMaterial.cpp:24-28,88-92. */
+int skyhip_cloud_sample_info(float repeat_size, int32_t tex_max_width, float fovy_degrees,
+                             float viewport_w, float viewport_h, double offset_x, double offset_y,
+                             SkyhipSampleInfo* out);
+
+/* ------------------------------------------------------------------------- */
+/* Atmosphere LUTs                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* Transmittance LUT (shaders/SkyRendering/Atmosphere.glsl:309-337), replaces the
+ * TRANSMITTANCE_COMPUTE_PROGRAM dispatch in Atmosphere::UpdateLuts
+ * (src/SkyRendering/Atmosphere.cpp:112-113).  out: RGBA32F width x height, a = 1. */
+int skyhip_transmittance_lut(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                             float* transmittance_rgba, int32_t width, int32_t height);
+
+/* Multiple-scattering LUT (Atmosphere.glsl:342-438), replaces the
+ * MULTISCATTERING_COMPUTE_PROGRAM dispatch (Atmosphere.cpp:116-121).  Reads the
+ * transmittance LUT with bilinear CLAMP_TO_EDGE filtering.  out: RGBA32F, a = 1. */
+int skyhip_multiscattering_lut(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                               const float* transmittance_rgba, int32_t transmittance_width,
+                               int32_t transmittance_height, float* multiscattering_rgba,
+                               int32_t width, int32_t height);
+
+/* Atmosphere::UpdateLuts (src/SkyRendering/Atmosphere.cpp:101-124): converts
+ * the parameters and builds both LUTs, stream-ordered (the reference's missing
+ * barrier between the two dispatches is a stream dependency here). */
+int skyhip_atmosphere_update_luts(SkyhipContext* ctx, const SkyhipAtmosphereParameters* parameters,
+                                  float* transmittance_rgba, int32_t transmittance_width,
+                                  int32_t transmittance_height, float* multiscattering_rgba,
+                                  int32_t multiscattering_width, int32_t multiscattering_height);
+
+/* Sky-view LUT (shaders/SkyRendering/AtmosphereRenderer.glsl:147-186), replaces
+ * sky_view_program_.Dispatch (src/SkyRendering/AtmosphereRenderer.cpp:221-226).
+ * The compile-time SKY_VIEW_LUT_SIZE becomes width x height.  blue_noise may be
+ * NULL (DITHER_SAMPLE_POINT_ENABLE 0, the reference default); otherwise it is an
+ * R16 UNORM tile of blue_noise_size^2 (power of two) read at (x, y) & (size-1).
+ * Outputs: RGBA32F (L, 0) and (T, 0). */
+int skyhip_sky_view_lut(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                        const SkyhipAtmosphereRenderBufferData* render,
+                        const float* transmittance_rgba, int32_t transmittance_width,
+                        int32_t transmittance_height, const float* multiscattering_rgba,
+                        int32_t multiscattering_width, int32_t multiscattering_height,
+                        const uint16_t* blue_noise, int32_t blue_noise_size,
+                        float* luminance_rgba, float* sky_transmittance_rgba, int32_t width,
+                        int32_t height);
+
+/* Aerial-perspective LUT (AtmosphereRenderer.glsl:191-243), replaces
+ * aerial_perspective_program_.Dispatch (AtmosphereRenderer.cpp:228-233).
+ * Outputs: RGBA32F 3D width x height x depth, (L, 0) and (T, 0). */
+int skyhip_aerial_perspective_lut(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                                  const SkyhipAtmosphereRenderBufferData* render,
+                                  const float* transmittance_rgba, int32_t transmittance_width,
+                                  int32_t transmittance_height, const float* multiscattering_rgba,
+                                  int32_t multiscattering_width, int32_t multiscattering_height,
+                                  const uint16_t* blue_noise, int32_t blue_noise_size,
+                                  float* luminance_rgba, float* ap_transmittance_rgba,
+                                  int32_t width, int32_t height, int32_t depth);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* SKYHIP_H */

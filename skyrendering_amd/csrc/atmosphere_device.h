@@ -1,0 +1,199 @@
+// Device-side atmosphere model shared by the LUT kernels, the batched LUT
+// builder, the surrogate's label generator and the cloud passes.
+//
+// The arithmetic restates shaders/SkyRendering/Atmosphere.glsl and
+// AtmosphereRenderer.glsl in forms that avoid FP32 cancellation at planetary
+// radii, with results equal to the GLSL expressions in exact arithmetic:
+//   r_d^2 - R_b^2   = d (d + 2 r mu) + (r - R_b)(r + R_b)        (altitude without r_d - R_b)
+//   altitude_d      = (r_d^2 - R_b^2) / (r_d + R_b)
+//   rho_d           = sqrt(r_d^2 - R_b^2)                          (GetTransmittanceTextureUvFromRMu)
+//   R_t^2 - r_d^2   = H^2 - rho_d^2                                (DistanceToTopAtmosphereBoundary)
+//   mu_s at d       = (r mu_s0 + d nu) / r_d                      (dot(sun, normalize(p_d - c)))
+//   (L - L T_i)/s   = L (1 - T_i)/s, 1 - T_i = -expm1(-s dx)      (Atmosphere.glsl:288,290)
+//   texel coord     = x (N - 1)                                    (u N - 0.5 with u = 0.5/N + x(1 - 1/N))
+// Texture fetches are software bilinear with exact weights (GL LINEAR,
+// CLAMP_TO_EDGE), not the hardware sampler, whose fixed-point weights would
+// not meet the 1e-3 relative bar.
+#pragma once
+
+#include "common.h"
+
+namespace skyhip {
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+struct AtmDev {
+    F3 solar;
+    float sun_angular_radius;
+    F3 rayleigh;
+    float inv_hr;
+    F3 mie_s;
+    float inv_hm;
+    F3 mie_ext;  // mie scattering + mie absorption
+    float ozone_center;
+    F3 ozone;
+    float inv_ozone_width;
+    F3 albedo;
+    float g;
+    float ms_mask;
+    float rb, rt;
+    float thickness;  // rt - rb
+    float H2;         // rt^2 - rb^2
+    float H;
+    float inv_H;
+    float t_steps, ms_steps;
+};
+
+__host__ __device__ inline AtmDev make_atm(const SkyhipAtmosphereBufferData& b) {
+    AtmDev a;
+    a.solar = f3(b.solar_illuminance);
+    a.sun_angular_radius = b.sun_angular_radius;
+    a.rayleigh = f3(b.rayleigh_scattering);
+    a.inv_hr = b.inv_rayleigh_exponential_distribution;
+    a.mie_s = f3(b.mie_scattering);
+    a.inv_hm = b.inv_mie_exponential_distribution;
+    a.mie_ext = f3(b.mie_scattering) + f3(b.mie_absorption);
+    a.ozone_center = b.ozone_center_altitude;
+    a.ozone = f3(b.ozone_absorption);
+    a.inv_ozone_width = b.inv_ozone_width;
+    a.albedo = f3(b.ground_albedo);
+    a.g = b.mie_phase_g;
+    a.ms_mask = b.multiscattering_mask;
+    a.rb = b.bottom_radius;
+    a.rt = b.top_radius;
+    a.thickness = b.top_radius - b.bottom_radius;
+    a.H2 = a.thickness * (b.top_radius + b.bottom_radius);
+    a.H = sqrtf(a.H2);
+    a.inv_H = 1.0f / a.H;
+    a.t_steps = b.transmittance_steps;
+    a.ms_steps = b.multiscattering_steps;
+    return a;
+}
+
+// Atmosphere.glsl:119-132 / :156-159 split into densities so the exps are shared.
+struct Densities {
+    float rayleigh, mie, ozone;
+};
+__device__ inline Densities densities(const AtmDev& A, float altitude) {
+    Densities d;
+    d.rayleigh = clamp01(__expf(-altitude * A.inv_hr));
+    d.mie = clamp01(__expf(-altitude * A.inv_hm));
+    d.ozone = fmaxf(0.0f, 1.0f - fabsf(altitude - A.ozone_center) * A.inv_ozone_width);
+    return d;
+}
+__device__ inline F3 extinction(const AtmDev& A, const Densities& d) {
+    return A.rayleigh * d.rayleigh + A.mie_ext * d.mie + A.ozone * d.ozone;
+}
+
+// Atmosphere.glsl:138-154 (non-multiscattering variants).
+__device__ inline float rayleigh_phase(float c) { return 3.0f / (16.0f * kPiF) * (1.0f + c * c); }
+__device__ inline float mie_phase(float g, float c) {
+    float k = 3.0f / (8.0f * kPiF) * (1.0f - g * g) / (2.0f + g * g);
+    float b = 1.0f + g * g - 2.0f * g * c;
+    return k * (1.0f + c * c) / (b * sqrtf(b));
+}
+
+// Bilinear CLAMP_TO_EDGE fetch of RGB from an RGBA32F LUT at texel-space
+// coordinates (0 .. N-1 covers texel centres 0 .. N-1).
+__device__ inline F3 lut_bilinear(const float4* __restrict__ lut, int w, int h, float fx, float fy) {
+    fx = fminf(fmaxf(fx, 0.0f), float(w - 1));
+    fy = fminf(fmaxf(fy, 0.0f), float(h - 1));
+    int x0 = int(fx), y0 = int(fy);
+    int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+    float ax = fx - float(x0), ay = fy - float(y0);
+    float4 a = lut[y0 * w + x0], b = lut[y0 * w + x1], c = lut[y1 * w + x0], d = lut[y1 * w + x1];
+    float bx = 1.0f - ax, by = 1.0f - ay;
+    return {by * (bx * a.x + ax * b.x) + ay * (bx * c.x + ax * d.x),
+            by * (bx * a.y + ax * b.y) + ay * (bx * c.y + ax * d.y),
+            by * (bx * a.z + ax * b.z) + ay * (bx * c.z + ax * d.z)};
+}
+
+// GetSunVisibility (Atmosphere.glsl:105-117) at a point given by its radius
+// r, rho = sqrt(r^2 - R_b^2), altitude and mu_s.
+__device__ inline F3 sun_visibility(const AtmDev& A, const float4* __restrict__ T, int tw, int th, float r,
+                                    float rho, float altitude, float mu_s) {
+    // GetTransmittanceTextureUvFromRMu (:90-103)
+    float disc = r * r * mu_s * mu_s + (A.H2 - rho * rho);
+    float d = fmaxf(-r * mu_s + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+    float d_min = A.thickness - altitude;
+    float d_max = rho + A.H;
+    float x_mu = (d - d_min) / (d_max - d_min);
+    float x_r = rho * A.inv_H;
+    F3 t = lut_bilinear(T, tw, th, x_mu * float(tw - 1), x_r * float(th - 1));
+    float sin_theta_h = A.rb / r;
+    float cos_theta_h = -rho / r;
+    float e = sin_theta_h * A.sun_angular_radius;
+    return t * smoothstepf(-e, e, mu_s - cos_theta_h);
+}
+
+// GetMultiscatteringContribution (Atmosphere.glsl:169-178).
+__device__ inline F3 ms_contribution(const AtmDev& A, const float4* __restrict__ MS, int mw, int mh, float altitude,
+                                     float mu_s) {
+    float x_mu_s = mu_s * 0.5f + 0.5f;
+    float x_r = altitude / A.thickness;
+    return lut_bilinear(MS, mw, mh, x_mu_s * float(mw - 1), x_r * float(mh - 1));
+}
+
+// Geometry of a ray that starts at radius r (altitude alt0 = r - R_b) with
+// zenith cosine mu, sun zenith cosine mu_s0 and sun-view cosine nu.
+struct RayStart {
+    float r, alt0, mu, mu_s0, nu;
+};
+
+// RayIntersectsGround / DistanceTo{Top,Bottom}AtmosphereBoundary (Atmosphere.glsl:57-69)
+__device__ inline bool ray_hits_ground(const AtmDev& A, float r, float alt0, float mu) {
+    return mu < 0.0f && r * r * mu * mu - alt0 * (r + A.rb) >= 0.0f;
+}
+__device__ inline float distance_to_top(const AtmDev& A, float r, float alt0, float mu) {
+    float disc = r * r * mu * mu + (A.H2 - alt0 * (r + A.rb));
+    return fmaxf(-r * mu + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+}
+__device__ inline float distance_to_bottom(const AtmDev& A, float r, float alt0, float mu) {
+    float disc = r * r * mu * mu - alt0 * (r + A.rb);
+    return fmaxf(-r * mu - sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+}
+
+// ComputeScatteredLuminance (Atmosphere.glsl:220-295).  kMS selects the
+// MULTISCATTERING_COMPUTE_PROGRAM variant.  The loop keeps the shader's float
+// counter (i = start_i; i < steps; ++i), so fractional step counts iterate
+// exactly as the GLSL does.
+template <bool kMS>
+__device__ inline void scattered_luminance(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                           const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
+                                           float marching_distance, float steps, float start_i, F3& L, F3& Tr,
+                                           F3& Lf) {
+    const float dx = marching_distance / steps;
+    const float iso = 1.0f / (4.0f * kPiF);
+    const float rp = kMS ? iso : rayleigh_phase(s.nu);
+    const float mp = kMS ? iso : mie_phase(A.g, s.nu);
+    const float two_r_mu = 2.0f * s.r * s.mu;
+    const float rho0_2 = s.alt0 * (s.r + A.rb);
+    const float rb2 = A.rb * A.rb;
+    L = f3(0.0f);
+    Tr = f3(1.0f);
+    Lf = f3(0.0f);
+    for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f) {
+        float d = i * dx;
+        float q = fmaxf(fmaf(d, d + two_r_mu, rho0_2), 0.0f);  // r_d^2 - R_b^2
+        float r_d = sqrtf(q + rb2);
+        float alt = q / (r_d + A.rb);
+        float rho = sqrtf(q);
+        float mu_s = (s.r * s.mu_s0 + d * s.nu) / r_d;
+        Densities dn = densities(A, alt);
+        F3 sr = A.rayleigh * dn.rayleigh;
+        F3 sm = A.mie_s * dn.mie;
+        F3 ext = sr + A.mie_ext * dn.mie + A.ozone * dn.ozone;
+        F3 one_minus_t = {-expm1f(-ext.x * dx), -expm1f(-ext.y * dx), -expm1f(-ext.z * dx)};
+        F3 Li = (sr * rp + sm * mp) * sun_visibility(A, T, tw, th, r_d, rho, alt, mu_s);
+        F3 inv_ext = {1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z};
+        if (!kMS) {
+            Li += A.ms_mask * ms_contribution(A, MS, mw, mh, alt, mu_s) * (sr + sm);
+            Li *= A.solar;
+        }
+        L += Tr * Li * one_minus_t * inv_ext;
+        if (kMS) Lf += Tr * (sr + sm) * one_minus_t * inv_ext;
+        Tr *= F3{__expf(-ext.x * dx), __expf(-ext.y * dx), __expf(-ext.z * dx)};
+    }
+}
+
+}  // namespace skyhip

@@ -1,0 +1,119 @@
+"""C-ABI checks that run without a GPU: the library loads, exports every symbol the header
+declares, converts parameters like the reference's host code, and rejects bad input
+before any device work."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from skyrendering_amd import _abi, presets
+from skyrendering_amd.atmosphere import AtmosphereParameters, AtmosphereRenderParameters, Camera
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.lib()
+    declared = _abi.declared_symbols()
+    assert len(declared) >= 20
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_assign_buffer_data_matches_reference_conversion():
+    p = presets.reference_earth()
+    b = p.buffer_data()
+    f = np.float32
+    # Atmosphere.cpp:49-70: colour * scale, 1 / scale heights, degrees -> radians, top = bottom + thickness.
+    assert b.rayleigh_scattering[2] == f(1.0) * f(0.03310000151395798)
+    assert b.mie_absorption[0] == f(0.0004440000047907233)
+    assert b.inv_rayleigh_exponential_distribution == f(1.0) / f(8.0)
+    assert b.top_radius == f(6360.0) + f(60.0)
+    assert abs(b.sun_angular_radius - math.radians(0.2667)) < 1e-7
+    assert b.transmittance_steps == 40.0 and b.multiscattering_steps == 30.0
+
+
+def test_assign_buffer_data_rejects_degenerate_parameters():
+    p = presets.reference_earth()
+    p.rayleigh_exponential_distribution = 0.0
+    out = _abi.AtmosphereBufferData()
+    rc = _abi.lib().skyhip_atmosphere_assign_buffer_data(C.byref(p.to_c()), C.byref(out))
+    assert rc == _abi.ERR_NON_FINITE
+    p = presets.reference_earth()
+    p.thickness = float("nan")
+    rc = _abi.lib().skyhip_atmosphere_assign_buffer_data(C.byref(p.to_c()), C.byref(out))
+    assert rc == _abi.ERR_NON_FINITE
+    assert b"non-finite" in _abi.lib().skyhip_last_error()
+    assert _abi.lib().skyhip_atmosphere_assign_buffer_data(None, C.byref(out)) == _abi.ERR_INVALID_ARGUMENT
+
+
+def test_render_assign_buffer_data_builds_sun_aligned_basis():
+    cam = presets.bench_camera()
+    r = AtmosphereRenderParameters(sun_direction_theta=60.0, sun_direction_phi=180.0, camera_position=cam.position,
+                                   view_projection=cam.view_projection())
+    b = r.buffer_data(6360.0)
+    sun = np.array(b.sun_direction[:])
+    # Utils.cpp:23-31 FromThetaPhiToDirection(60 deg, 180 deg)
+    np.testing.assert_allclose(sun, [math.cos(math.pi) * math.sin(math.pi / 3), 0.5, 0.0], atol=1e-6)
+    up, right, front = (np.array(getattr(b, k)[:]) for k in ("up_direction", "right_direction", "front_direction"))
+    np.testing.assert_allclose(up, [0, 1, 0], atol=1e-6)
+    assert abs(np.dot(right, sun)) < 1e-6 and abs(np.dot(right, up)) < 1e-6
+    np.testing.assert_allclose(front, np.cross(up, right), atol=1e-6)
+    assert abs(b.camera_earth_center_distance - 6361.0) < 1e-3
+    # Zenith sun: cross(sun, up) = 0 takes the biased fallback (AtmosphereRenderer.cpp:76-81).
+    r.sun_direction_theta = 0.0
+    b0 = r.buffer_data(6360.0)
+    assert abs(np.linalg.norm(np.array(b0.right_direction[:])) - 1) < 1e-6
+
+
+def test_camera_matches_glm_lookat_perspective():
+    cam = Camera(position=(1.0, 2.0, 3.0), yaw=30.0, pitch=10.0, aspect=1.5, fovy=45.0, zNear=0.3, zFar=500.0)
+    vp = cam.view_projection().reshape(4, 4).T  # row-major view of the column-major matrix
+    front, right, up = cam.basis()
+    th, ph = math.radians(80.0), math.radians(30.0)
+    np.testing.assert_allclose(front, [math.cos(ph) * math.sin(th), math.cos(th), math.sin(ph) * math.sin(th)],
+                               atol=1e-6)
+    # A point straight ahead at distance 10 projects to the screen centre.
+    p = np.array([1.0, 2.0, 3.0]) + 10 * front
+    clip = vp @ np.append(p, 1.0)
+    np.testing.assert_allclose(clip[:2] / clip[3], [0, 0], atol=1e-5)
+    # The near plane maps to NDC z = -1, the far plane to +1.
+    for dist, z in [(0.3, -1.0), (500.0, 1.0)]:
+        clip = vp @ np.append(np.array([1.0, 2.0, 3.0]) + dist * front, 1.0)
+        assert abs(clip[2] / clip[3] - z) < 1e-4
+
+
+def test_compute_entry_points_reject_bad_arguments_before_device_work():
+    L = _abi.lib()
+    b = presets.earth_100km().buffer_data()
+    assert L.skyhip_transmittance_lut(None, C.byref(b), None, 256, 64) == _abi.ERR_INVALID_ARGUMENT
+    assert L.skyhip_multiscattering_lut(None, C.byref(b), None, 256, 64, None, 32, 32) == _abi.ERR_INVALID_ARGUMENT
+    assert L.skyhip_malloc(None, 16, C.byref(C.c_void_p())) == _abi.ERR_INVALID_ARGUMENT
+
+
+def test_create_without_device_fails_cleanly():
+    import os
+    h = C.c_void_p()
+    rc = _abi.lib().skyhip_create(0, C.byref(h))
+    if rc == _abi.OK:  # running on a GPU host
+        _abi.lib().skyhip_destroy(h.value)
+        pytest.skip("a HIP device is present")
+    assert rc == _abi.ERR_DEVICE and h.value is None
+
+
+def test_cloud_assign_buffer_data_builds_local_frame():
+    cam = presets.bench_camera()
+    cp = _abi.VolumetricCloudParametersC(bottom_altitude=1.5, thickness=2.5, max_raymarch_distance=33.0,
+                                         max_raymarch_steps=64.0, max_visible_distance=2573.0,
+                                         shadow_froxel_max_distance=30.0, shadow_steps=6.0, shadow_distance=2.0)
+    common, cloud = _abi.CloudCommonBufferData(), _abi.CloudBufferData()
+    model = (C.c_float * 16)()
+    sun = (C.c_float * 3)(0.0, math.sin(math.radians(20)), math.cos(math.radians(20)))
+    rc = _abi.lib().skyhip_cloud_assign_buffer_data(C.byref(cp), C.byref(cam.to_c()), 6360.0, sun, 203.0, 5,
+                                                    C.byref(common), C.byref(cloud), model)
+    assert rc == 0
+    # Local frame: z up at the ground point below the camera; camera at z = altitude.
+    np.testing.assert_allclose(common.uCameraPos[:], [0, 0, 1.0], atol=1e-4)
+    assert abs(np.linalg.norm(common.uSunDirection[:]) - 1) < 1e-6
+    assert abs(common.uSunDirection[2] - math.sin(math.radians(20))) < 1e-6
+    assert common.uTopAltitude == np.float32(1.5) + np.float32(2.5)
+    assert common.uBaseShadingIndex == 1 and common.uFrameID == 5.0
