@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark: volumetric-cloud raymarch throughput on one MI355X, with the Earth LUT-set
+build time beside it.
+
+A step is one raymarch pass over a 1920x1080 frame (64 primary steps, 6 light steps,
+cloud layer 1.5-4 km, extinction 50 / km), inputs resident in HBM.  `value` is
+executed primary sample-steps per second (the per-frame count comes from the kernel's
+counting variant, run once untimed).  Timing: HIP events on the library's stream
+around exactly --steps passes after --warmup passes.
+
+Printed as one JSON line.  Also reported: the Earth LUT set (transmittance 256x64/40,
+multiple scattering 32x32x64/30, sky view 192x108/30, aerial perspective 32^3/50.867)
+in ms per set over 16 sun angles; the roofline of the raymarch kernel; and the CPU
+oracle's FP32 OpenMP path on a bounded sample of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "cloud raymarch Gsample-steps/s + LUT-set build ms; surrogate vs plain @1e-2"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic FP32 operation tally of the raymarch (VolumetricCloudRender.comp with
+# VolumetricCloudDefaultMaterial1.glsl), counting + - * / min max as 1, FMA as 2 and
+# each exp / log2 / sqrt / rcp as 1:
+FLOP_PER_DENSITY_EVAL = 46      # sample position, CalHeight01, distance, uv, LOD, cloud-map fetch, base density
+FLOP_PER_DETAIL_FETCH = 28      # detail uvw, LOD, fetch, erosion arithmetic (only where base density > 0)
+FLOP_PER_CLOUD_STEP = 72        # extinction, shadow-loop bookkeeping, phase (2 HG), env term, accumulation
+FLOP_PER_STEP = 8               # ray position and loop
+FLOP_PER_PIXEL = 260            # unprojection, shell intersection, composite with T-LUT and AP-LUT lookups
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lut-rounds", type=int, default=20)
+    return ap.parse_args()
+
+
+def bench_raymarch(ctx, args):
+    from skyrendering_amd.scenes import CloudScene
+    scene = CloudScene(ctx, 1920, 1080)
+    counts = scene.count_work()
+    for _ in range(args.warmup):
+        scene.run()
+    ctx.synchronize()
+    ctx.record(0)
+    for _ in range(args.steps):
+        scene.run()
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1) / args.steps
+    return scene, counts, ms
+
+
+def bench_lut_set(ctx, rounds):
+    from skyrendering_amd import presets
+    from skyrendering_amd.lutset import LutSet
+    atm, renders, cam, sizes = presets.earth_lut_workload()
+    ls = LutSet(ctx, sizes)
+    for r in renders:  # warm-up, one of each sun angle
+        ls.build(atm, r)
+    ctx.synchronize()
+    ctx.record(2)
+    for _ in range(rounds):
+        for r in renders:
+            ls.build(atm, r)
+    ctx.record(3)
+    ms = ctx.elapsed_ms(2, 3) / (rounds * len(renders))
+    return ms, presets.integrator_steps_per_lut_set(sizes)
+
+
+def cpu_baseline(scene, target_seconds):
+    """The oracle's FP32 path (OpenMP over pixels, all host cores) on whole rows sampled
+    evenly over the frame; Gsample-steps/s = executed steps of the sample / wall time."""
+    import oracle
+    from skyrendering_amd import _abi  # noqa: F401
+    a, r = scene.atmosphere, scene.renderer
+    apl, apt = r.aerial_perspective()
+    cm, det, disp = scene.textures.download()
+    inputs = oracle.HostCloudInputs(scene.mat_block, cloud_map=cm, detail=det, displacement=disp,
+                                    T=a.transmittance(), ap_l=apl, ap_t=apt, blue_noise=scene.blue_noise_host,
+                                    image_size=(scene.width, scene.height))
+    w, h = scene.width, scene.height
+    threads = oracle.max_threads()
+
+    def run(rows):
+        px = (rows[:, None] * w + np.arange(w)[None, :]).reshape(-1)
+        t0 = time.perf_counter()
+        _, _, steps = oracle.cloud_raymarch(a.buffer, scene.common, scene.cloud_block, inputs, w, px, "f32")
+        return time.perf_counter() - t0, int(steps.sum()), len(px)
+
+    probe_rows = np.linspace(0, h - 1, 6).astype(np.int64)
+    t, s, n = run(probe_rows)
+    nrows = int(np.clip(target_seconds / max(t, 1e-3) * len(probe_rows), 6, h))
+    rows = np.linspace(0, h - 1, nrows).astype(np.int64)
+    t, s, n = run(rows)
+    return {"value": s / t / 1e9, "unit": "Gsample-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nrows} of {h} rows evenly spaced ({n} pixels, {s} executed steps) of the same frame, "
+                      f"oracle FP32 OpenMP, {t:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the raymarch shards by "
+              f"image rows and the path tracer by frames (see DESIGN.md), but no multi-GPU figure is produced.",
+              file=sys.stderr)
+        sys.exit(2)
+    from skyrendering_amd import Context
+    ctx = Context(0)
+    dev = ctx.device_info()
+    scene, counts, ms = bench_raymarch(ctx, args)
+    steps = counts["steps"]
+    value = steps / (ms * 1e-3) / 1e9
+    flops = (counts["evals"] * FLOP_PER_DENSITY_EVAL + counts["detail"] * FLOP_PER_DETAIL_FETCH +
+             counts["cloud_steps"] * FLOP_PER_CLOUD_STEP + steps * FLOP_PER_STEP +
+             scene.width * scene.height * FLOP_PER_PIXEL)
+    achieved = flops / (ms * 1e-3) / 1e12
+    lut_ms, lut_steps = bench_lut_set(ctx, args.lut_rounds)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gsample-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded weather map, noise volumes, blue noise)",
+        "config": {"workload": "volumetric cloud raymarch 1920x1080, one ray per pixel, 64 primary + 6 light "
+                               "steps, layer 1.5-4 km, extinction 50/km, Material1 (512^2 RG8 uniform weather "
+                               "map, 128^3 R8 Perlin-Worley detail), camera 1 km, sun elevation 20 deg",
+                   "width": 1920, "height": 1080, "executed_steps_per_frame": steps,
+                   "density_evals_per_frame": counts["evals"], "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak; the kernel is VALU/L2-latency bound",
+                     "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "cloud_raymarch_kernel<1,false>",
+                     "algorithmic_flop_per_launch": flops},
+        "lut_set_ms": round(lut_ms, 4),
+        "lut_set_integrator_steps": lut_steps,
+        "device": dev,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

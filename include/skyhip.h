@@ -429,6 +429,78 @@ int skyhip_aerial_perspective_lut(SkyhipContext* ctx, const SkyhipAtmosphereBuff
                                   float* luminance_rgba, float* ap_transmittance_rgba,
                                   int32_t width, int32_t height, int32_t depth);
 
+
+
+/* One frame's atmosphere LUT set, in the reference's order: Atmosphere::UpdateLuts
+ * (transmittance then multiple scattering, Atmosphere.cpp:101-124) followed by the LUT
+ * passes of AtmosphereRenderer::Render (sky view and aerial perspective,
+ * AtmosphereRenderer.cpp:221-233), which run concurrently on a forked stream and are
+ * joined back before the call's work completes.  Dithering off (the reference default). */
+int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                              const SkyhipAtmosphereRenderBufferData* render, float* transmittance_rgba,
+                              int32_t transmittance_width, int32_t transmittance_height,
+                              float* multiscattering_rgba, int32_t multiscattering_width,
+                              int32_t multiscattering_height, float* sky_view_luminance_rgba,
+                              float* sky_view_transmittance_rgba, int32_t sky_view_width, int32_t sky_view_height,
+                              float* ap_luminance_rgba, float* ap_transmittance_rgba, int32_t ap_width,
+                              int32_t ap_height, int32_t ap_depth);
+
+/* ------------------------------------------------------------------------- */
+/* Cloud textures: noise generators and mip chains                           */
+/* ------------------------------------------------------------------------- */
+
+/* GetMipmapLevels, src/Base/include/ImageLoader.h:8-14: floor(log2(max dim)) + 1. */
+int skyhip_mip_levels(int32_t width, int32_t height, int32_t depth);
+/* Bytes of a packed mip chain (levels back to back, level n = max(1, dim >> n)). */
+size_t skyhip_mip_chain_bytes(int32_t width, int32_t height, int32_t depth, int32_t channels, int32_t levels);
+
+/* NoiseGen.comp CLOUD_MAP_GEN (shaders/SkyRendering/NoiseGen.comp:52-67), the
+ * cloud_map_ DynamicTexture of VolumetricCloudDefaultMaterial.cpp:31-43: level 0
+ * of a size^2 RG8 texture (R = Perlin-FBM coverage, G = Worley-FBM type). */
+int skyhip_noise_cloud_map(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* density,
+                           const SkyhipNoiseCreateInfo* height, uint8_t* out_rg8, int32_t size);
+/* NoiseGen.comp DETAIL_MAP_GEN (:69-86), detail_ (VolumetricCloudDefaultMaterial.cpp:45-57):
+ * level 0 of a size^3 R8 Perlin-Worley volume. */
+int skyhip_noise_detail(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* perlin,
+                        const SkyhipNoiseCreateInfo* worley, uint8_t* out_r8, int32_t size);
+/* NoiseGen.comp DISPLACEMENT_GEN (:88-101), displacement_ (:59-71): level 0 of a
+ * size^2 RGBA8 texture. */
+int skyhip_noise_displacement(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* perlin, uint8_t* out_rgba8,
+                              int32_t size);
+/* glGenerateTextureMipmap (DynamicTexture::Generate, VolumetricCloudDefaultMaterial.h:39-49)
+ * for a packed UNORM8 chain: fills levels 1..levels-1 from level 0 with a
+ * 2x2 (dims = 2) or 2x2x2 (dims = 3) box filter, rounded half up. */
+int skyhip_generate_mipmaps(SkyhipContext* ctx, uint8_t* texture, int32_t dims, int32_t channels, int32_t width,
+                            int32_t height, int32_t depth, int32_t levels);
+
+/* ------------------------------------------------------------------------- */
+/* Volumetric-cloud raymarch                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* The render pass of VolumetricCloud::Render (src/SkyRendering/VolumetricCloud.cpp:362-386,
+ * shaders/SkyRendering/VolumetricCloudRender.comp:139-210) with the material's
+ * SampleSigmaT compiled in.  One ray per output pixel.
+ *   depth:          R32F scene depth (GL window z in [0, 1]) at the sample grid, or NULL
+ *                   for the far plane everywhere;
+ *   shading_index:  NULL for one ray per output pixel (sample grid = output grid), or the
+ *                   RG32F index_linear_depth texture of the reference's checkerboard
+ *                   pipeline (sample grid = 2x the output grid, VolumetricCloudRender.comp:140-145);
+ *   blue_noise:     R16 UNORM tile of blue_noise_size^2 (power of two) read at pixel & (size-1);
+ *   out_rgba16f:    RGBA16F (luminance, transmittance); out_distance: R32F mean distance;
+ *   work_counters:  NULL, or device uint64[4] to which a counting kernel variant adds the
+ *                   executed primary march steps, density evaluations (primary + light march),
+ *                   detail-texture fetches and in-cloud steps (sigma >= 1e-5); leave NULL when timing.
+ * The cloud shadow froxel visibility (VolumetricCloudShadowInterface.glsl:10-13) is 1:
+ * the cloud shadow passes are not part of this library. */
+int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                          const SkyhipCloudCommonBufferData* common, const SkyhipCloudBufferData* cloud,
+                          const SkyhipCloudMaterial* material, const float* transmittance_rgba,
+                          int32_t transmittance_width, int32_t transmittance_height,
+                          const float* ap_luminance_rgba, const float* ap_transmittance_rgba, int32_t ap_width,
+                          int32_t ap_height, int32_t ap_depth, const uint16_t* blue_noise, int32_t blue_noise_size,
+                          const float* depth, const float* shading_index, void* out_rgba16f, float* out_distance,
+                          int32_t width, int32_t height, unsigned long long* work_counters);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -127,3 +127,101 @@ def max_threads():
 
 def set_threads(n):
     lib().oracle_set_threads(int(n))
+
+
+# ---------------------------------------------------------------------------
+# Noise and cloud raymarch
+# ---------------------------------------------------------------------------
+class CloudImagesC(C.Structure):
+    _fields_ = [("transmittance", vp), ("tw", i32), ("th", i32), ("ap_luminance", vp), ("ap_transmittance", vp),
+                ("aw", i32), ("ah", i32), ("ad", i32), ("blue_noise", vp), ("bn_size", i32), ("depth", vp),
+                ("index", vp), ("image_w", i32), ("image_h", i32)]
+
+
+_NI = P(_abi.NoiseCreateInfo)
+_PROTO.update({
+    "oracle_cloud_map_texels": (None, [_NI, _NI, i32, vp, C.c_long, vp, i32]),
+    "oracle_detail_texels": (None, [_NI, _NI, i32, vp, C.c_long, vp, i32]),
+    "oracle_displacement_texels": (None, [_NI, i32, vp, C.c_long, vp, i32]),
+    "oracle_perlin_noise": (f64, [f64, f64, f64, C.c_uint, C.c_uint]),
+    "oracle_worley_noise": (f64, [f64, f64, f64, C.c_uint, C.c_uint]),
+    "oracle_wang_hash": (C.c_uint, [C.c_uint]),
+    "oracle_pcg_hash": (C.c_uint, [C.c_uint]),
+    "oracle_generate_mipmaps": (None, [vp, i32, i32, i32, i32, i32, i32]),
+    "oracle_cloud_raymarch_f64": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
+                                         P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32, vp,
+                                         C.c_long, vp, vp, vp]),
+    "oracle_cloud_raymarch_f32": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
+                                         P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32, vp,
+                                         C.c_long, vp, vp, vp]),
+    "oracle_cloud_sigma_t_f64": (f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudBufferData),
+                                       P(_abi.CloudMaterial), f64, f64, f64]),
+    "oracle_cloud_shadow_f64": (f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudBufferData),
+                                      P(_abi.CloudMaterial), f64, f64, f64]),
+})
+
+
+def cloud_map_texels(density, height, size, texels, precision="f64"):
+    t = np.ascontiguousarray(texels, dtype=np.int32)
+    out = np.empty((len(t), 2), np.uint8)
+    lib().oracle_cloud_map_texels(C.byref(density), C.byref(height), size, _ptr(t), len(t), _ptr(out),
+                                  1 if precision == "f64" else 0)
+    return out
+
+
+def detail_texels(perlin, worley, size, texels, precision="f64"):
+    t = np.ascontiguousarray(texels, dtype=np.int32)
+    out = np.empty(len(t), np.uint8)
+    lib().oracle_detail_texels(C.byref(perlin), C.byref(worley), size, _ptr(t), len(t), _ptr(out),
+                               1 if precision == "f64" else 0)
+    return out
+
+
+def displacement_texels(perlin, size, texels, precision="f64"):
+    t = np.ascontiguousarray(texels, dtype=np.int32)
+    out = np.empty((len(t), 4), np.uint8)
+    lib().oracle_displacement_texels(C.byref(perlin), size, _ptr(t), len(t), _ptr(out),
+                                     1 if precision == "f64" else 0)
+    return out
+
+
+def generate_mipmaps(chain: np.ndarray, dims, channels, w, h, d, levels):
+    chain = np.ascontiguousarray(chain, dtype=np.uint8)
+    lib().oracle_generate_mipmaps(_ptr(chain), dims, channels, w, h, d, levels)
+    return chain
+
+
+class HostCloudInputs:
+    """Host copies of every raymarch input, kept alive for the oracle call."""
+
+    def __init__(self, material, cloud_map=None, detail=None, displacement=None, voxel=None, T=None, ap_l=None,
+                 ap_t=None, blue_noise=None, depth=None, index=None, image_size=None):
+        self.keep = [x for x in (cloud_map, detail, displacement, voxel, T, ap_l, ap_t, blue_noise, depth, index)
+                     if x is not None]
+        m = _abi.CloudMaterial()
+        C.memmove(C.byref(m), C.byref(material), C.sizeof(m))
+        m.cloud_map = _ptr(cloud_map)
+        m.detail = _ptr(detail)
+        m.displacement = _ptr(displacement)
+        m.voxel_texture = _ptr(voxel)
+        self.material = m
+        im = CloudImagesC()
+        im.transmittance, im.tw, im.th = _ptr(T), T.shape[1], T.shape[0]
+        im.ap_luminance, im.ap_transmittance = _ptr(ap_l), _ptr(ap_t)
+        im.ad, im.ah, im.aw = ap_l.shape[0], ap_l.shape[1], ap_l.shape[2]
+        im.blue_noise, im.bn_size = _ptr(blue_noise), blue_noise.shape[0]
+        im.depth, im.index = _ptr(depth), _ptr(index)
+        im.image_w, im.image_h = image_size
+        self.images = im
+
+
+def cloud_raymarch(atm, common, cloud, inputs: HostCloudInputs, width, pixels, precision="f64"):
+    px = np.ascontiguousarray(pixels, dtype=np.int32)
+    dt = _dt(precision)
+    rgba = np.empty((len(px), 4), dt)
+    dist = np.empty(len(px), dt)
+    steps = np.empty(len(px), np.int32)
+    getattr(lib(), f"oracle_cloud_raymarch_{precision}")(
+        C.byref(atm), C.byref(common), C.byref(cloud), C.byref(inputs.material), C.byref(inputs.images), width,
+        _ptr(px), len(px), _ptr(rgba), _ptr(dist), _ptr(steps))
+    return rgba, dist, steps

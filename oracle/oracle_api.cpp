@@ -7,7 +7,8 @@
 
 #include <vector>
 
-#include "oracle_atmosphere.h"
+#include "oracle_cloud.h"
+#include "oracle_noise.h"
 
 using namespace oracle;
 
@@ -67,6 +68,23 @@ void aerial_perspective_lut(const SkyhipAtmosphereBufferData* b, const SkyhipAtm
         AerialPerspectiveTexel(A, R, T, tw, th, MS, mw, mh, bn, bn_size, ix, iy, iz, w, h, d, L, Tr);
         out_l[t * 4 + 0] = O(L.x), out_l[t * 4 + 1] = O(L.y), out_l[t * 4 + 2] = O(L.z), out_l[t * 4 + 3] = O(0);
         out_t[t * 4 + 0] = O(Tr.x), out_t[t * 4 + 1] = O(Tr.y), out_t[t * 4 + 2] = O(Tr.z), out_t[t * 4 + 3] = O(0);
+    }
+}
+
+template <class S, class O>
+void cloud_raymarch(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
+                           const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
+                           const CloudImages* images, int width, const int32_t* pixels, long count, O* out_rgba,
+                           O* out_dist, int32_t* out_steps) {
+    Atmosphere<S> A(*atm);
+    CloudCtx<S> Cc(common, cloud, material);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < count; ++i) {
+        int p = pixels[i];
+        CloudPixelOut<S> o = CloudRaymarchPixel<S>(A, Cc, *images, p % width, p / width);
+        for (int k = 0; k < 4; ++k) out_rgba[i * 4 + k] = O(o.rgba[k]);
+        out_dist[i] = O(o.distance);
+        if (out_steps) out_steps[i] = o.steps;
     }
 }
 
@@ -157,6 +175,97 @@ void oracle_scattered_luminance_f64(const SkyhipAtmosphereBufferData* b, const f
                                                steps, tr, nullptr);
     out_l[0] = L.x, out_l[1] = L.y, out_l[2] = L.z;
     out_t[0] = tr.x, out_t[1] = tr.y, out_t[2] = tr.z;
+}
+
+
+// ---------------------------------------------------------------------------
+// Noise textures (NoiseGen.comp) -- level 0 over a list of texel indices, so
+// large volumes can be checked on a sample.
+// ---------------------------------------------------------------------------
+void oracle_cloud_map_texels(const SkyhipNoiseCreateInfo* density, const SkyhipNoiseCreateInfo* height, int size,
+                             const int32_t* texels, long count, uint8_t* out_rg, int f64) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < count; ++i) {
+        int t = texels[i];
+        if (f64)
+            CloudMapTexel<double>(t % size, t / size, size, *density, *height, out_rg + 2 * i);
+        else
+            CloudMapTexel<float>(t % size, t / size, size, *density, *height, out_rg + 2 * i);
+    }
+}
+void oracle_detail_texels(const SkyhipNoiseCreateInfo* perlin, const SkyhipNoiseCreateInfo* worley, int size,
+                          const int32_t* texels, long count, uint8_t* out, int f64) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < count; ++i) {
+        int t = texels[i];
+        int x = t % size, y = (t / size) % size, z = t / (size * size);
+        out[i] = f64 ? DetailTexel<double>(x, y, z, size, *perlin, *worley)
+                     : DetailTexel<float>(x, y, z, size, *perlin, *worley);
+    }
+}
+void oracle_displacement_texels(const SkyhipNoiseCreateInfo* perlin, int size, const int32_t* texels, long count,
+                                uint8_t* out_rgba, int f64) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < count; ++i) {
+        int t = texels[i];
+        if (f64)
+            DisplacementTexel<double>(t % size, t / size, size, *perlin, out_rgba + 4 * i);
+        else
+            DisplacementTexel<float>(t % size, t / size, size, *perlin, out_rgba + 4 * i);
+    }
+}
+double oracle_perlin_noise(double x, double y, double z, unsigned freq, unsigned seed) {
+    return PerlinNoise<double>({x, y, z}, freq, seed);
+}
+double oracle_worley_noise(double x, double y, double z, unsigned freq, unsigned seed) {
+    return WorleyNoise<double>({x, y, z}, freq, seed);
+}
+unsigned oracle_wang_hash(unsigned x) { return WangHash(x); }
+unsigned oracle_pcg_hash(unsigned x) { return PCGHash(x); }
+
+// Fills levels 1.. of a packed UNORM8 mip chain from level 0.
+void oracle_generate_mipmaps(uint8_t* tex, int dims, int channels, int w, int h, int d, int levels) {
+    if (dims == 2) d = 1;
+    int sd[3] = {w, h, d};
+    size_t off = 0;
+    for (int l = 1; l < levels; ++l) {
+        int dd[3] = {std::max(1, sd[0] / 2), std::max(1, sd[1] / 2), dims == 3 ? std::max(1, sd[2] / 2) : 1};
+        size_t src_bytes = size_t(sd[0]) * sd[1] * sd[2] * channels;
+        downsample_u8(tex + off, sd, tex + off + src_bytes, dd, dims, channels);
+        off += src_bytes;
+        sd[0] = dd[0], sd[1] = dd[1], sd[2] = dd[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cloud raymarch (VolumetricCloudRender.comp) over a list of output pixels.
+// out_rgba: 4 per pixel, out_dist: 1 per pixel, out_steps: executed primary
+// steps per pixel (may be NULL).
+// ---------------------------------------------------------------------------
+void oracle_cloud_raymarch_f64(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
+                               const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
+                               const CloudImages* images, int width, const int32_t* pixels, long count,
+                               double* out_rgba, double* out_dist, int32_t* out_steps) {
+    cloud_raymarch<double>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps);
+}
+void oracle_cloud_raymarch_f32(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
+                               const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
+                               const CloudImages* images, int width, const int32_t* pixels, long count,
+                               float* out_rgba, float* out_dist, int32_t* out_steps) {
+    cloud_raymarch<float>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps);
+}
+
+// One density evaluation (for material tests).
+double oracle_cloud_sigma_t_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudBufferData* cloud,
+                                const SkyhipCloudMaterial* material, double x, double y, double z) {
+    CloudCtx<double> Cc(common, cloud, material);
+    V3<double> p = {x, y, z};
+    return Cc.SampleSigmaT(p, Cc.CalHeight01(p));
+}
+double oracle_cloud_shadow_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudBufferData* cloud,
+                               const SkyhipCloudMaterial* material, double x, double y, double z) {
+    CloudCtx<double> Cc(common, cloud, material);
+    return Cc.SampleShadow({x, y, z});
 }
 
 }  // extern "C"

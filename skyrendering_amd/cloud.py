@@ -1,0 +1,288 @@
+"""Host mirror of the reference's volumetric-cloud classes over the skyhip C ABI.
+
+Names, fields and defaults follow src/SkyRendering/VolumetricCloud.h:16-56 and
+VolumetricCloudDefaultMaterial.h:57-207 (the material plugin of
+IVolumetricCloudMaterial.h:9-30 becomes a tagged parameter set: SkyhipCloudMaterial).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+from . import _abi
+from .atmosphere import Camera
+from .device import Context, DeviceBuffer
+
+u8p = C.POINTER(C.c_uint8)
+_abi.register("skyhip_mip_levels", C.c_int, [C.c_int32, C.c_int32, C.c_int32])
+_abi.register("skyhip_mip_chain_bytes", C.c_size_t, [C.c_int32] * 5)
+_abi.register("skyhip_noise_cloud_map", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseCreateInfo),
+                                                   C.POINTER(_abi.NoiseCreateInfo), C.c_void_p, C.c_int32])
+_abi.register("skyhip_noise_detail", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseCreateInfo),
+                                                C.POINTER(_abi.NoiseCreateInfo), C.c_void_p, C.c_int32])
+_abi.register("skyhip_noise_displacement", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseCreateInfo), C.c_void_p,
+                                                      C.c_int32])
+_abi.register("skyhip_generate_mipmaps", C.c_int, [C.c_void_p, C.c_void_p] + [C.c_int32] * 6)
+_abi.register("skyhip_cloud_raymarch", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.AtmosphereBufferData), C.POINTER(_abi.CloudCommonBufferData),
+    C.POINTER(_abi.CloudBufferData), C.POINTER(_abi.CloudMaterial), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+    C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+    C.c_void_p, C.c_int32, C.c_int32, C.c_void_p])
+
+
+def mip_levels(w, h=1, d=1) -> int:
+    """GetMipmapLevels, src/Base/include/ImageLoader.h:8-14."""
+    return int(_abi.lib().skyhip_mip_levels(w, h, d))
+
+
+def mip_chain_bytes(w, h, d, channels, levels) -> int:
+    return int(_abi.lib().skyhip_mip_chain_bytes(w, h, d, channels, levels))
+
+
+@dataclass
+class NoiseCreateInfo:
+    """NoiseCreateInfo, VolumetricCloudDefaultMaterial.h:57-62."""
+    seed: int = 0
+    base_frequency: int = 1
+    remap_min: float = 0.0
+    remap_max: float = 1.0
+
+    def to_c(self):
+        return _abi.NoiseCreateInfo(self.seed & 0xFFFFFFFF, self.base_frequency, self.remap_min, self.remap_max)
+
+
+@dataclass
+class DefaultMaterialCommon:
+    """VolumetricCloudDefaultMaterialCommon, VolumetricCloudDefaultMaterial.h:86-140 and .cpp:30-76."""
+    cloud_map_density: NoiseCreateInfo = field(default_factory=lambda: NoiseCreateInfo(0, 3, 0.35, 0.75))
+    cloud_map_height: NoiseCreateInfo = field(default_factory=lambda: NoiseCreateInfo(0, 5, 0.8, 0.4))
+    detail_perlin: NoiseCreateInfo = field(default_factory=lambda: NoiseCreateInfo(0, 7, 0.23, 1.0))
+    detail_worley: NoiseCreateInfo = field(default_factory=lambda: NoiseCreateInfo(0, 11, 1.0, 0.0))
+    displacement_perlin: NoiseCreateInfo = field(default_factory=lambda: NoiseCreateInfo(0, 6, 0.25, 0.75))
+    cloud_map_repeat_size: float = 18.99
+    detail_repeat_size: float = 5.33
+    displacement_repeat_size: float = 3.51
+    lod_bias: float = 2.75
+    density: float = 15.0
+    wind_speed: float = 0.05
+    detail_wind_magnify: float = 1.0
+    cloud_map_size: int = 512
+    detail_size: int = 128
+    displacement_size: int = 128
+
+    def load_json(self, d: dict):
+        """The `materail_common_` object of bin/config*.json (flattened FIELD_DECLARE keys)."""
+        noise = {"cloud_map_.buffer.uDensity": "cloud_map_density", "cloud_map_.buffer.uHeight": "cloud_map_height",
+                 "detail_.buffer.uPerlin": "detail_perlin", "detail_.buffer.uWorley": "detail_worley",
+                 "displacement_.buffer.uPerlin": "displacement_perlin"}
+        for prefix, attr in noise.items():
+            n = getattr(self, attr)
+            for k in ("seed", "base_frequency", "remap_min", "remap_max"):
+                if f"{prefix}.{k}" in d:
+                    setattr(n, k, type(getattr(n, k))(d[f"{prefix}.{k}"]))
+        plain = {"cloud_map_.texture.repeat_size": "cloud_map_repeat_size",
+                 "detail_.texture.repeat_size": "detail_repeat_size",
+                 "displacement_.texture.repeat_size": "displacement_repeat_size", "density_": "density",
+                 "lod_bias_": "lod_bias", "wind_speed_": "wind_speed", "detail_wind_magnify_": "detail_wind_magnify"}
+        for k, attr in plain.items():
+            if k in d:
+                setattr(self, attr, float(d[k]))
+        return self
+
+    def common_block(self, viewport, camera: Camera, offset=(0.0, 0.0), detail_offset=(0.0, 0.0)):
+        """VolumetricCloudDefaultMaterialCommon::Update (VolumetricCloudDefaultMaterial.cpp:78-102) for a
+        static frame (additional wind delta 0)."""
+        blk = _abi.CloudMaterialCommonBufferData()
+        blk.uLodBias = self.lod_bias
+        blk.uDensity = self.density
+        L = _abi.lib()
+        vw, vh = viewport
+        for info, repeat, width, off in [
+                (blk.uCloudMapSampleInfo, self.cloud_map_repeat_size, self.cloud_map_size, offset),
+                (blk.uDetailSampleInfo, self.detail_repeat_size, self.detail_size,
+                 (offset[0] + detail_offset[0], offset[1] + detail_offset[1])),
+                (blk.uDisplacementSampleInfo, self.displacement_repeat_size, self.displacement_size, offset)]:
+            _abi.check(L.skyhip_cloud_sample_info(repeat, width, camera.fovy, vw, vh, off[0], off[1], C.byref(info)))
+        return blk
+
+
+@dataclass
+class VolumetricCloudDefaultMaterial1(DefaultMaterialCommon):
+    """VolumetricCloudDefaultMaterial1, VolumetricCloudDefaultMaterial.h:171-207, .cpp:224-257."""
+    detail_base: float = 0.67
+    detail_scale: float = 1.86
+    base_density_threshold: float = 0.4
+    base_height_hardness: float = 6.0
+    base_edge_hardness: float = 6.0
+    height_cut: float = 0.9
+    edge_cut: float = 0.8
+    kind = _abi.MATERIAL_DEFAULT1
+
+    @classmethod
+    def from_json(cls, data: dict):
+        m = cls()
+        m.load_json(data.get("materail_common_", {}))
+        for k in ("detail_base", "detail_scale", "base_density_threshold", "base_height_hardness",
+                  "base_edge_hardness", "height_cut", "edge_cut"):
+            if k + "_" in data:
+                setattr(m, k, float(data[k + "_"]))
+        return m
+
+    def material1_block(self):
+        b = _abi.CloudMaterial1BufferData()
+        b.uBaseDensityThreshold = self.base_density_threshold
+        b.uBaseHeightHardness = self.base_height_hardness
+        b.uBaseEdgeHardness = self.base_edge_hardness
+        b.uDetailBase = self.detail_base
+        b.uDetailScale = self.detail_scale
+        b.uHeightCut = np.float32(1.0) - np.float32(self.height_cut)  # .cpp:253
+        b.uEdgeCur = self.edge_cut
+        return b
+
+
+@dataclass
+class VolumetricCloudMinimalMaterial:
+    """VolumetricCloudMinimalMaterial (constant extinction), VolumetricCloudMinimalMaterial.h."""
+    density: float = 0.5
+    kind = _abi.MATERIAL_MINIMAL
+
+
+class CloudTextures:
+    """Device mip chains of the default material's textures (cloud map RG8 2D, detail R8 3D,
+    displacement RGBA8 2D), generated by the noise kernels or uploaded from the host."""
+
+    def __init__(self, ctx: Context, cloud_map_size=512, detail_size=128, displacement_size=128):
+        self.ctx = ctx
+        self.cloud_map_size, self.detail_size, self.displacement_size = cloud_map_size, detail_size, displacement_size
+        self.cloud_map_levels = mip_levels(cloud_map_size, cloud_map_size)
+        self.detail_levels = mip_levels(detail_size, detail_size, detail_size)
+        self.displacement_levels = mip_levels(displacement_size, displacement_size)
+        self.cloud_map = ctx.alloc(mip_chain_bytes(cloud_map_size, cloud_map_size, 1, 2, self.cloud_map_levels))
+        self.detail = ctx.alloc(mip_chain_bytes(detail_size, detail_size, detail_size, 1, self.detail_levels))
+        self.displacement = ctx.alloc(
+            mip_chain_bytes(displacement_size, displacement_size, 1, 4, self.displacement_levels))
+
+    def generate(self, m: DefaultMaterialCommon):
+        """DynamicTexture::Generate for all three textures (VolumetricCloudDefaultMaterial.h:39-49)."""
+        L = _abi.lib()
+        h = self.ctx.handle
+        _abi.check(L.skyhip_noise_cloud_map(h, C.byref(m.cloud_map_density.to_c()), C.byref(m.cloud_map_height.to_c()),
+                                            self.cloud_map.ptr, self.cloud_map_size))
+        _abi.check(L.skyhip_noise_detail(h, C.byref(m.detail_perlin.to_c()), C.byref(m.detail_worley.to_c()),
+                                         self.detail.ptr, self.detail_size))
+        _abi.check(L.skyhip_noise_displacement(h, C.byref(m.displacement_perlin.to_c()), self.displacement.ptr,
+                                               self.displacement_size))
+        self.build_mips()
+        return self
+
+    def upload_level0(self, cloud_map=None, detail=None, displacement=None):
+        if cloud_map is not None:
+            self.cloud_map.upload(np.ascontiguousarray(cloud_map, dtype=np.uint8))
+        if detail is not None:
+            self.detail.upload(np.ascontiguousarray(detail, dtype=np.uint8))
+        if displacement is not None:
+            self.displacement.upload(np.ascontiguousarray(displacement, dtype=np.uint8))
+        self.build_mips()
+        return self
+
+    def build_mips(self):
+        L = _abi.lib()
+        h = self.ctx.handle
+        n = self.cloud_map_size
+        _abi.check(L.skyhip_generate_mipmaps(h, self.cloud_map.ptr, 2, 2, n, n, 1, self.cloud_map_levels))
+        n = self.detail_size
+        _abi.check(L.skyhip_generate_mipmaps(h, self.detail.ptr, 3, 1, n, n, n, self.detail_levels))
+        n = self.displacement_size
+        _abi.check(L.skyhip_generate_mipmaps(h, self.displacement.ptr, 2, 4, n, n, 1, self.displacement_levels))
+
+    def download(self):
+        return (self.cloud_map.download(np.uint8, (self.cloud_map.nbytes,)),
+                self.detail.download(np.uint8, (self.detail.nbytes,)),
+                self.displacement.download(np.uint8, (self.displacement.nbytes,)))
+
+    def fill_material(self, mat: _abi.CloudMaterial):
+        mat.cloud_map, mat.cloud_map_size, mat.cloud_map_levels = self.cloud_map.ptr, self.cloud_map_size, \
+            self.cloud_map_levels
+        mat.detail, mat.detail_size, mat.detail_levels = self.detail.ptr, self.detail_size, self.detail_levels
+        mat.displacement, mat.displacement_size, mat.displacement_levels = self.displacement.ptr, \
+            self.displacement_size, self.displacement_levels
+
+
+@dataclass
+class VolumetricCloud:
+    """VolumetricCloud's serialised fields (VolumetricCloud.h:20-35) and its Update/Render
+    (VolumetricCloud.cpp:168-280, 327-386) for the raymarch pass."""
+    bottom_altitude: float = 2.0
+    thickness: float = 2.0
+    max_raymarch_distance: float = 30.0
+    max_raymarch_steps: float = 128.0
+    max_visible_distance: float = 120.0
+    env_color: tuple = (1.0, 1.0, 1.0)
+    env_color_scale: float = 0.1
+    sun_illuminance_scale: float = 1.0
+    shadow_steps: float = 5.0
+    shadow_distance: float = 2.0
+    shadow_map_max_distance: float = 20.0
+    shadow_froxel_max_distance: float = 20.0
+    sun_multiscattering_sigma_scale: float = 0.3
+    env_multiscattering_sigma_scale: float = 0.5
+    env_bottom_visibility: float = 0.4
+    env_sun_height_curve_exp: float = 1.0
+
+    @classmethod
+    def from_json(cls, d: dict):
+        v = cls()
+        for f in fields(cls):
+            k = f.name + "_" if (f.name + "_") in d else f.name
+            if k in d:
+                setattr(v, f.name, tuple(d[k]) if isinstance(d[k], list) else float(d[k]))
+        return v
+
+    def to_c(self):
+        c = _abi.VolumetricCloudParametersC()
+        for f in fields(self):
+            val = getattr(self, f.name)
+            if isinstance(val, tuple):
+                for i, x in enumerate(val):
+                    c.env_color[i] = x
+            else:
+                setattr(c, f.name, val)
+        return c
+
+    def update(self, camera: Camera, bottom_radius: float, sun_direction_world, ap_max_distance: float,
+               frame_id: int = 0):
+        """Returns (common block, cloud block, model matrix) as VolumetricCloud::Update fills them."""
+        common, cloud = _abi.CloudCommonBufferData(), _abi.CloudBufferData()
+        model = (C.c_float * 16)()
+        sun = (C.c_float * 3)(*[float(x) for x in sun_direction_world])
+        _abi.check(_abi.lib().skyhip_cloud_assign_buffer_data(C.byref(self.to_c()), C.byref(camera.to_c()),
+                                                              float(bottom_radius), sun, float(ap_max_distance),
+                                                              int(frame_id), C.byref(common), C.byref(cloud), model))
+        return common, cloud, np.array(model[:], dtype=np.float32)
+
+
+def make_material(material, textures: CloudTextures | None, viewport, camera: Camera, offset=(0.0, 0.0)):
+    m = _abi.CloudMaterial()
+    m.kind = material.kind
+    if material.kind == _abi.MATERIAL_MINIMAL:
+        m.minimal.uDensity = material.density
+        return m
+    m.common = material.common_block(viewport, camera, offset)
+    m.material1 = material.material1_block()
+    textures.fill_material(m)
+    return m
+
+
+def cloud_raymarch(ctx: Context, atmosphere_block, common, cloud, material: _abi.CloudMaterial,
+                   transmittance: DeviceBuffer, t_size, ap_l: DeviceBuffer, ap_t: DeviceBuffer, ap_size,
+                   blue_noise: DeviceBuffer, blue_noise_size, out_rgba16f: DeviceBuffer, out_distance: DeviceBuffer,
+                   width, height, depth: DeviceBuffer | None = None, shading_index: DeviceBuffer | None = None,
+                   work_counters: DeviceBuffer | None = None):
+    """skyhip_cloud_raymarch: the VolumetricCloud render pass."""
+    _abi.check(_abi.lib().skyhip_cloud_raymarch(
+        ctx.handle, C.byref(atmosphere_block), C.byref(common), C.byref(cloud), C.byref(material), transmittance.ptr,
+        t_size[0], t_size[1], ap_l.ptr, ap_t.ptr, ap_size[0], ap_size[1], ap_size[2], blue_noise.ptr, blue_noise_size,
+        depth.ptr if depth is not None else None, shading_index.ptr if shading_index is not None else None,
+        out_rgba16f.ptr, out_distance.ptr, width, height, work_counters.ptr if work_counters is not None else None))

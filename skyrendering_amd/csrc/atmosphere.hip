@@ -50,8 +50,7 @@ __device__ inline F3 transmittance_texel(const SkyhipAtmosphereBufferData& b, in
     S od_r = 0, od_m = 0, od_o = 0;  // density integrals (extinction is linear in them)
     for (float i = 0.5f; i < steps; i += 1.0f) {
         S di = S(i) * dx;
-        S q = di * (di + two_r_mu) + rho2;
-        q = q > S(0) ? q : S(0);
+        S q = di * (di + two_r_mu) + rho2;  // r_i^2 - R_b^2
         S rd = ssqrt(q + rb2);
         S alt = q / (rd + rb);
         S er = sexp(-alt * inv_hr), em = sexp(-alt * inv_hm);
@@ -432,4 +431,42 @@ extern "C" int skyhip_aerial_perspective_lut(SkyhipContext* ctx, const SkyhipAtm
                                      transmittance_height, multiscattering_rgba, multiscattering_width,
                                      multiscattering_height, blue_noise, blue_noise_size, luminance_rgba,
                                      ap_transmittance_rgba, width, height, depth);
+}
+
+extern "C" int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                                         const SkyhipAtmosphereRenderBufferData* render, float* transmittance_rgba,
+                                         int32_t transmittance_width, int32_t transmittance_height,
+                                         float* multiscattering_rgba, int32_t multiscattering_width,
+                                         int32_t multiscattering_height, float* sky_view_luminance_rgba,
+                                         float* sky_view_transmittance_rgba, int32_t sky_view_width,
+                                         int32_t sky_view_height, float* ap_luminance_rgba,
+                                         float* ap_transmittance_rgba, int32_t ap_width, int32_t ap_height,
+                                         int32_t ap_depth) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(validate_atmosphere(atmosphere));
+    SKYHIP_CHECK(validate_render(render));
+    const int tw = transmittance_width, th = transmittance_height, mw = multiscattering_width,
+              mh = multiscattering_height;
+    SKYHIP_CHECK(validate_lut(transmittance_rgba, tw, th, "transmittance"));
+    SKYHIP_CHECK(validate_lut(multiscattering_rgba, mw, mh, "multiscattering"));
+    SKYHIP_CHECK(validate_lut(sky_view_luminance_rgba, sky_view_width, sky_view_height, "sky-view luminance"));
+    SKYHIP_CHECK(validate_lut(sky_view_transmittance_rgba, sky_view_width, sky_view_height, "sky-view transmittance"));
+    SKYHIP_CHECK(validate_lut(ap_luminance_rgba, ap_width, ap_height, "aerial-perspective luminance"));
+    SKYHIP_CHECK(validate_lut(ap_transmittance_rgba, ap_width, ap_height, "aerial-perspective transmittance"));
+    if (ap_depth < 2 || ap_depth > 1024) return fail(SKYHIP_ERR_INVALID_SIZE, "aerial-perspective depth outside [2, 1024]");
+    hipStream_t st = ctx->stream;
+    SKYHIP_CHECK(launch_transmittance(st, *atmosphere, transmittance_rgba, tw, th));
+    SKYHIP_CHECK(launch_multiscattering(st, *atmosphere, transmittance_rgba, tw, th, multiscattering_rgba, mw, mh));
+    // Sky view and aerial perspective both read T and MS only: run them side by side.
+    SKYHIP_HIP(hipEventRecord(ctx->fork_event, st));
+    SKYHIP_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_event, 0));
+    SKYHIP_CHECK(launch_aerial_perspective(ctx->aux_stream, *atmosphere, *render, transmittance_rgba, tw, th,
+                                           multiscattering_rgba, mw, mh, nullptr, 1, ap_luminance_rgba,
+                                           ap_transmittance_rgba, ap_width, ap_height, ap_depth));
+    SKYHIP_CHECK(launch_sky_view(st, *atmosphere, *render, transmittance_rgba, tw, th, multiscattering_rgba, mw, mh,
+                                 nullptr, 1, sky_view_luminance_rgba, sky_view_transmittance_rgba, sky_view_width,
+                                 sky_view_height));
+    SKYHIP_HIP(hipEventRecord(ctx->join_event, ctx->aux_stream));
+    SKYHIP_HIP(hipStreamWaitEvent(st, ctx->join_event, 0));
+    return SKYHIP_OK;
 }

@@ -108,12 +108,15 @@ __device__ inline F3 lut_bilinear(const float4* __restrict__ lut, int w, int h, 
             by * (bx * a.z + ax * b.z) + ay * (bx * c.z + ax * d.z)};
 }
 
-// GetSunVisibility (Atmosphere.glsl:105-117) at a point given by its radius
-// r, rho = sqrt(r^2 - R_b^2), altitude and mu_s.
+// GetSunVisibility (Atmosphere.glsl:105-117) at a point given by its radius r,
+// q = r^2 - R_b^2 (negative below the ground, where the reference's aerial-
+// perspective froxels can reach), altitude and mu_s.
 __device__ inline F3 sun_visibility(const AtmDev& A, const float4* __restrict__ T, int tw, int th, float r,
-                                    float rho, float altitude, float mu_s) {
-    // GetTransmittanceTextureUvFromRMu (:90-103)
-    float disc = r * r * mu_s * mu_s + (A.H2 - rho * rho);
+                                    float q, float altitude, float mu_s) {
+    // GetTransmittanceTextureUvFromRMu (:90-103): rho = SafeSqrt(r^2 - R_b^2),
+    // discriminant r^2 (mu^2 - 1) + R_t^2 = r^2 mu^2 + (H^2 - q).
+    float rho = sqrtf(fmaxf(q, 0.0f));
+    float disc = r * r * mu_s * mu_s + (A.H2 - q);
     float d = fmaxf(-r * mu_s + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
     float d_min = A.thickness - altitude;
     float d_max = rho + A.H;
@@ -144,13 +147,22 @@ struct RayStart {
 __device__ inline bool ray_hits_ground(const AtmDev& A, float r, float alt0, float mu) {
     return mu < 0.0f && r * r * mu * mu - alt0 * (r + A.rb) >= 0.0f;
 }
+// Roots of t^2 + 2 r mu t - c = 0 in the conjugate form that does not cancel:
+// the far root -r mu + sqrt(disc) equals c / (r mu + sqrt(disc)) when mu > 0,
+// and the near root -r mu - sqrt(disc) equals -c / (-r mu + sqrt(disc)).
 __device__ inline float distance_to_top(const AtmDev& A, float r, float alt0, float mu) {
-    float disc = r * r * mu * mu + (A.H2 - alt0 * (r + A.rb));
-    return fmaxf(-r * mu + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+    float c = A.H2 - alt0 * (r + A.rb);  // R_t^2 - r^2
+    float disc = r * r * mu * mu + c;
+    float sq = sqrtf(fmaxf(disc, 0.0f));
+    float d = mu > 0.0f ? c / (r * mu + sq) : -r * mu + sq;
+    return fmaxf(d, 0.0f);
 }
 __device__ inline float distance_to_bottom(const AtmDev& A, float r, float alt0, float mu) {
-    float disc = r * r * mu * mu - alt0 * (r + A.rb);
-    return fmaxf(-r * mu - sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+    float c = alt0 * (r + A.rb);  // r^2 - R_b^2
+    float disc = r * r * mu * mu - c;
+    float sq = sqrtf(fmaxf(disc, 0.0f));
+    float d = (mu < 0.0f && disc >= 0.0f) ? c / (-r * mu + sq) : -r * mu - sq;
+    return fmaxf(d, 0.0f);
 }
 
 // ComputeScatteredLuminance (Atmosphere.glsl:220-295).  kMS selects the
@@ -174,17 +186,16 @@ __device__ inline void scattered_luminance(const AtmDev& A, const float4* __rest
     Lf = f3(0.0f);
     for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f) {
         float d = i * dx;
-        float q = fmaxf(fmaf(d, d + two_r_mu, rho0_2), 0.0f);  // r_d^2 - R_b^2
-        float r_d = sqrtf(q + rb2);
+        float q = fmaf(d, d + two_r_mu, rho0_2);  // r_d^2 - R_b^2, negative below ground
+        float r_d = sqrtf(fmaxf(q + rb2, 0.0f));
         float alt = q / (r_d + A.rb);
-        float rho = sqrtf(q);
         float mu_s = (s.r * s.mu_s0 + d * s.nu) / r_d;
         Densities dn = densities(A, alt);
         F3 sr = A.rayleigh * dn.rayleigh;
         F3 sm = A.mie_s * dn.mie;
         F3 ext = sr + A.mie_ext * dn.mie + A.ozone * dn.ozone;
         F3 one_minus_t = {-expm1f(-ext.x * dx), -expm1f(-ext.y * dx), -expm1f(-ext.z * dx)};
-        F3 Li = (sr * rp + sm * mp) * sun_visibility(A, T, tw, th, r_d, rho, alt, mu_s);
+        F3 Li = (sr * rp + sm * mp) * sun_visibility(A, T, tw, th, r_d, q, alt, mu_s);
         F3 inv_ext = {1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z};
         if (!kMS) {
             Li += A.ms_mask * ms_contribution(A, MS, mw, mh, alt, mu_s) * (sr + sm);

@@ -1,0 +1,395 @@
+// Volumetric-cloud raymarch for gfx950 (VolumetricCloudRender.comp:139-210).
+//
+// One ray per lane.  A wave64 covers an 8x8 pixel tile (a 256-lane workgroup
+// covers 16x16), so the rays of a wave march similar step counts through
+// neighbouring texels of the L2-resident cloud textures.  The march loop is
+// wave-uniform: it ends when every lane of the wave has finished or dropped
+// below the transmittance threshold.  Outputs: RGBA16F (luminance,
+// transmittance) and R32F mean cloud distance, GL texel order.
+#include "cloud_device.h"
+
+namespace skyhip {
+
+struct CloudArgs {
+    SkyhipAtmosphereBufferData atm;
+    SkyhipCloudCommonBufferData common;
+    SkyhipCloudBufferData cloud;
+    MaterialDev mat;
+    const float4* T;
+    int tw, th;
+    const float4* ap_l;
+    const float4* ap_t;
+    int aw, ah, ad;
+    const uint16_t* blue_noise;
+    int bn_mask, bn_size;
+    const float* depth;  // R32F at image size, or null (far plane)
+    const float* index;  // RG32F shading index at output size (checkerboard mode), or null
+    int image_w, image_h;
+    uint16_t* out_rgba16f;
+    float* out_distance;
+    int width, height;
+    unsigned long long* counters;  // counting variant: steps, density evals, detail fetches, cloud steps
+};
+
+// Trilinear CLAMP_TO_EDGE fetch of an RGBA32F 3D LUT at texel-space coordinates.
+__device__ inline F3 lut3d_trilinear(const float4* __restrict__ lut, int w, int h, int d, float fx, float fy,
+                                     float fz) {
+    fx = fminf(fmaxf(fx, 0.0f), float(w - 1));
+    fy = fminf(fmaxf(fy, 0.0f), float(h - 1));
+    fz = fminf(fmaxf(fz, 0.0f), float(d - 1));
+    int x0 = int(fx), y0 = int(fy), z0 = int(fz);
+    int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1), z1 = min(z0 + 1, d - 1);
+    float ax = fx - float(x0), ay = fy - float(y0), az = fz - float(z0);
+    F3 r = f3(0.0f);
+    for (int k = 0; k < 8; ++k) {
+        int xx = (k & 1) ? x1 : x0, yy = (k & 2) ? y1 : y0, zz = (k & 4) ? z1 : z0;
+        float wgt = ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az);
+        float4 v = lut[(zz * h + yy) * w + xx];
+        r += F3{v.x, v.y, v.z} * wgt;
+    }
+    return r;
+}
+
+struct MarchAcc {
+    float transmittance, transmittance_sum, weighted_t_sum;
+    float sun_env_r, sun_env_g;
+};
+
+// RayMarchStep, VolumetricCloudRender.comp:116-137.
+template <int kMat, bool kCount>
+__device__ inline void march_step(const CloudArgs& a, const Shell& S, F3 pos, float t, float h01, float step_size,
+                                  float cos_sun_view, MarchAcc& acc, WorkCount* wc) {
+    const float sigma_t = sample_sigma_t<kMat, kCount>(a.mat, pos, h01, S.cam, wc);
+    if (sigma_t < 1e-5f) return;
+    if (kCount) wc->cloud_steps++;
+    const float tr = __expf(-step_size * sigma_t);
+    const float t_sun = sample_shadow<kMat, kCount>(a.mat, S, pos, a.cloud.uShadowSteps, a.cloud.uShadowDistance, wc);
+    const float w = __expf(-a.cloud.uSunMultiscatteringSigmaScale * sigma_t);
+    const float phase = henyey_greenstein(cos_sun_view, -0.15f) * 2.16f * (1.0f - w) +
+                        henyey_greenstein(cos_sun_view, 0.85f) * w;
+    float se_r = t_sun * phase;
+    float se_g = a.cloud.uEnvBottomVisibility * (1.0f - h01) + h01;
+    se_g = se_g - se_g * __expf(-a.cloud.uEnvMultiscatteringSigmaScale * sigma_t);
+    se_r = se_r - se_r * tr;
+    se_g = se_g - se_g * tr;
+    acc.sun_env_r += acc.transmittance * se_r;
+    acc.sun_env_g += acc.transmittance * se_g;
+    acc.transmittance_sum += acc.transmittance;
+    acc.weighted_t_sum += t * acc.transmittance;
+    acc.transmittance *= tr;
+}
+
+// FP64 unprojection of (u, v, depth) through uInvMVP, relative to the camera.
+__device__ inline F3 unproject(const float* m, float x, float y, float z, F3 cam, float* frag_dist) {
+    double v[4] = {x, y, z, 1.0}, o[4];
+    for (int row = 0; row < 4; ++row)
+        o[row] = double(m[row]) * v[0] + double(m[4 + row]) * v[1] + double(m[8 + row]) * v[2] +
+                 double(m[12 + row]) * v[3];
+    double px = o[0] / o[3] - double(cam.x), py = o[1] / o[3] - double(cam.y), pz = o[2] / o[3] - double(cam.z);
+    double len = sqrt(px * px + py * py + pz * pz);
+    *frag_dist = float(len);
+    return {float(px / len), float(py / len), float(pz / len)};
+}
+
+template <int kMat, bool kCount>
+__global__ __launch_bounds__(256) void cloud_raymarch_kernel(CloudArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool active = px < a.width && py < a.height;
+    int executed = 0;
+    WorkCount wc;
+    if (active) {
+        const SkyhipCloudCommonBufferData& cb = a.common;
+        Shell S;
+        S.cam = f3(cb.uCameraPos);
+        S.sun = f3(cb.uSunDirection);
+        S.R = cb.uEarthRadius;
+        S.hb = cb.uBottomAltitude;
+        S.ht = cb.uTopAltitude;
+        S.inv_thickness = 1.0f / (S.ht - S.hb);
+        const AtmDev A = make_atm(a.atm);
+
+        int pcx = px, pcy = py;
+        if (a.index) {  // checkerboard sample choice, IndexToOffset (VolumetricCloudCommon.glsl:42-52)
+            uint32_t index = uint32_t(a.index[(py * a.width + px) * 2]);
+            pcx = px * 2 + int(((index + 1u) >> 1) & 1u);
+            pcy = py * 2 + int(((index + 2u) >> 1) & 1u);
+        }
+        const float u = (float(pcx) + 0.5f) / float(a.image_w), v = (float(pcy) + 0.5f) / float(a.image_h);
+        const float depth = a.depth ? a.depth[pcy * a.image_w + pcx] : 1.0f;
+        float frag_dist;
+        const F3 view = unproject(cb.uInvMVP, u * 2.0f - 1.0f, v * 2.0f - 1.0f, depth * 2.0f - 1.0f, S.cam, &frag_dist);
+
+        // RayShellIntersect (:39-71) with b = r mu and c = R'^2 - r^2 factored.
+        const float r = S.cam.z + S.R;
+        const float mu = view.z;
+        const float b = r * mu;
+        const float cb_ = (S.hb - S.cam.z) * (2.0f * S.R + S.hb + S.cam.z);
+        const float ct_ = (S.ht - S.cam.z) * (2.0f * S.R + S.ht + S.cam.z);
+        float nb, fb, db, nt, ft, dt;
+        sphere_roots(b, cb_, nb, fb, db);
+        sphere_roots(b, ct_, nt, ft, dt);
+        float t1[2] = {0.0f, 0.0f}, t2[2] = {0.0f, 0.0f};
+        if (S.cam.z < S.hb) {
+            t1[0] = fb;
+            t2[0] = ft;
+        } else if (S.cam.z < S.ht) {
+            if (db >= 0.0f && mu < 0.0f) {
+                t2[0] = nb;
+                t1[1] = fb;
+                t2[1] = ft;
+            } else {
+                t2[0] = ft;
+            }
+        } else {
+            if (db >= 0.0f && mu < 0.0f) {
+                t1[0] = nt;
+                t2[0] = nb;
+                t1[1] = fb;
+                t2[1] = ft;
+            } else if (dt >= 0.0f && mu < 0.0f) {
+                t1[0] = nt;
+                t2[0] = ft;
+            }
+        }
+        const float vis = fminf(frag_dist, a.cloud.uMaxVisibleDistance);
+        for (int i = 0; i < 2; ++i) t2[i] = fminf(fmaxf(vis, t1[i]), t2[i]);
+
+        const float cos_sun_view = dot(S.sun, view);
+        const float noise = a.blue_noise[(py & a.bn_mask) * a.bn_size + (px & a.bn_mask)] * (1.0f / 65535.0f);
+        float jitter = noise + cb.uFrameID * 0.61803398875f;
+        jitter = jitter - floorf(jitter);
+        MarchAcc acc{1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int seg = 0; seg < 2; ++seg) {
+            float dist = t2[seg] - t1[seg];
+            if (seg == 1 && !(dist > 0.0f)) break;
+            dist = fminf(dist, a.cloud.uMaxRaymarchDistance);
+            const uint32_t n = uint32_t(fmaxf(a.cloud.uMaxRaymarchSteps * (dist / a.cloud.uMaxRaymarchDistance), 1.0f));
+            const float step_size = dist / float(n);
+            float t = t1[seg] + step_size * jitter;
+            for (uint32_t cnt = n; cnt != 0; --cnt, t += step_size) {
+                const F3 pos = S.cam + view * t;
+                const float h01 = height01(S, pos);
+                march_step<kMat, kCount>(a, S, pos, t, h01, step_size, cos_sun_view, acc, &wc);
+                ++executed;
+                if (acc.transmittance < 0.01f) break;  // kMinTransmittance
+            }
+        }
+
+        const float average_t = acc.weighted_t_sum == 0.0f ? frag_dist : acc.weighted_t_sum / acc.transmittance_sum;
+        a.out_distance[py * a.width + px] = average_t;
+        const F3 ap = S.cam + view * average_t;
+        // GetSunVisibility(transmittance_texture, pos) and SunCosTheta at the mean distance.
+        float q = ap.x * ap.x + ap.y * ap.y + ap.z * (ap.z + 2.0f * S.R) + (S.R - A.rb) * (S.R + A.rb);
+        const float rr = sqrtf(fmaxf(q + A.rb * A.rb, 0.0f));
+        const float mu_s = dot(S.sun, F3{ap.x, ap.y, ap.z + S.R}) / rr;
+        const F3 sun_vis = sun_visibility(A, a.T, a.tw, a.th, rr, q, q / (rr + A.rb), mu_s);
+        const float sun_cos = clamp01(mu_s);
+        F3 luminance = sun_vis * A.solar * (acc.sun_env_r * a.cloud.uSunIlluminanceScale) +
+                       f3(a.cloud.uEnvColorScale) * (acc.sun_env_g * __powf(sun_cos, a.cloud.uEnvSunHeightCurveExp));
+        // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98)
+        float t_ap = average_t;
+        if (r > A.rt) {
+            float disc = r * r * (mu * mu - 1.0f) + A.rt * A.rt;
+            if (mu < 0.0f && disc >= 0.0f)
+                t_ap -= fmaxf(-r * mu - sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+            else
+                t_ap = 0.0f;
+        }
+        const float w = sqrtf(t_ap / cb.uAerialPerspectiveLutMaxDistance);
+        const F3 ap_t = lut3d_trilinear(a.ap_t, a.aw, a.ah, a.ad, u * float(a.aw - 1), v * float(a.ah - 1),
+                                        w * float(a.ad - 1));
+        const F3 ap_l = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, u * float(a.aw - 1), v * float(a.ah - 1),
+                                        w * float(a.ad - 1));
+        luminance = luminance * ap_t + ap_l * (1.0f - acc.transmittance);
+        luminance = luminance * (1.0f / fmaxf(1e-5f, 1.0f - acc.transmittance));
+        const float fade = smoothstepf(a.cloud.uMaxVisibleDistance * 0.75f, a.cloud.uMaxVisibleDistance, t1[0]);
+        const float T = acc.transmittance * (1.0f - fade) + fade;
+        luminance = luminance * (1.0f - T);
+        ushort4 o;
+        o.x = __half_as_ushort(__float2half_rn(luminance.x));
+        o.y = __half_as_ushort(__float2half_rn(luminance.y));
+        o.z = __half_as_ushort(__float2half_rn(luminance.z));
+        o.w = __half_as_ushort(__float2half_rn(T));
+        reinterpret_cast<ushort4*>(a.out_rgba16f)[py * a.width + px] = o;
+    }
+    if (kCount) {
+        uint32_t v[4] = {uint32_t(executed), wc.evals, wc.detail, wc.cloud_steps};
+        for (int k = 0; k < 4; ++k) {
+            uint32_t s = v[k];
+            for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) atomicAdd(a.counters + k, (unsigned long long)s);
+        }
+    }
+}
+
+// Texture descriptor from the ABI material (levels packed back to back).
+static void make_tex2d(Tex2D& t, const uint8_t* data, int size, int levels, int channels) {
+    t.data = data;
+    t.size = size;
+    t.levels = levels;
+    t.channels = channels;
+    uint32_t off = 0;
+    for (int l = 0; l < kMaxMipLevels; ++l) {
+        t.offset[l] = off;
+        if (l < levels) {
+            uint32_t s = uint32_t(size >> l > 0 ? size >> l : 1);
+            off += s * s * uint32_t(channels);
+        }
+    }
+}
+static void make_tex3d(Tex3D& t, const uint8_t* data, const int dim[3], int levels) {
+    t.data = data;
+    t.dim[0] = dim[0], t.dim[1] = dim[1], t.dim[2] = dim[2];
+    t.levels = levels;
+    uint32_t off = 0;
+    for (int l = 0; l < kMaxMipLevels; ++l) {
+        t.offset[l] = off;
+        if (l < levels) {
+            uint32_t a = uint32_t(dim[0] >> l > 0 ? dim[0] >> l : 1), b = uint32_t(dim[1] >> l > 0 ? dim[1] >> l : 1),
+                     c = uint32_t(dim[2] >> l > 0 ? dim[2] >> l : 1);
+            off += a * b * c;
+        }
+    }
+}
+
+static bool pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+
+extern "C" int skyhip_mip_levels(int32_t width, int32_t height, int32_t depth);
+
+int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
+    if (m == nullptr) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "material is NULL");
+    std::memset(&d, 0, sizeof(d));
+    d.kind = m->kind;
+    d.common = m->common;
+    d.m1 = m->material1;
+    d.m0 = m->material0;
+    d.minimal_density = m->minimal.uDensity;
+    d.voxel = m->voxel;
+    auto finite = [](const void* p, size_t bytes) { return finite_block(reinterpret_cast<const float*>(p), bytes / 4); };
+    switch (m->kind) {
+        case SKYHIP_MATERIAL_DEFAULT0:
+        case SKYHIP_MATERIAL_DEFAULT1: {
+            if (!finite(&m->common, sizeof(m->common)) || !finite(&m->material1, sizeof(m->material1)) ||
+                !finite(&m->material0, sizeof(m->material0)))
+                return fail(SKYHIP_ERR_NON_FINITE, "material block has a non-finite field");
+            if (!m->cloud_map || !m->detail) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "material texture is NULL");
+            if (!pow2(m->cloud_map_size) || !pow2(m->detail_size) || m->cloud_map_size > 16384 || m->detail_size > 1024)
+                return fail(SKYHIP_ERR_INVALID_SIZE, "cloud map / detail sizes must be powers of two");
+            if (m->cloud_map_levels < 1 || m->cloud_map_levels > skyhip_mip_levels(m->cloud_map_size, m->cloud_map_size, 1) ||
+                m->detail_levels < 1 || m->detail_levels > skyhip_mip_levels(m->detail_size, m->detail_size, m->detail_size))
+                return fail(SKYHIP_ERR_INVALID_SIZE, "bad mip level count");
+            make_tex2d(d.cloud_map, m->cloud_map, m->cloud_map_size, m->cloud_map_levels, 2);
+            int dim[3] = {m->detail_size, m->detail_size, m->detail_size};
+            make_tex3d(d.detail, m->detail, dim, m->detail_levels);
+            if (m->kind == SKYHIP_MATERIAL_DEFAULT0) {
+                if (!m->displacement || !pow2(m->displacement_size) || m->displacement_levels < 1 ||
+                    m->displacement_levels > skyhip_mip_levels(m->displacement_size, m->displacement_size, 1))
+                    return fail(SKYHIP_ERR_INVALID_SIZE, "bad displacement texture");
+                make_tex2d(d.displacement, m->displacement, m->displacement_size, m->displacement_levels, 4);
+            }
+            return SKYHIP_OK;
+        }
+        case SKYHIP_MATERIAL_MINIMAL:
+            if (!std::isfinite(m->minimal.uDensity)) return fail(SKYHIP_ERR_NON_FINITE, "non-finite density");
+            return SKYHIP_OK;
+        case SKYHIP_MATERIAL_VOXEL:
+            if (!finite(&m->voxel, sizeof(m->voxel))) return fail(SKYHIP_ERR_NON_FINITE, "voxel block non-finite");
+            if (!m->voxel_texture) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "voxel texture is NULL");
+            for (int k = 0; k < 3; ++k)
+                if (m->voxel_dim[k] < 1 || m->voxel_dim[k] > 4096)
+                    return fail(SKYHIP_ERR_INVALID_SIZE, "voxel dims outside [1, 4096]");
+            if (m->voxel_levels < 1 ||
+                m->voxel_levels > skyhip_mip_levels(m->voxel_dim[0], m->voxel_dim[1], m->voxel_dim[2]))
+                return fail(SKYHIP_ERR_INVALID_SIZE, "bad voxel mip level count");
+            make_tex3d(d.voxel_tex, m->voxel_texture, m->voxel_dim, m->voxel_levels);
+            return SKYHIP_OK;
+        default:
+            return fail(SKYHIP_ERR_INVALID_ARGUMENT, "unknown material kind %d", m->kind);
+    }
+}
+
+template <bool kCount>
+static int launch_raymarch(hipStream_t st, const CloudArgs& a) {
+    dim3 grid(div_up(a.width, 16), div_up(a.height, 16));
+    switch (a.mat.kind) {
+        case SKYHIP_MATERIAL_DEFAULT0:
+            hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_DEFAULT0, kCount>), grid, dim3(256), 0, st, a);
+            break;
+        case SKYHIP_MATERIAL_DEFAULT1:
+            hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_DEFAULT1, kCount>), grid, dim3(256), 0, st, a);
+            break;
+        case SKYHIP_MATERIAL_MINIMAL:
+            hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_MINIMAL, kCount>), grid, dim3(256), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_VOXEL, kCount>), grid, dim3(256), 0, st, a);
+            break;
+    }
+    SKYHIP_LAUNCH_CHECK("cloud_raymarch_kernel");
+    return SKYHIP_OK;
+}
+
+int validate_atmosphere(const SkyhipAtmosphereBufferData* b);
+int validate_lut(const void* p, int w, int h, const char* what);
+int validate_blue_noise(const uint16_t* bn, int size);
+
+}  // namespace skyhip
+
+using namespace skyhip;
+
+extern "C" int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                                     const SkyhipCloudCommonBufferData* common, const SkyhipCloudBufferData* cloud,
+                                     const SkyhipCloudMaterial* material, const float* transmittance_rgba,
+                                     int32_t transmittance_width, int32_t transmittance_height,
+                                     const float* ap_luminance_rgba, const float* ap_transmittance_rgba,
+                                     int32_t ap_width, int32_t ap_height, int32_t ap_depth,
+                                     const uint16_t* blue_noise, int32_t blue_noise_size, const float* depth,
+                                     const float* shading_index, void* out_rgba16f, float* out_distance,
+                                     int32_t width, int32_t height, unsigned long long* work_counters) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(validate_atmosphere(atmosphere));
+    if (!common || !cloud) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "cloud blocks are NULL");
+    if (!finite_block(reinterpret_cast<const float*>(common->uInvMVP), 16) ||
+        !finite_block(common->uCameraPos, 3) || !finite_block(reinterpret_cast<const float*>(&common->uLinearDepthParam), 12) ||
+        !finite_block(reinterpret_cast<const float*>(cloud), sizeof(*cloud) / 4))
+        return fail(SKYHIP_ERR_NON_FINITE, "cloud blocks have a non-finite field");
+    if (!(common->uTopAltitude > common->uBottomAltitude) || !(cloud->uMaxRaymarchDistance > 0.0f) ||
+        !(common->uAerialPerspectiveLutMaxDistance > 0.0f))
+        return fail(SKYHIP_ERR_NON_FINITE, "degenerate cloud layer or raymarch distance");
+    if (!(cloud->uShadowSteps > 0.0f && cloud->uShadowSteps <= 256.0f) ||
+        !(cloud->uMaxRaymarchSteps >= 0.0f && cloud->uMaxRaymarchSteps <= 4096.0f))
+        return fail(SKYHIP_ERR_INVALID_SIZE, "shadow steps must be in (0, 256], raymarch steps in [0, 4096]");
+    CloudArgs a;
+    std::memset(&a, 0, sizeof(a));
+    SKYHIP_CHECK(build_material(material, a.mat));
+    SKYHIP_CHECK(validate_lut(transmittance_rgba, transmittance_width, transmittance_height, "transmittance"));
+    SKYHIP_CHECK(validate_lut(ap_luminance_rgba, ap_width, ap_height, "aerial-perspective luminance"));
+    SKYHIP_CHECK(validate_lut(ap_transmittance_rgba, ap_width, ap_height, "aerial-perspective transmittance"));
+    if (ap_depth < 2 || ap_depth > 1024) return fail(SKYHIP_ERR_INVALID_SIZE, "aerial-perspective depth outside [2, 1024]");
+    if (blue_noise == nullptr) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "blue noise is NULL");
+    SKYHIP_CHECK(validate_blue_noise(blue_noise, blue_noise_size));
+    if (!out_rgba16f || !out_distance) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "outputs are NULL");
+    if (width < 1 || height < 1 || width > 16384 || height > 16384)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "image size %dx%d outside [1, 16384]", width, height);
+    a.atm = *atmosphere;
+    a.common = *common;
+    a.cloud = *cloud;
+    a.T = reinterpret_cast<const float4*>(transmittance_rgba);
+    a.tw = transmittance_width, a.th = transmittance_height;
+    a.ap_l = reinterpret_cast<const float4*>(ap_luminance_rgba);
+    a.ap_t = reinterpret_cast<const float4*>(ap_transmittance_rgba);
+    a.aw = ap_width, a.ah = ap_height, a.ad = ap_depth;
+    a.blue_noise = blue_noise;
+    a.bn_size = blue_noise_size, a.bn_mask = blue_noise_size - 1;
+    a.depth = depth;
+    a.index = shading_index;
+    a.image_w = shading_index ? width * 2 : width;
+    a.image_h = shading_index ? height * 2 : height;
+    a.out_rgba16f = reinterpret_cast<uint16_t*>(out_rgba16f);
+    a.out_distance = out_distance;
+    a.width = width, a.height = height;
+    a.counters = work_counters;
+    return work_counters ? launch_raymarch<true>(ctx->stream, a) : launch_raymarch<false>(ctx->stream, a);
+}

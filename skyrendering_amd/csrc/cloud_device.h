@@ -1,0 +1,225 @@
+// Device-side cloud model shared by the raymarcher and the path tracer:
+// UNORM8 texture sampling with the reference's sampler state, the material
+// plugins' SampleSigmaT, the light march and the phase functions.
+//
+// Texture sampling is software, with exact GL semantics (GL 4.6, 8.14):
+// textureLod with LINEAR magnification and NEAREST_MIPMAP_NEAREST
+// minification: lambda <= 0.5 -> bilinear/trilinear on level 0, otherwise the
+// nearest texel of level min(ceil(lambda + 0.5) - 1, q).  REPEAT wrapping for
+// the default materials (power-of-two sizes), CLAMP_TO_BORDER(0) for voxels.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "atmosphere_device.h"
+
+namespace skyhip {
+
+constexpr int kMaxMipLevels = 16;
+
+struct Tex2D {
+    const uint8_t* data;
+    int size, levels, channels;
+    uint32_t offset[kMaxMipLevels];
+};
+
+struct Tex3D {
+    const uint8_t* data;
+    int dim[3];
+    int levels;
+    uint32_t offset[kMaxMipLevels];
+};
+
+// GL NEAREST_MIPMAP_NEAREST level for lambda > 0.5 (the caller handles
+// lambda <= 0.5, including NaN and -inf, as magnification).
+__device__ __forceinline__ int nearest_level(float lambda, int levels) {
+    float d = ceilf(fminf(lambda, 64.0f) + 0.5f) - 1.0f;
+    return min(int(d), levels - 1);
+}
+
+__device__ __forceinline__ int floor_to_int(float x) { return int(floorf(fminf(fmaxf(x, -1.0e9f), 1.0e9f))); }
+
+// textureLod(sampler2D REPEAT) of the first two channels of an RG8/RGBA8 texture.
+template <int kChannels>
+__device__ inline void tex2d_repeat(const Tex2D& t, float u, float v, float lambda, float out[kChannels]) {
+    if (!(lambda > 0.5f)) {
+        const int s = t.size, m = s - 1;
+        float x = u * float(s) - 0.5f, y = v * float(s) - 0.5f;
+        int ix = floor_to_int(x), iy = floor_to_int(y);
+        float ax = x - floorf(x), ay = y - floorf(y);
+        int x0 = ix & m, x1 = (ix + 1) & m, y0 = iy & m, y1 = (iy + 1) & m;
+        const uint8_t* d = t.data;
+        for (int c = 0; c < kChannels; ++c) {
+            float t00 = d[(y0 * s + x0) * t.channels + c], t10 = d[(y0 * s + x1) * t.channels + c];
+            float t01 = d[(y1 * s + x0) * t.channels + c], t11 = d[(y1 * s + x1) * t.channels + c];
+            out[c] = ((1.0f - ay) * ((1.0f - ax) * t00 + ax * t10) + ay * ((1.0f - ax) * t01 + ax * t11)) *
+                     (1.0f / 255.0f);
+        }
+        return;
+    }
+    const int l = nearest_level(lambda, t.levels);
+    const int s = max(1, t.size >> l), m = s - 1;
+    int xi = floor_to_int(u * float(s)) & m, yi = floor_to_int(v * float(s)) & m;
+    const uint8_t* d = t.data + t.offset[l] + (yi * s + xi) * t.channels;
+    for (int c = 0; c < kChannels; ++c) out[c] = float(d[c]) * (1.0f / 255.0f);
+}
+
+// textureLod(sampler3D) of an R8 volume; kBorder selects CLAMP_TO_BORDER(0),
+// otherwise REPEAT (power-of-two dims).
+template <bool kBorder>
+__device__ inline float tex3d_r8(const Tex3D& t, float u, float v, float w, float lambda) {
+    const uint8_t* d = t.data;
+    if (!(lambda > 0.5f)) {
+        const int sx = t.dim[0], sy = t.dim[1], sz = t.dim[2];
+        float x = u * float(sx) - 0.5f, y = v * float(sy) - 0.5f, z = w * float(sz) - 0.5f;
+        int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
+        float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
+        float r = 0.0f;
+        for (int k = 0; k < 8; ++k) {
+            int xx = ix + (k & 1), yy = iy + ((k >> 1) & 1), zz = iz + ((k >> 2) & 1);
+            float wgt = ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az);
+            float texel;
+            if (kBorder) {
+                bool inside = xx >= 0 && xx < sx && yy >= 0 && yy < sy && zz >= 0 && zz < sz;
+                texel = inside ? float(d[((long long)zz * sy + yy) * sx + xx]) : 0.0f;
+            } else {
+                xx &= sx - 1, yy &= sy - 1, zz &= sz - 1;
+                texel = float(d[((long long)zz * sy + yy) * sx + xx]);
+            }
+            r += wgt * texel;
+        }
+        return r * (1.0f / 255.0f);
+    }
+    const int l = nearest_level(lambda, t.levels);
+    const int sx = max(1, t.dim[0] >> l), sy = max(1, t.dim[1] >> l), sz = max(1, t.dim[2] >> l);
+    int xi = floor_to_int(u * float(sx)), yi = floor_to_int(v * float(sy)), zi = floor_to_int(w * float(sz));
+    if (kBorder) {
+        if (xi < 0 || xi >= sx || yi < 0 || yi >= sy || zi < 0 || zi >= sz) return 0.0f;
+    } else {
+        xi &= sx - 1, yi &= sy - 1, zi &= sz - 1;
+    }
+    return float(d[t.offset[l] + ((long long)zi * sy + yi) * sx + xi]) * (1.0f / 255.0f);
+}
+
+// Material parameters as the kernels see them.
+struct MaterialDev {
+    int kind;
+    SkyhipCloudMaterialCommonBufferData common;
+    SkyhipCloudMaterial1BufferData m1;
+    SkyhipCloudMaterial0BufferData m0;
+    float minimal_density;
+    SkyhipCloudVoxelBufferData voxel;
+    Tex2D cloud_map, displacement;
+    Tex3D detail, voxel_tex;
+};
+
+__device__ __forceinline__ float length3(F3 a) { return sqrtf(dot(a, a)); }
+
+// Work tally of the counting kernel variants (per lane).
+struct WorkCount {
+    uint32_t evals = 0, detail = 0, cloud_steps = 0;
+};
+
+// SampleSigmaT(pos, height01) of the selected material (cam = uCameraPos,
+// whose distance drives every LOD).
+template <int kMat, bool kCount = false>
+__device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float height01, F3 cam,
+                                       WorkCount* wc = nullptr) {
+    if (kCount) wc->evals++;
+    if (kMat == SKYHIP_MATERIAL_MINIMAL) return M.minimal_density;  // VolumetricCloudMaterialMinimal.glsl:6-7
+    const float dist = length3(pos - cam);
+    if (kMat == SKYHIP_MATERIAL_VOXEL) {  // VolumetricCloudMaterialVoxel.glsl:12-16
+        float u = pos.x * M.voxel.uSampleFrequency[0] + M.voxel.uSampleBias[0];
+        float v = pos.y * M.voxel.uSampleFrequency[1] + M.voxel.uSampleBias[1];
+        float lod = __log2f(M.voxel.uSampleLodK * dist) + M.voxel.uLodBias;
+        return tex3d_r8<true>(M.voxel_tex, u, v, height01, lod) * M.voxel.uDensity;
+    }
+    const SkyhipSampleInfo& ci = M.common.uCloudMapSampleInfo;
+    const SkyhipSampleInfo& di = M.common.uDetailSampleInfo;
+    float ct[2];
+    tex2d_repeat<2>(M.cloud_map, pos.x * ci.frequency + ci.bias[0], pos.y * ci.frequency + ci.bias[1],
+                    __log2f(ci.k_lod * dist) + M.common.uLodBias, ct);
+    if (kMat == SKYHIP_MATERIAL_DEFAULT1) {  // VolumetricCloudDefaultMaterial1.glsl:14-28
+        const SkyhipCloudMaterial1BufferData& p = M.m1;
+        float density = clamp01((ct[0] - p.uBaseDensityThreshold) * p.uBaseEdgeHardness);
+        density *= clamp01((1.0f - height01) * p.uBaseHeightHardness);
+        if (density == 0.0f) return 0.0f;
+        if (kCount) wc->detail++;
+        float detail = tex3d_r8<false>(M.detail, pos.x * di.frequency + di.bias[0], pos.y * di.frequency + di.bias[1],
+                                       pos.z * di.frequency, __log2f(di.k_lod * dist) + M.common.uLodBias);
+        detail = (detail + p.uDetailBase) * p.uDetailScale;
+        detail *= fmaxf(clamp01(height01 - p.uHeightCut), clamp01(p.uEdgeCur - ct[0]));
+        return clamp01(density - detail) * M.common.uDensity * height01;
+    }
+    // SKYHIP_MATERIAL_DEFAULT0, VolumetricCloudDefaultMaterial0.glsl:18-31
+    const SkyhipSampleInfo& si = M.common.uDisplacementSampleInfo;
+    float lod = __log2f(si.k_lod * dist) + M.common.uLodBias;
+    float u = pos.x * si.frequency + si.bias[0], v = pos.y * si.frequency + si.bias[1], w = pos.z * si.frequency;
+    float d0[4], d1[4];
+    tex2d_repeat<4>(M.displacement, u, v, lod, d0);
+    tex2d_repeat<4>(M.displacement, u, w, lod, d1);
+    F3 p2 = pos + M.m0.uDisplacementScale * F3{d0[0] + d1[2], d0[1], d1[3]};
+    float detail = tex3d_r8<false>(M.detail, p2.x * di.frequency + di.bias[0], p2.y * di.frequency + di.bias[1],
+                                   p2.z * di.frequency, __log2f(di.k_lod * length3(p2 - cam)) + M.common.uLodBias);
+    detail = detail * M.m0.uDetailParam[0] + M.m0.uDetailParam[1];
+    float hit = clamp01(height01 / ct[1]);                  // CalHeightMask (:9-12)
+    float mask = clamp01(hit * (hit - 1.0f) * -4.0f);
+    float x = ct[0] * mask;
+    return clamp01((x - detail) / (1.0f - detail)) * height01 * M.common.uDensity;  // Remap01 (:14-16)
+}
+
+// HenyeyGreenstein, VolumetricCloudCommon.glsl:58-63.
+__device__ __forceinline__ float henyey_greenstein(float c, float g) {
+    float a = 1.0f - g * g;
+    float b = 1.0f + g * g - 2.0f * g * c;
+    return (0.25f / kPiF) * a / (b * sqrtf(b));
+}
+
+// Cloud-layer geometry in the local z-up frame (planet centre (0, 0, -R)).
+struct Shell {
+    F3 cam, sun;
+    float R, hb, ht, inv_thickness;
+};
+
+// CalHeight01 (VolumetricCloudCommon.glsl:36-40) via |p'|^2 - R^2 = x^2 + y^2 + z (z + 2R).
+__device__ __forceinline__ float height01(const Shell& S, F3 p, float* q_out = nullptr) {
+    float q = p.x * p.x + p.y * p.y + p.z * (p.z + 2.0f * S.R);
+    float alt = q / (sqrtf(fmaxf(q + S.R * S.R, 0.0f)) + S.R);
+    if (q_out) *q_out = q;
+    return clamp01((alt - S.hb) * S.inv_thickness);
+}
+
+// SampleShadow, VolumetricCloudRender.comp:98-114: float counter, t^2 spacing.
+template <int kMat, bool kCount = false>
+__device__ inline float sample_shadow(const MaterialDev& M, const Shell& S, F3 pos, float shadow_steps,
+                                      float shadow_distance, WorkCount* wc = nullptr) {
+    float optical_depth = 0.0f;
+    const float inv_steps = 1.0f / shadow_steps;
+    const F3 sample_vector = S.sun * shadow_distance;
+    float previous_t = 0.0f;
+    for (float t = inv_steps; t <= 1.0f; t += inv_steps) {
+        float current_t = t * t;
+        float delta_t = current_t - previous_t;
+        F3 sp = pos + sample_vector * (previous_t + 0.5f * delta_t);
+        optical_depth += sample_sigma_t<kMat, kCount>(M, sp, height01(S, sp), S.cam, wc) * shadow_distance * delta_t;
+        previous_t = current_t;
+    }
+    return __expf(-optical_depth);
+}
+
+// Roots of |o + t d|^2 = R'^2 for a ray at radius r with zenith cosine mu:
+// t^2 + 2 b t - c = 0, b = r mu, c = R'^2 - r^2 (passed pre-factored), in
+// the conjugate forms that do not cancel.  disc = b^2 + c.
+__device__ __forceinline__ void sphere_roots(float b, float c, float& near_t, float& far_t, float& disc) {
+    disc = b * b + c;
+    float s = sqrtf(fmaxf(disc, 0.0f));
+    if (b >= 0.0f) {
+        near_t = -b - s;
+        far_t = (b + s) > 0.0f ? c / (b + s) : -b + s;
+    } else {
+        far_t = -b + s;
+        near_t = (s - b) > 0.0f ? -c / (s - b) : -b - s;
+    }
+}
+
+}  // namespace skyhip

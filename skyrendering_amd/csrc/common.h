@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <cstdio>
 #include <string>
 
@@ -45,6 +47,8 @@ int require(bool cond, int code, const char* what);
 struct SkyhipContext {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux_stream = nullptr;  // fork/join partner for independent passes
+    hipEvent_t fork_event = nullptr, join_event = nullptr;
     hipEvent_t events[64] = {};
     int cu_count = 0;
     int wave_size = 64;

@@ -189,6 +189,13 @@ extern "C" int skyhip_create(int device, SkyhipContext** out_ctx) {
         delete ctx;
         return check_hip(e, "hipStreamCreateWithFlags");
     }
+    e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        skyhip_destroy(ctx);
+        return check_hip(e, "auxiliary stream / events");
+    }
     for (auto& ev : ctx->events) {
         e = hipEventCreate(&ev);
         if (e != hipSuccess) {
@@ -204,8 +211,12 @@ extern "C" int skyhip_destroy(SkyhipContext* ctx) {
     if (ctx == nullptr) return SKYHIP_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
     for (auto& ev : ctx->events)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+    if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SKYHIP_OK;

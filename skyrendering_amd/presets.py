@@ -96,3 +96,71 @@ def integrator_steps_per_lut_set(sizes=None, t_steps=40, ms_steps=30, sv_steps=3
     sw, sh = sizes["sky_view"]
     aw, ah, ad = sizes["aerial_perspective"]
     return (tw * th * n(t_steps) + mw * mh * 64 * n(ms_steps) + sw * sh * n(sv_steps) + aw * ah * ad * n(ap_steps))
+
+
+# ---------------------------------------------------------------------------
+# Volumetric-cloud raymarch workload: 1920x1080, one ray per pixel, 64 primary steps
+# (uMaxRaymarchSteps), 6 light steps, cloud layer 1.5-4 km, extinction 50 / km
+# (0.05 / m), camera at 1 km altitude looking along world +x pitched up 10 degrees,
+# sun elevation 20 degrees (zenith 70, azimuth 180).  Weather map 512^2 RG8 with
+# i.i.d. uniform coverage (PCG64 seed 2) and type (seed 3); detail 128^3 R8
+# Perlin-Worley from the noise generator with the bin/config.json detail parameters;
+# blue noise: a 128^2 uniform tile (PCG64 seed 11).  Everything not fixed above is
+# bin/config.json's volumetric_cloud_ object (VolumetricCloud.h:20-35 field names).
+# ---------------------------------------------------------------------------
+def cloud_material_reference():
+    """bin/config.json:132-175 (VolumetricCloudDefaultMaterial1)."""
+    from .cloud import NoiseCreateInfo, VolumetricCloudDefaultMaterial1
+    m = VolumetricCloudDefaultMaterial1(
+        detail_base=0.5260000228881836, detail_scale=1.8600000143051147,
+        base_density_threshold=0.5175999999046326, base_height_hardness=6.0, base_edge_hardness=6.0,
+        height_cut=0.8009999990463257, edge_cut=0.7170000076293945)
+    m.cloud_map_density = NoiseCreateInfo(0, 8, 0.20200000703334808, 0.8360000252723694)
+    m.cloud_map_height = NoiseCreateInfo(0, 7, 0.8519999980926514, 0.40700000524520874)
+    m.detail_perlin = NoiseCreateInfo(0, 7, 1.0, 1.0)
+    m.detail_worley = NoiseCreateInfo(0, 5, 0.12399999797344208, 0.8009999990463257)
+    m.displacement_perlin = NoiseCreateInfo(0, 6, 0.8009999990463257, 0.1589999943971634)
+    m.cloud_map_repeat_size = 94.34500122070313
+    m.detail_repeat_size = 5.97599983215332
+    m.displacement_repeat_size = 4.138999938964844
+    m.density = 25.0
+    m.lod_bias = 2.4230000972747803
+    m.wind_speed = 0.0
+    m.detail_wind_magnify = 1.0
+    return m
+
+
+def cloud_reference():
+    """bin/config.json:121-186 (volumetric_cloud_)."""
+    from .cloud import VolumetricCloud
+    return VolumetricCloud(
+        bottom_altitude=1.6759999990463257, thickness=3.3959999084472656, max_raymarch_distance=33.05799865722656,
+        max_raymarch_steps=143.4530029296875, max_visible_distance=2573.0,
+        env_color=(0.6372548937797546, 0.7652825117111206, 1.0), env_color_scale=0.2750000059604645,
+        sun_illuminance_scale=1.0, shadow_steps=5.0, shadow_distance=2.0, shadow_map_max_distance=48.26599884033203,
+        shadow_froxel_max_distance=30.63599967956543, sun_multiscattering_sigma_scale=0.30000001192092896,
+        env_multiscattering_sigma_scale=0.5, env_bottom_visibility=0.3919999897480011,
+        env_sun_height_curve_exp=0.8130000233650208)
+
+
+def cloud_workload():
+    cloud = cloud_reference()
+    cloud.bottom_altitude = 1.5
+    cloud.thickness = 2.5
+    cloud.max_raymarch_steps = 64.0
+    cloud.shadow_steps = 6.0
+    mat = cloud_material_reference()
+    mat.density = 50.0
+    return {"cloud": cloud, "material": mat, "camera": bench_camera(), "width": 1920, "height": 1080,
+            "sun_theta": 70.0, "sun_phi": 180.0, "weather_seeds": (2, 3), "blue_noise_seed": 11,
+            "blue_noise_size": 128, "atmosphere": earth_100km()}
+
+
+def uniform_weather_map(size=512, seeds=(2, 3)) -> np.ndarray:
+    cov = np.random.default_rng(seeds[0]).integers(0, 256, size=(size, size), dtype=np.uint8)
+    typ = np.random.default_rng(seeds[1]).integers(0, 256, size=(size, size), dtype=np.uint8)
+    return np.stack([cov, typ], axis=-1)
+
+
+def uniform_blue_noise(size=128, seed=11) -> np.ndarray:
+    return np.random.default_rng(seed).integers(0, 65536, size=(size, size), dtype=np.uint16)

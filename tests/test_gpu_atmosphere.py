@@ -127,3 +127,22 @@ def test_update_luts_rejects_bad_sizes(ctx):
     buf = ctx.alloc(16 * 4)
     rc = _abi.lib().skyhip_atmosphere_update_luts(ctx.handle, C.byref(p.to_c()), buf.ptr, 1, 64, buf.ptr, 32, 32)
     assert rc == _abi.ERR_INVALID_SIZE
+
+
+def test_lut_set_single_call_matches_oracle(ctx):
+    """skyhip_atmosphere_lut_set (T -> MS -> SV || AP on a forked stream) at the Earth LUT-set sizes."""
+    from skyrendering_amd.lutset import LutSet
+    atm_p, renders, cam, sizes = presets.earth_lut_workload()
+    ls = LutSet(ctx, sizes)
+    for r in (renders[0], renders[-1]):
+        ls.build(atm_p, r)
+        out = ls.download()
+        refT = oracle.transmittance_lut(ls.atm_block, 256, 64)
+        refMS = oracle.multiscattering_lut(ls.atm_block, out["T"], 32, 32)
+        refL, refTr = oracle.sky_view_lut(ls.atm_block, ls.render_block, out["T"], out["MS"], 192, 108)
+        apL, apT = oracle.aerial_perspective_lut(ls.atm_block, ls.render_block, out["T"], out["MS"], 32, 32, 32)
+        errs = {k: rel_err(a[..., :3], b[..., :3])[0] for k, a, b in [
+            ("T", out["T"], refT), ("MS", out["MS"], refMS), ("SV_L", out["SV_L"], refL),
+            ("SV_T", out["SV_T"], refTr), ("AP_L", out["AP_L"], apL), ("AP_T", out["AP_T"], apT)]}
+        print(f"LUT set theta={r.sun_direction_theta:.2f}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+        assert max(errs.values()) < TOL

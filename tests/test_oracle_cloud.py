@@ -1,0 +1,131 @@
+"""Pins the cloud raymarch oracle on the CPU with cases that reduce to closed forms.
+
+With the Minimal material (constant extinction sigma, VolumetricCloudMaterialMinimal.glsl)
+the light march is sigma * D * t_last^2 (t_last the last FP32 counter value <= 1), the
+primary march telescopes to sum_k T_k (1 - tr) = 1 - exp(-sigma * dist), and with unit
+LUTs the composite reduces to T_sun * phase * (1 - T).  Mip selection is pinned with a
+texture whose levels hold distinct constants.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+import oracle
+from skyrendering_amd import _abi, presets
+from skyrendering_amd.cloud import VolumetricCloud
+
+
+def _blocks(pitch=60.0, shadow_steps=6.0, width=33, height=19, max_distance=3.0):
+    cam = presets.bench_camera(pitch_deg=pitch, aspect=width / height)
+    cloud = VolumetricCloud(bottom_altitude=1.5, thickness=2.5, max_raymarch_distance=max_distance, max_raymarch_steps=64.0,
+                            max_visible_distance=2573.0, env_color=(0.0, 0.0, 0.0), env_color_scale=0.0,
+                            shadow_steps=shadow_steps, shadow_distance=2.0, shadow_froxel_max_distance=30.0)
+    sun = np.array([0.0, math.sin(math.radians(40)), math.cos(math.radians(40))], np.float32)
+    common, blk, _ = cloud.update(cam, 6360.0, sun, 203.0, 0)
+    return cam, common, blk
+
+
+def _unit_images(width, height):
+    T = np.ones((2, 2, 4), np.float32)
+    apl = np.zeros((2, 2, 2, 4), np.float32)
+    apt = np.ones((2, 2, 2, 4), np.float32)
+    bn = np.zeros((1, 1), np.uint16)
+    return T, apl, apt, bn
+
+
+def _hg(c, g):
+    return (1 - g * g) / (4 * math.pi * (1 + g * g - 2 * g * c) ** 1.5)
+
+
+def test_minimal_material_raymarch_closed_form():
+    width, height = 33, 19
+    cam, common, blk = _blocks(width=width, height=height)
+    sigma = float(np.float32(0.05))  # the block stores FP32
+    mat = _abi.CloudMaterial()
+    mat.kind = _abi.MATERIAL_MINIMAL
+    mat.minimal.uDensity = sigma
+    T, apl, apt, bn = _unit_images(width, height)
+    inputs = oracle.HostCloudInputs(mat, T=T, ap_l=apl, ap_t=apt, blue_noise=bn, image_size=(width, height))
+    atm = presets.earth_100km().buffer_data()
+    px = (height // 2) * width + width // 2
+    rgba, dist, steps = oracle.cloud_raymarch(atm, common, blk, inputs, width, [px])
+    # Independent geometry: the centre pixel's ray and its chord through the shell.
+    ivp = np.array(common.uInvMVP[:], np.float64).reshape(4, 4).T
+    u, v = (width // 2 + 0.5) / width, (height // 2 + 0.5) / height
+    p = ivp @ np.array([2 * u - 1, 2 * v - 1, 1.0, 1.0])
+    camp = np.array(common.uCameraPos[:], np.float64)
+    d = p[:3] / p[3] - camp
+    d /= np.linalg.norm(d)
+    R, r = 6360.0, 6360.0 + camp[2]
+    far = lambda Rp: -r * d[2] + math.sqrt(r * r * (d[2] ** 2 - 1) + Rp * Rp)
+    chord = far(R + 4.0) - far(R + 1.5)
+    t_expected = math.exp(-sigma * chord)
+    sun = np.array(common.uSunDirection[:], np.float64)
+    c = float(np.dot(sun, d))
+    w = math.exp(-float(np.float32(0.3)) * sigma)
+    phase = _hg(c, -0.15) * 2.16 * (1 - w) + _hg(c, 0.85) * w
+    t_sun = math.exp(-sigma * 2.0)  # 6 light steps: t reaches exactly 1.0
+    # num_steps = uint(max(steps * dist / max_dist, 1)) with dist = min(chord, max_dist)
+    assert steps[0] == int(64 * min(chord, 3.0) / 3.0)
+    assert abs(rgba[0, 3] - t_expected) < 1e-9
+    np.testing.assert_allclose(rgba[0, :3], t_sun * phase * (1 - t_expected), rtol=1e-9)
+
+
+def test_shadow_loop_float_counter():
+    """`for (float t = 1/n; t <= 1.0; t += 1/n)` in FP32: n = 5, 6 reach t = 1 exactly; n = 10 stops at t = 0.9
+    (the accumulated counter overshoots 1.0), so the optical depth is sigma * D * 0.81."""
+    sigma = float(np.float32(0.3))
+    mat = _abi.CloudMaterial()
+    mat.kind = _abi.MATERIAL_MINIMAL
+    mat.minimal.uDensity = sigma
+    for n, t_last in [(5, 1.0), (6, 1.0), (10, 0.9)]:
+        cam, common, blk = _blocks(shadow_steps=float(n))
+        got = oracle.lib().oracle_cloud_shadow_f64(C.byref(common), C.byref(blk), C.byref(mat), 0.0, 0.0, 2.0)
+        t32 = np.float32(t_last)
+        assert abs(got - math.exp(-sigma * 2.0 * float(t32) * float(t32))) < 1e-6, (n, got)
+
+
+def test_mip_level_selection_follows_gl_rule():
+    """Cloud-map levels hold distinct constants, so the sampled coverage reveals the level:
+    lambda = log2(k_lod * distance) + bias; lambda <= 0.5 -> level 0 (LINEAR), else
+    level ceil(lambda + 0.5) - 1 (NEAREST_MIPMAP_NEAREST, GL 4.6 section 8.14.3)."""
+    size, levels = 64, 7
+    chain = []
+    for l in range(levels):
+        s = max(1, size >> l)
+        lvl = np.zeros((s, s, 2), np.uint8)
+        lvl[..., 0] = 200 + 5 * l  # coverage
+        chain.append(lvl.reshape(-1))
+    cloud_map = np.concatenate(chain)
+    detail = np.zeros(sum(max(1, 8 >> l) ** 3 for l in range(4)), np.uint8)
+    mat = _abi.CloudMaterial()
+    mat.kind = _abi.MATERIAL_DEFAULT1
+    mat.common.uCloudMapSampleInfo.frequency = 0.1
+    mat.common.uCloudMapSampleInfo.k_lod = 1.0
+    mat.common.uDetailSampleInfo.frequency = 0.1
+    mat.common.uDetailSampleInfo.k_lod = 1.0
+    mat.common.uLodBias = 0.0
+    mat.common.uDensity = 1.0
+    mat.material1.uBaseDensityThreshold = 0.0
+    mat.material1.uBaseEdgeHardness = 1.0
+    mat.material1.uBaseHeightHardness = 100.0
+    mat.material1.uDetailBase = 0.0
+    mat.material1.uDetailScale = 0.0
+    mat.cloud_map_size, mat.cloud_map_levels = size, levels
+    mat.detail_size, mat.detail_levels = 8, 4
+    cam, common, blk = _blocks()
+    inputs = oracle.HostCloudInputs(mat, cloud_map=cloud_map, detail=detail, T=np.ones((2, 2, 4), np.float32),
+                                    ap_l=np.zeros((2, 2, 2, 4), np.float32), ap_t=np.ones((2, 2, 2, 4), np.float32),
+                                    blue_noise=np.zeros((1, 1), np.uint16), image_size=(1, 1))
+    camp = np.array(common.uCameraPos[:], np.float64)
+    pos = camp + np.array([2.0, 0.0, 1.75])  # inside the layer, fixed distance d0 from the camera
+    d0 = float(np.linalg.norm(pos - camp))
+    want_h01 = (math.sqrt(pos[0] ** 2 + pos[1] ** 2 + (pos[2] + 6360.0) ** 2) - 6360.0 - 1.5) / 2.5
+    for lam, level in [(0.0, 0), (0.49, 0), (0.51, 1), (1.49, 1), (1.51, 2), (5.2, 5), (40.0, 6)]:
+        # lambda = log2(k_lod * d0): level 0 (LINEAR) for lambda <= 0.5, else ceil(lambda + 0.5) - 1, clamped.
+        inputs.material.common.uCloudMapSampleInfo.k_lod = 2.0 ** lam / d0
+        got = oracle.lib().oracle_cloud_sigma_t_f64(C.byref(common), C.byref(blk), C.byref(inputs.material), *pos)
+        # density = cov * clamp((1 - h01) * 100) * 1.0 * h01 with the detail term scaled to 0.
+        expected_cov = (200 + 5 * level) / 255.0
+        assert abs(got - expected_cov * want_h01) < 1e-9, (lam, level, got)

@@ -1,0 +1,101 @@
+"""Pins the noise oracle (shaders/Base/Noise.glsl, NoiseGen.comp) on the CPU.
+
+Integer hashes: known-answer values worked by hand from the published definitions
+(Thomas Wang's 32-bit hash and the PCG RXS-M-XS hash as given in
+shaders/Base/Noise.glsl:1-15), with every intermediate value written out below.
+Noise functions: lattice zeros, periodicity, range, and a brute-force Worley search
+over every cell of a small lattice.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from skyrendering_amd import _abi
+
+# WangHash(0): (0 ^ 61) ^ (0 >> 16) = 61; * 9 = 549; 549 ^ (549 >> 4 = 34) = 519;
+#   519 * 0x27d4eb2d mod 2^32 = 3232286779; ^ (3232286779 >> 15 = 98641) = 3232319850.
+# WangHash(1): (1 ^ 61) = 60; * 9 = 540; 540 ^ 33 = 573; * 0x27d4eb2d = 663905209;
+#   ^ (663905209 >> 15 = 20260) = 663891101.
+# PCGHash(0): state = 2891336453; state >> 28 = 10, so shift 14: state >> 14 = 176473;
+#   176473 ^ state = 2891250268; * 277803737 mod 2^32 = 129708028 = word; (word >> 22 = 30) ^ word = 129708002.
+# PCGHash(1): state = 747796405 + 2891336453 = 3639132858; state >> 28 = 13, so shift 17: 27764;
+#   27764 ^ state = 3639127758; * 277803737 mod 2^32 = 2831083678; (word >> 22 = 674) ^ word = 2831084092.
+HASH_KATS = {"wang": {0: 3232319850, 1: 663891101, 12345: 232713235},
+             "pcg": {0: 129708002, 1: 2831084092, 12345: 4099845390}}
+
+
+def test_hash_known_answers():
+    L = oracle.lib()
+    for x, want in HASH_KATS["wang"].items():
+        assert L.oracle_wang_hash(x) == want
+    for x, want in HASH_KATS["pcg"].items():
+        assert L.oracle_pcg_hash(x) == want
+
+
+def test_perlin_zero_on_lattice_and_periodic():
+    L = oracle.lib()
+    for f in [3, 7, 8]:
+        for (i, j, k) in [(0, 0, 0), (1, 2, 0), (2, 1, 1)]:
+            assert L.oracle_perlin_noise(i / f, j / f, k / f, f, 0) == 0.0
+        rng = np.random.default_rng(f)
+        for p in rng.random((20, 3)):
+            a = L.oracle_perlin_noise(*p, f, 5)
+            b = L.oracle_perlin_noise(p[0] + 1.0, p[1], p[2] + 1.0, f, 5)
+            assert abs(a - b) < 1e-12
+            assert -1.5 <= a <= 1.5
+
+
+def test_worley_matches_brute_force_search():
+    """Min distance to one hashed feature point per cell, searched over *every* cell of the
+    periodic lattice (not just the 27 neighbours the shader visits)."""
+    L = oracle.lib()
+    wang = L.oracle_wang_hash
+    f, seed = 4, 9
+    feats = []
+    for gz in range(f):
+        for gy in range(f):
+            for gx in range(f):
+                r = [wang((seed + wang((gx + wang((gy + wang((gz + c) & 0xFFFFFFFF)) & 0xFFFFFFFF)) & 0xFFFFFFFF))
+                          & 0xFFFFFFFF) for c in range(3)]
+                feats.append([gx + r[0] / 2**32, gy + r[1] / 2**32, gz + r[2] / 2**32])
+    feats = np.array(feats)
+    rng = np.random.default_rng(0)
+    for p in rng.random((30, 3)):
+        q = p * f
+        best = np.inf
+        for off in np.array(np.meshgrid([-f, 0, f], [-f, 0, f], [-f, 0, f])).reshape(3, -1).T:
+            best = min(best, np.min(np.linalg.norm(feats + off - q, axis=1)))
+        got = L.oracle_worley_noise(*p, f, seed)
+        assert abs(got - best) < 1e-9, (p, got, best)
+        assert 0 <= got <= math.sqrt(3)
+
+
+def test_fbm_textures_range_and_degenerate_remap():
+    # Degenerate remap (min == max == 1, bin/config.json:153-156): every octave maps to 0.
+    perlin = _abi.NoiseCreateInfo(0, 7, 1.0, 1.0)
+    worley = _abi.NoiseCreateInfo(0, 5, 0.124, 0.801)
+    texels = np.arange(0, 32 ** 3, 997)
+    got = oracle.detail_texels(perlin, worley, 32, texels)
+    only_worley = oracle.detail_texels(_abi.NoiseCreateInfo(0, 7, 0.0, 0.0), worley, 32, texels)
+    # RemapFrom01(0, w, 1) = w, so the degenerate-Perlin volume equals the pure-Worley FBM.
+    # (x0 == x1 == 0: noise in [0, 1] > 0 -> +inf -> clamps to 1, except 0/0 at exactly 0.)
+    assert got.dtype == np.uint8
+    assert np.all(got <= only_worley)
+
+
+def test_mipmap_box_filter():
+    base = np.full((8, 8, 2), 77, np.uint8)
+    chain = np.zeros(8 * 8 * 2 + 4 * 4 * 2 + 2 * 2 * 2 + 2, np.uint8)
+    chain[:128] = base.reshape(-1)
+    out = oracle.generate_mipmaps(chain, 2, 2, 8, 8, 1, 4)
+    assert np.all(out == 77)
+    # 2x2 block [0, 1, 2, 3] -> (6 + 2) / 4 = 2 (round half up), 3D block of 0..7 -> (28 + 4) / 8 = 4.
+    c2 = np.zeros(4 + 1, np.uint8)
+    c2[:4] = [0, 1, 2, 3]
+    assert oracle.generate_mipmaps(c2, 2, 1, 2, 2, 1, 2)[4] == 2
+    c3 = np.zeros(8 + 1, np.uint8)
+    c3[:8] = np.arange(8)
+    assert oracle.generate_mipmaps(c3, 3, 1, 2, 2, 2, 2)[8] == 4
