@@ -489,7 +489,12 @@ int skyhip_generate_mipmaps(SkyhipContext* ctx, uint8_t* texture, int32_t dims, 
  *   out_rgba16f:    RGBA16F (luminance, transmittance); out_distance: R32F mean distance;
  *   work_counters:  NULL, or device uint64[4] to which a counting kernel variant adds the
  *                   executed primary march steps, density evaluations (primary + light march),
- *                   detail-texture fetches and in-cloud steps (sigma >= 1e-5); leave NULL when timing.
+ *                   detail-texture fetches and in-cloud steps (sigma >= 1e-5); leave NULL when timing;
+ *   pixel_steps:    NULL, or (with work_counters) device uint32[2 * width * height] receiving,
+ *                   per pixel, the executed steps and then the scheduled steps
+ *                   uint(max(steps * dist / max_dist, 1)), each as segment 0 in bits 0-15 and
+ *                   segment 1 in bits 16-31, so a checker can replay the discrete decisions
+ *                   (step count, early exit at T < 0.01).
  * The cloud shadow froxel visibility (VolumetricCloudShadowInterface.glsl:10-13) is 1:
  * the cloud shadow passes are not part of this library. */
 int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
@@ -499,7 +504,16 @@ int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* 
                           const float* ap_luminance_rgba, const float* ap_transmittance_rgba, int32_t ap_width,
                           int32_t ap_height, int32_t ap_depth, const uint16_t* blue_noise, int32_t blue_noise_size,
                           const float* depth, const float* shading_index, void* out_rgba16f, float* out_distance,
-                          int32_t width, int32_t height, unsigned long long* work_counters);
+                          int32_t width, int32_t height, unsigned long long* work_counters,
+                          uint32_t* pixel_steps);
+
+/* The material plugin's density, SampleSigmaT(pos, CalHeight01(pos)) (the shader-side
+ * contract of IVolumetricCloudMaterial, VolumetricCloudRender.comp:96 and
+ * VolumetricCloudCommon.glsl:36-40), at `count` local-frame points (float xyz triples,
+ * device memory).  out_sigma_t: device float[count], extinction per km. */
+int skyhip_cloud_sample_sigma_t(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                                const SkyhipCloudMaterial* material, const float* positions_xyz, int32_t count,
+                                float* out_sigma_t);
 
 #ifdef __cplusplus
 } /* extern "C" */

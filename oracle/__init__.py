@@ -154,6 +154,12 @@ _PROTO.update({
     "oracle_cloud_raymarch_f32": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
                                          P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32, vp,
                                          C.c_long, vp, vp, vp]),
+    "oracle_cloud_raymarch_forced_f64": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
+                                                P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32,
+                                                vp, C.c_long, vp, vp, vp, vp, vp, vp]),
+    "oracle_cloud_trace_f64": (i32, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
+                                     P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32, i32, vp,
+                                     i32]),
     "oracle_cloud_sigma_t_f64": (f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudBufferData),
                                        P(_abi.CloudMaterial), f64, f64, f64]),
     "oracle_cloud_shadow_f64": (f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudBufferData),
@@ -225,3 +231,20 @@ def cloud_raymarch(atm, common, cloud, inputs: HostCloudInputs, width, pixels, p
         C.byref(atm), C.byref(common), C.byref(cloud), C.byref(inputs.material), C.byref(inputs.images), width,
         _ptr(px), len(px), _ptr(rgba), _ptr(dist), _ptr(steps))
     return rgba, dist, steps
+
+
+def cloud_raymarch_forced(atm, common, cloud, inputs: HostCloudInputs, width, pixels, forced_steps):
+    """Checker mode: replays another implementation's per-pixel step decisions
+    (uint32[n, 2]: executed, scheduled; each segment 0 | segment 1 << 16) and returns
+    (rgba, distance, steps, early-exit margin, step-count margin)."""
+    px = np.ascontiguousarray(pixels, dtype=np.int32)
+    fs = np.ascontiguousarray(forced_steps, dtype=np.uint32).reshape(-1, 2)
+    rgba = np.empty((len(px), 4), np.float64)
+    dist = np.empty(len(px), np.float64)
+    steps = np.empty(len(px), np.int32)
+    margin = np.empty(len(px), np.float64)
+    step_margin = np.empty(len(px), np.float64)
+    lib().oracle_cloud_raymarch_forced_f64(C.byref(atm), C.byref(common), C.byref(cloud), C.byref(inputs.material),
+                                           C.byref(inputs.images), width, _ptr(px), len(px), _ptr(fs), _ptr(rgba),
+                                           _ptr(dist), _ptr(steps), _ptr(margin), _ptr(step_margin))
+    return rgba, dist, steps, margin, step_margin

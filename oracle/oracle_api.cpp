@@ -75,13 +75,19 @@ template <class S, class O>
 void cloud_raymarch(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
                            const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
                            const CloudImages* images, int width, const int32_t* pixels, long count, O* out_rgba,
-                           O* out_dist, int32_t* out_steps) {
+                           O* out_dist, int32_t* out_steps, const uint32_t* forced_steps = nullptr,
+                           double* out_margin = nullptr, double* out_step_margin = nullptr) {
     Atmosphere<S> A(*atm);
     CloudCtx<S> Cc(common, cloud, material);
 #pragma omp parallel for schedule(dynamic, 16)
     for (long i = 0; i < count; ++i) {
         int p = pixels[i];
-        CloudPixelOut<S> o = CloudRaymarchPixel<S>(A, Cc, *images, p % width, p / width);
+        S margin = 0, step_margin = 0;
+        CloudPixelOut<S> o = CloudRaymarchPixel<S>(A, Cc, *images, p % width, p / width,
+                                                   forced_steps ? forced_steps + 2 * i : nullptr, &margin, nullptr, 0,
+                                                   &step_margin);
+        if (out_margin) out_margin[i] = double(margin);
+        if (out_step_margin) out_step_margin[i] = double(step_margin);
         for (int k = 0; k < 4; ++k) out_rgba[i * 4 + k] = O(o.rgba[k]);
         out_dist[i] = O(o.distance);
         if (out_steps) out_steps[i] = o.steps;
@@ -253,6 +259,26 @@ void oracle_cloud_raymarch_f32(const SkyhipAtmosphereBufferData* atm, const Skyh
                                const CloudImages* images, int width, const int32_t* pixels, long count,
                                float* out_rgba, float* out_dist, int32_t* out_steps) {
     cloud_raymarch<float>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps);
+}
+
+// Checker mode: replays per-pixel step decisions (forced_steps[2 i], [2 i + 1] for
+// pixels[i]) and reports each pixel's decision margins (see CloudRaymarchPixel).
+void oracle_cloud_raymarch_forced_f64(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
+                                      const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
+                                      const CloudImages* images, int width, const int32_t* pixels, long count,
+                                      const uint32_t* forced_steps, double* out_rgba, double* out_dist,
+                                      int32_t* out_steps, double* out_margin, double* out_step_margin) {
+    cloud_raymarch<double>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps,
+                           forced_steps, out_margin, out_step_margin);
+}
+
+// Per-step trace of one pixel (8 doubles per step, see CloudRaymarchPixel); returns the step count.
+int oracle_cloud_trace_f64(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
+                           const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
+                           const CloudImages* images, int px, int py, double* trace, int trace_max) {
+    Atmosphere<double> A(*atm);
+    CloudCtx<double> Cc(common, cloud, material);
+    return CloudRaymarchPixel<double>(A, Cc, *images, px, py, nullptr, nullptr, trace, trace_max).steps;
 }
 
 // One density evaluation (for material tests).

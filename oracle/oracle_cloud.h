@@ -184,9 +184,22 @@ struct CloudPixelOut {
 };
 
 // VolumetricCloudRender.comp:139-210 for output pixel (px, py).
+//
+// Checker mode: `forced` (non-null) holds another implementation's discrete
+// decisions for this pixel: forced[0] the executed steps, forced[1] the scheduled
+// steps uint(max(N dist / dist_max, 1)), each segment 0 in bits 0-15 and
+// segment 1 in bits 16-31.  The march then uses those step counts (step size
+// dist / scheduled) instead of deciding them itself, and reports how far the
+// replayed decisions are from this evaluation's own: *margin = the largest
+// |T - 0.01| / 0.01 over early exits it would have taken the other way, and
+// *step_margin = the largest |x - round(x)| of x = N dist / dist_max over
+// segments whose scheduled count differs (0 when every decision agrees).
+// `trace` (optional): per executed step, 8 values (t, x, y, z, height01, sigma_t,
+// transmittance after the step, step size), at most trace_max steps.
 template <class S>
 CloudPixelOut<S> CloudRaymarchPixel(const Atmosphere<S>& A, const CloudCtx<S>& C, const CloudImages& im, int px,
-                                    int py) {
+                                    int py, const uint32_t* forced = nullptr, S* margin = nullptr,
+                                    double* trace = nullptr, int trace_max = 0, S* step_margin = nullptr) {
     const SkyhipCloudCommonBufferData& cb = *C.c;
     const SkyhipCloudBufferData& b = *C.b;
     int pcx = px, pcy = py;
@@ -240,8 +253,7 @@ CloudPixelOut<S> CloudRaymarchPixel(const Atmosphere<S>& A, const CloudCtx<S>& C
     S dist = t2[0] - t1[0];
     dist = std::fmin(dist, S(b.uMaxRaymarchDistance));
     // uint(max(x, 1.0)): truncation toward zero
-    uint32_t num_steps = uint32_t(std::fmax(S(b.uMaxRaymarchSteps) * (dist / S(b.uMaxRaymarchDistance)), S(1)));
-    ctx.step_size = dist / S(num_steps);
+    uint32_t num_steps = 0;  // scheduled below (needs the segment lambdas)
     ctx.transmittance = 1;
     ctx.transmittance_sum = 0;
     ctx.weighted_t_sum = 0;
@@ -249,28 +261,58 @@ CloudPixelOut<S> CloudRaymarchPixel(const Atmosphere<S>& A, const CloudCtx<S>& C
     S noise = S(im.blue_noise[(py & (im.bn_size - 1)) * im.bn_size + (px & (im.bn_size - 1))]) / S(65535);
     S jitter = fract(noise + S(cb.uFrameID) * S(0.61803398875));
     int steps = 0;
+    S worst = 0;
+    // One march segment: the shader's loop, or the forced count in checker mode.
+    S worst_steps = 0;
+    // Scheduled step count of a segment; in checker mode the replayed one.
+    auto schedule = [&](S seg_dist, int seg) -> uint32_t {
+        S x = S(b.uMaxRaymarchSteps) * (seg_dist / S(b.uMaxRaymarchDistance));
+        uint32_t own = uint32_t(std::fmax(x, S(1)));  // uint(max(x, 1.0)): truncation
+        if (!forced) return own;
+        uint32_t replay = (forced[1] >> (16 * seg)) & 0xFFFFu;
+        if (replay != own) worst_steps = std::fmax(worst_steps, std::fabs(x - std::round(x)));
+        return replay == 0 ? own : replay;
+    };
+    auto segment = [&](uint32_t n_steps, int seg) {
+        uint32_t forced_n = forced ? ((forced[0] >> (16 * seg)) & 0xFFFFu) : 0u;
+        uint32_t k = 0;
+        for (uint32_t cnt = n_steps; cnt != 0; cnt--, ctx.t += ctx.step_size) {
+            ctx.pos = C.cam + view_dir * ctx.t;  // UpdateContext (:90-93)
+            ctx.height01 = C.CalHeight01(ctx.pos);
+            RayMarchStep(C, ctx);
+            if (trace && steps < trace_max) {
+                double* tr = trace + 8 * steps;
+                tr[0] = double(ctx.t), tr[1] = double(ctx.pos.x), tr[2] = double(ctx.pos.y), tr[3] = double(ctx.pos.z);
+                tr[4] = double(ctx.height01), tr[5] = double(C.SampleSigmaT(ctx.pos, ctx.height01));
+                tr[6] = double(ctx.transmittance), tr[7] = double(ctx.step_size);
+            }
+            ++steps;
+            ++k;
+            bool stop_here = ctx.transmittance < S(0.01);  // kMinTransmittance
+            if (!forced) {
+                if (stop_here) break;
+                continue;
+            }
+            bool forced_stop = k >= forced_n;
+            if (stop_here != forced_stop && !(forced_stop && cnt == 1))
+                worst = std::fmax(worst, std::fabs(ctx.transmittance - S(0.01)) / S(0.01));
+            if (forced_stop) break;
+        }
+    };
+    num_steps = schedule(dist, 0);
+    ctx.step_size = dist / S(num_steps);
     ctx.t = t1[0] + ctx.step_size * jitter;
-    for (uint32_t cnt = num_steps; cnt != 0; cnt--, ctx.t += ctx.step_size) {
-        ctx.pos = C.cam + view_dir * ctx.t;  // UpdateContext (:90-93)
-        ctx.height01 = C.CalHeight01(ctx.pos);
-        RayMarchStep(C, ctx);
-        ++steps;
-        if (ctx.transmittance < S(0.01)) break;  // kMinTransmittance
-    }
+    segment(num_steps, 0);
     S dist1 = t2[1] - t1[1];
     if (dist1 > S(0)) {
         dist1 = std::fmin(dist1, S(b.uMaxRaymarchDistance));
-        uint32_t num_steps1 = uint32_t(std::fmax(S(b.uMaxRaymarchSteps) * (dist1 / S(b.uMaxRaymarchDistance)), S(1)));
+        uint32_t num_steps1 = schedule(dist1, 1);
         ctx.step_size = dist1 / S(num_steps1);
         ctx.t = t1[1] + ctx.step_size * jitter;
-        for (uint32_t cnt = num_steps1; cnt != 0; cnt--, ctx.t += ctx.step_size) {
-            ctx.pos = C.cam + view_dir * ctx.t;
-            ctx.height01 = C.CalHeight01(ctx.pos);
-            RayMarchStep(C, ctx);
-            ++steps;
-            if (ctx.transmittance < S(0.01)) break;
-        }
+        segment(num_steps1, 1);
     }
+    if (margin) *margin = worst;
+    if (step_margin) *step_margin = worst_steps;
     CloudPixelOut<S> out;
     out.steps = steps;
     S average_t = ctx.weighted_t_sum == S(0) ? frag_dist : ctx.weighted_t_sum / ctx.transmittance_sum;

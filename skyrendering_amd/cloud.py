@@ -29,7 +29,22 @@ _abi.register("skyhip_cloud_raymarch", C.c_int, [
     C.c_void_p, C.POINTER(_abi.AtmosphereBufferData), C.POINTER(_abi.CloudCommonBufferData),
     C.POINTER(_abi.CloudBufferData), C.POINTER(_abi.CloudMaterial), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
     C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
-    C.c_void_p, C.c_int32, C.c_int32, C.c_void_p])
+    C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p])
+
+
+_abi.register("skyhip_cloud_sample_sigma_t", C.c_int, [C.c_void_p, C.POINTER(_abi.CloudCommonBufferData),
+                                                       C.POINTER(_abi.CloudMaterial), C.c_void_p, C.c_int32,
+                                                       C.c_void_p])
+
+
+def sample_sigma_t(ctx: Context, common, material: _abi.CloudMaterial, positions: np.ndarray) -> np.ndarray:
+    """The material's SampleSigmaT at local-frame points (n x 3), evaluated on the device."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    dpos = ctx.upload(pos)
+    out = ctx.alloc(max(4 * len(pos), 4))
+    _abi.check(_abi.lib().skyhip_cloud_sample_sigma_t(ctx.handle, C.byref(common), C.byref(material), dpos.ptr,
+                                                      len(pos), out.ptr))
+    return out.download(np.float32, (len(pos),))
 
 
 def mip_levels(w, h=1, d=1) -> int:
@@ -279,10 +294,11 @@ def cloud_raymarch(ctx: Context, atmosphere_block, common, cloud, material: _abi
                    transmittance: DeviceBuffer, t_size, ap_l: DeviceBuffer, ap_t: DeviceBuffer, ap_size,
                    blue_noise: DeviceBuffer, blue_noise_size, out_rgba16f: DeviceBuffer, out_distance: DeviceBuffer,
                    width, height, depth: DeviceBuffer | None = None, shading_index: DeviceBuffer | None = None,
-                   work_counters: DeviceBuffer | None = None):
+                   work_counters: DeviceBuffer | None = None, pixel_steps: DeviceBuffer | None = None):
     """skyhip_cloud_raymarch: the VolumetricCloud render pass."""
     _abi.check(_abi.lib().skyhip_cloud_raymarch(
         ctx.handle, C.byref(atmosphere_block), C.byref(common), C.byref(cloud), C.byref(material), transmittance.ptr,
         t_size[0], t_size[1], ap_l.ptr, ap_t.ptr, ap_size[0], ap_size[1], ap_size[2], blue_noise.ptr, blue_noise_size,
         depth.ptr if depth is not None else None, shading_index.ptr if shading_index is not None else None,
-        out_rgba16f.ptr, out_distance.ptr, width, height, work_counters.ptr if work_counters is not None else None))
+        out_rgba16f.ptr, out_distance.ptr, width, height, work_counters.ptr if work_counters is not None else None,
+        pixel_steps.ptr if pixel_steps is not None else None))

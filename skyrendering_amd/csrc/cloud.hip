@@ -29,6 +29,7 @@ struct CloudArgs {
     float* out_distance;
     int width, height;
     unsigned long long* counters;  // counting variant: steps, density evals, detail fetches, cloud steps
+    uint32_t* pixel_steps;  // counting variant, optional: per pixel {executed steps, scheduled steps}, seg0 | seg1 << 16
 };
 
 // Trilinear CLAMP_TO_EDGE fetch of an RGBA32F 3D LUT at texel-space coordinates.
@@ -161,22 +162,32 @@ __global__ __launch_bounds__(256) void cloud_raymarch_kernel(CloudArgs a) {
         float jitter = noise + cb.uFrameID * 0.61803398875f;
         jitter = jitter - floorf(jitter);
         MarchAcc acc{1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t seg_steps[2] = {0u, 0u}, seg_sched[2] = {0u, 0u};
         for (int seg = 0; seg < 2; ++seg) {
             float dist = t2[seg] - t1[seg];
             if (seg == 1 && !(dist > 0.0f)) break;
             dist = fminf(dist, a.cloud.uMaxRaymarchDistance);
             const uint32_t n = uint32_t(fmaxf(a.cloud.uMaxRaymarchSteps * (dist / a.cloud.uMaxRaymarchDistance), 1.0f));
             const float step_size = dist / float(n);
-            float t = t1[seg] + step_size * jitter;
-            for (uint32_t cnt = n; cnt != 0; --cnt, t += step_size) {
+            // Sample k sits at t1 + (k + jitter) * step_size; computing it directly instead of
+            // accumulating `t += step_size` keeps FP32 rounding from drifting along the ray.
+            const float t_start = t1[seg] + step_size * jitter;
+            if (kCount) seg_sched[seg] = n;
+            for (uint32_t k = 0; k < n; ++k) {
+                const float t = fmaf(float(k), step_size, t_start);
                 const F3 pos = S.cam + view * t;
                 const float h01 = height01(S, pos);
                 march_step<kMat, kCount>(a, S, pos, t, h01, step_size, cos_sun_view, acc, &wc);
                 ++executed;
+                if (kCount) ++seg_steps[seg];
                 if (acc.transmittance < 0.01f) break;  // kMinTransmittance
             }
         }
 
+        if (kCount && a.pixel_steps) {
+            a.pixel_steps[2 * (py * a.width + px)] = seg_steps[0] | (seg_steps[1] << 16);
+            a.pixel_steps[2 * (py * a.width + px) + 1] = seg_sched[0] | (seg_sched[1] << 16);
+        }
         const float average_t = acc.weighted_t_sum == 0.0f ? frag_dist : acc.weighted_t_sum / acc.transmittance_sum;
         a.out_distance[py * a.width + px] = average_t;
         const F3 ap = S.cam + view * average_t;
@@ -347,7 +358,8 @@ extern "C" int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereB
                                      int32_t ap_width, int32_t ap_height, int32_t ap_depth,
                                      const uint16_t* blue_noise, int32_t blue_noise_size, const float* depth,
                                      const float* shading_index, void* out_rgba16f, float* out_distance,
-                                     int32_t width, int32_t height, unsigned long long* work_counters) {
+                                     int32_t width, int32_t height, unsigned long long* work_counters,
+                                     uint32_t* pixel_steps) {
     SKYHIP_CHECK(check_ctx(ctx));
     SKYHIP_CHECK(validate_atmosphere(atmosphere));
     if (!common || !cloud) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "cloud blocks are NULL");
@@ -391,5 +403,61 @@ extern "C" int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereB
     a.out_distance = out_distance;
     a.width = width, a.height = height;
     a.counters = work_counters;
+    a.pixel_steps = work_counters ? pixel_steps : nullptr;
     return work_counters ? launch_raymarch<true>(ctx->stream, a) : launch_raymarch<false>(ctx->stream, a);
+}
+
+// ---------------------------------------------------------------------------
+// The material plugin's SampleSigmaT at caller-given points (the contract of
+// IVolumetricCloudMaterial's shader side, VolumetricCloudRender.comp:96).
+// ---------------------------------------------------------------------------
+namespace skyhip {
+template <int kMat>
+__global__ __launch_bounds__(256) void sample_sigma_t_kernel(MaterialDev mat, SkyhipCloudCommonBufferData cb,
+                                                             const float* __restrict__ xyz, int n,
+                                                             float* __restrict__ out) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    Shell S;
+    S.cam = f3(cb.uCameraPos);
+    S.sun = f3(cb.uSunDirection);
+    S.R = cb.uEarthRadius;
+    S.hb = cb.uBottomAltitude;
+    S.ht = cb.uTopAltitude;
+    S.inv_thickness = 1.0f / (S.ht - S.hb);
+    F3 p = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    out[i] = sample_sigma_t<kMat>(mat, p, height01(S, p), S.cam);
+}
+}  // namespace skyhip
+
+extern "C" int skyhip_cloud_sample_sigma_t(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                                           const SkyhipCloudMaterial* material, const float* positions_xyz,
+                                           int32_t count, float* out_sigma_t) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!common || !positions_xyz || !out_sigma_t) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count < 0) return fail(SKYHIP_ERR_INVALID_SIZE, "negative count");
+    if (count == 0) return SKYHIP_OK;
+    MaterialDev m;
+    SKYHIP_CHECK(build_material(material, m));
+    dim3 grid(div_up(count, 256));
+    switch (m.kind) {
+        case SKYHIP_MATERIAL_DEFAULT0:
+            hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_DEFAULT0>, grid, dim3(256), 0, ctx->stream, m,
+                               *common, positions_xyz, count, out_sigma_t);
+            break;
+        case SKYHIP_MATERIAL_DEFAULT1:
+            hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_DEFAULT1>, grid, dim3(256), 0, ctx->stream, m,
+                               *common, positions_xyz, count, out_sigma_t);
+            break;
+        case SKYHIP_MATERIAL_MINIMAL:
+            hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_MINIMAL>, grid, dim3(256), 0, ctx->stream, m,
+                               *common, positions_xyz, count, out_sigma_t);
+            break;
+        default:
+            hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_VOXEL>, grid, dim3(256), 0, ctx->stream, m,
+                               *common, positions_xyz, count, out_sigma_t);
+            break;
+    }
+    SKYHIP_LAUNCH_CHECK("sample_sigma_t_kernel");
+    return SKYHIP_OK;
 }

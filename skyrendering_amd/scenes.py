@@ -63,6 +63,7 @@ class CloudScene:
         self.out_rgba = ctx.alloc(width * height * 8)
         self.out_dist = ctx.alloc(width * height * 4)
         self.counter = ctx.alloc(32)
+        self.pixel_steps = ctx.alloc(width * height * 8)
 
     def run(self, count_steps=False, depth=None, shading_index=None):
         if count_steps:
@@ -75,7 +76,8 @@ class CloudScene:
                        r.aerial_perspective_transmittance_texture, (aw, ah, r.init.aerial_perspective_lut_depth),
                        self.blue_noise, self.blue_noise_host.shape[0], self.out_rgba, self.out_dist, self.width,
                        self.height, depth=depth, shading_index=shading_index,
-                       work_counters=self.counter if count_steps else None)
+                       work_counters=self.counter if count_steps else None,
+                       pixel_steps=self.pixel_steps if count_steps else None)
 
     def executed_steps(self) -> int:
         return int(self.counter.download(np.uint64, (4,))[0])
@@ -86,6 +88,11 @@ class CloudScene:
         self.run(count_steps=True, **kw)
         c = self.counter.download(np.uint64, (4,))
         return {"steps": int(c[0]), "evals": int(c[1]), "detail": int(c[2]), "cloud_steps": int(c[3])}
+
+    def steps_per_pixel(self):
+        """Per-pixel step records of the last counting run, uint32[n, 2]: executed and
+        scheduled steps, each packed segment 0 | segment 1 << 16."""
+        return self.pixel_steps.download(np.uint32, (self.height * self.width, 2))
 
     def image(self):
         rgba = self.out_rgba.download(np.float16, (self.height, self.width, 4)).astype(np.float32)
