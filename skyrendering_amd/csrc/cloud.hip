@@ -30,6 +30,12 @@ struct CloudArgs {
     int width, height;
     unsigned long long* counters;  // counting variant: steps, density evals, detail fetches, cloud steps
     uint32_t* pixel_steps;  // counting variant, optional: per pixel {executed steps, scheduled steps}, seg0 | seg1 << 16
+    // Light-march schedule of SampleShadow (VolumetricCloudRender.comp:98-114), evaluated once on
+    // the host with the shader's FP32 counter: sample k sits at pos + D sun * light_offset[k] and
+    // weighs D * delta_t[k].  light_steps > kMaxLightSteps falls back to the per-lane loop.
+    int light_steps;
+    float light_offset[16];
+    float light_weight[16];
 };
 
 // Trilinear CLAMP_TO_EDGE fetch of an RGBA32F 3D LUT at texel-space coordinates.
@@ -56,15 +62,13 @@ struct MarchAcc {
     float sun_env_r, sun_env_g;
 };
 
-// RayMarchStep, VolumetricCloudRender.comp:116-137.
-template <int kMat, bool kCount>
-__device__ inline void march_step(const CloudArgs& a, const Shell& S, F3 pos, float t, float h01, float step_size,
-                                  float cos_sun_view, MarchAcc& acc, WorkCount* wc) {
-    const float sigma_t = sample_sigma_t<kMat, kCount>(a.mat, pos, h01, S.cam, wc);
-    if (sigma_t < 1e-5f) return;
-    if (kCount) wc->cloud_steps++;
+constexpr int kMaxLightSteps = 16;
+
+// The part of RayMarchStep (VolumetricCloudRender.comp:116-137) after sigma_t and the
+// transmittance to the sun are known.
+__device__ inline void march_accumulate(const CloudArgs& a, float sigma_t, float t_sun, float t, float h01,
+                                        float step_size, float cos_sun_view, MarchAcc& acc) {
     const float tr = __expf(-step_size * sigma_t);
-    const float t_sun = sample_shadow<kMat, kCount>(a.mat, S, pos, a.cloud.uShadowSteps, a.cloud.uShadowDistance, wc);
     const float w = __expf(-a.cloud.uSunMultiscatteringSigmaScale * sigma_t);
     const float phase = henyey_greenstein(cos_sun_view, -0.15f) * 2.16f * (1.0f - w) +
                         henyey_greenstein(cos_sun_view, 0.85f) * w;
@@ -80,6 +84,14 @@ __device__ inline void march_step(const CloudArgs& a, const Shell& S, F3 pos, fl
     acc.transmittance *= tr;
 }
 
+// Orders LDS traffic between the lanes of one wave (no cross-wave barrier: the
+// waves of a workgroup march different numbers of steps).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // FP64 unprojection of (u, v, depth) through uInvMVP, relative to the camera.
 __device__ inline F3 unproject(const float* m, float x, float y, float z, F3 cam, float* frag_dist) {
     double v[4] = {x, y, z, 1.0}, o[4];
@@ -92,24 +104,57 @@ __device__ inline F3 unproject(const float* m, float x, float y, float z, F3 cam
     return {float(px / len), float(py / len), float(pz / len)};
 }
 
+// Per-lane state of one ray's march (two shell segments).
+struct RayState {
+    F3 view;
+    float frag_dist, u, v, jitter, cos_sun_view;
+    float t1_0, t1_1, t2_1;
+    float t_start, step_size;
+    uint32_t k, n;
+    int seg;
+    bool done;
+    uint32_t seg_steps[2], seg_sched[2];
+    MarchAcc acc;
+
+    // Segment set-up of VolumetricCloudRender.comp:159-169 / :175-181.
+    __device__ void begin_segment(const CloudArgs& a, float t1, float t2, float jit, int s) {
+        float dist = fminf(t2 - t1, a.cloud.uMaxRaymarchDistance);
+        n = uint32_t(fmaxf(a.cloud.uMaxRaymarchSteps * (dist / a.cloud.uMaxRaymarchDistance), 1.0f));
+        step_size = dist / float(n);
+        // Sample k sits at t1 + (k + jitter) step_size, computed directly (no FP32 drift
+        // from accumulating `t += step_size`).
+        t_start = t1 + step_size * jit;
+        k = 0;
+        seg = s;
+        seg_sched[s] = n;
+    }
+};
+
 template <int kMat, bool kCount>
 __global__ __launch_bounds__(256) void cloud_raymarch_kernel(CloudArgs a) {
+    __shared__ float s_origin[4][64 * 3];
+    __shared__ float s_contrib[4][64 * kMaxLightSteps];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const bool active = px < a.width && py < a.height;
     int executed = 0;
     WorkCount wc;
-    if (active) {
+    Shell S;
+    {
         const SkyhipCloudCommonBufferData& cb = a.common;
-        Shell S;
         S.cam = f3(cb.uCameraPos);
         S.sun = f3(cb.uSunDirection);
         S.R = cb.uEarthRadius;
         S.hb = cb.uBottomAltitude;
         S.ht = cb.uTopAltitude;
         S.inv_thickness = 1.0f / (S.ht - S.hb);
-        const AtmDev A = make_atm(a.atm);
+    }
+    RayState ray;
+    ray.done = true;
+    ray.seg_steps[0] = ray.seg_steps[1] = ray.seg_sched[0] = ray.seg_sched[1] = 0u;
+    if (active) {
+        const SkyhipCloudCommonBufferData& cb = a.common;
 
         int pcx = px, pcy = py;
         if (a.index) {  // checkerboard sample choice, IndexToOffset (VolumetricCloudCommon.glsl:42-52)
@@ -161,29 +206,98 @@ __global__ __launch_bounds__(256) void cloud_raymarch_kernel(CloudArgs a) {
         const float noise = a.blue_noise[(py & a.bn_mask) * a.bn_size + (px & a.bn_mask)] * (1.0f / 65535.0f);
         float jitter = noise + cb.uFrameID * 0.61803398875f;
         jitter = jitter - floorf(jitter);
-        MarchAcc acc{1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t seg_steps[2] = {0u, 0u}, seg_sched[2] = {0u, 0u};
-        for (int seg = 0; seg < 2; ++seg) {
-            float dist = t2[seg] - t1[seg];
-            if (seg == 1 && !(dist > 0.0f)) break;
-            dist = fminf(dist, a.cloud.uMaxRaymarchDistance);
-            const uint32_t n = uint32_t(fmaxf(a.cloud.uMaxRaymarchSteps * (dist / a.cloud.uMaxRaymarchDistance), 1.0f));
-            const float step_size = dist / float(n);
-            // Sample k sits at t1 + (k + jitter) * step_size; computing it directly instead of
-            // accumulating `t += step_size` keeps FP32 rounding from drifting along the ray.
-            const float t_start = t1[seg] + step_size * jitter;
-            if (kCount) seg_sched[seg] = n;
-            for (uint32_t k = 0; k < n; ++k) {
-                const float t = fmaf(float(k), step_size, t_start);
-                const F3 pos = S.cam + view * t;
-                const float h01 = height01(S, pos);
-                march_step<kMat, kCount>(a, S, pos, t, h01, step_size, cos_sun_view, acc, &wc);
-                ++executed;
-                if (kCount) ++seg_steps[seg];
-                if (acc.transmittance < 0.01f) break;  // kMinTransmittance
+        ray.acc = MarchAcc{1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        ray.begin_segment(a, t1[0], t2[0], jitter, 0);
+        ray.t1_1 = t1[1];
+        ray.t2_1 = t2[1];
+        ray.jitter = jitter;
+        ray.cos_sun_view = cos_sun_view;
+        ray.view = view;
+        ray.frag_dist = frag_dist;
+        ray.t1_0 = t1[0];
+        ray.u = u;
+        ray.v = v;
+        ray.done = false;
+    }
+
+    // The march loop is wave-synchronous: every lane iterates until the whole wave is done, so
+    // the light march of the lanes whose sample is inside a cloud can be spread over all 64 lanes.
+    const bool coop = a.light_steps <= kMaxLightSteps;
+    const F3 sun_vector = S.sun * a.cloud.uShadowDistance;  // sample_vector of SampleShadow
+    while (__ballot(!ray.done) != 0) {
+        float t = 0.0f, h01 = 0.0f, sigma = 0.0f;
+        F3 pos = f3(0.0f);
+        if (!ray.done) {
+            t = fmaf(float(ray.k), ray.step_size, ray.t_start);
+            pos = S.cam + ray.view * t;
+            h01 = height01(S, pos);
+            sigma = sample_sigma_t<kMat, kCount>(a.mat, pos, h01, S.cam, &wc);
+        }
+        const bool lit = !ray.done && sigma >= 1e-5f;  // RayMarchStep's early return
+        const uint64_t lit_mask = __ballot(lit);
+        float t_sun = 1.0f;
+        if (lit_mask != 0) {
+            if (coop) {
+                // Stage the lit lanes' sample positions, evaluate all (ray, light step) pairs
+                // across the wave, then each lit lane sums its own in the shader's order.
+                const int m = __popcll(lit_mask);
+                const int rank = __builtin_amdgcn_mbcnt_hi(uint32_t(lit_mask >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo(uint32_t(lit_mask), 0u));
+                if (lit) {
+                    s_origin[wave][3 * rank + 0] = pos.x;
+                    s_origin[wave][3 * rank + 1] = pos.y;
+                    s_origin[wave][3 * rank + 2] = pos.z;
+                }
+                wave_lds_sync();
+                const int L = a.light_steps, total = m * L;
+                for (int base = 0; base < total; base += 64) {
+                    const int item = base + lane;
+                    if (item < total) {
+                        const int r = item / L, ls = item - r * L;
+                        const F3 o = {s_origin[wave][3 * r], s_origin[wave][3 * r + 1], s_origin[wave][3 * r + 2]};
+                        const F3 sp = o + sun_vector * a.light_offset[ls];
+                        s_contrib[wave][item] =
+                            sample_sigma_t<kMat, kCount>(a.mat, sp, height01(S, sp), S.cam, &wc) * a.light_weight[ls];
+                    }
+                }
+                wave_lds_sync();
+                if (lit) {
+                    float od = 0.0f;
+                    for (int ls = 0; ls < L; ++ls) od += s_contrib[wave][rank * L + ls];
+                    t_sun = __expf(-od);
+                }
+                wave_lds_sync();
+            } else if (lit) {
+                t_sun = sample_shadow<kMat, kCount>(a.mat, S, pos, a.cloud.uShadowSteps, a.cloud.uShadowDistance, &wc);
             }
         }
+        if (lit) {
+            if (kCount) wc.cloud_steps++;
+            march_accumulate(a, sigma, t_sun, t, h01, ray.step_size, ray.cos_sun_view, ray.acc);
+        }
+        if (!ray.done) {
+            ++executed;
+            if (kCount) ++ray.seg_steps[ray.seg];
+            ++ray.k;
+            if (ray.acc.transmittance < 0.01f || ray.k >= ray.n) {  // kMinTransmittance or segment end
+                if (ray.seg == 0 && ray.t2_1 - ray.t1_1 > 0.0f)
+                    ray.begin_segment(a, ray.t1_1, ray.t2_1, ray.jitter, 1);
+                else
+                    ray.done = true;
+            }
+        }
+    }
 
+    if (active) {
+        const SkyhipCloudCommonBufferData& cb = a.common;
+        const AtmDev A = make_atm(a.atm);
+        const MarchAcc& acc = ray.acc;
+        const F3 view = ray.view;
+        const float frag_dist = ray.frag_dist, u = ray.u, v = ray.v;
+        const float r = S.cam.z + S.R, mu = view.z;
+        const uint32_t* seg_steps = ray.seg_steps;
+        const uint32_t* seg_sched = ray.seg_sched;
+        float t1[1] = {ray.t1_0};
         if (kCount && a.pixel_steps) {
             a.pixel_steps[2 * (py * a.width + px)] = seg_steps[0] | (seg_steps[1] << 16);
             a.pixel_steps[2 * (py * a.width + px) + 1] = seg_sched[0] | (seg_sched[1] << 16);
@@ -404,6 +518,24 @@ extern "C" int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereB
     a.width = width, a.height = height;
     a.counters = work_counters;
     a.pixel_steps = work_counters ? pixel_steps : nullptr;
+    // SampleShadow's FP32 counter `for (t = 1/n; t <= 1.0; t += 1/n)`, evaluated once here with
+    // the same float arithmetic the shader uses (the iteration count depends on its rounding).
+    {
+        const float inv = 1.0f / cloud->uShadowSteps;
+        float prev = 0.0f;
+        int L = 0;
+        for (float t = inv; t <= 1.0f && L <= 256; t += inv) {
+            const float cur = t * t;
+            const float dt = cur - prev;
+            if (L < kMaxLightSteps) {
+                a.light_offset[L] = prev + 0.5f * dt;
+                a.light_weight[L] = cloud->uShadowDistance * dt;
+            }
+            prev = cur;
+            ++L;
+        }
+        a.light_steps = L;
+    }
     return work_counters ? launch_raymarch<true>(ctx->stream, a) : launch_raymarch<false>(ctx->stream, a);
 }
 
