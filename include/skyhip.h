@@ -270,7 +270,12 @@ typedef enum SkyhipCloudMaterialKind {
     SKYHIP_MATERIAL_DEFAULT0 = 0, /* VolumetricCloudDefaultMaterial0.glsl */
     SKYHIP_MATERIAL_DEFAULT1 = 1, /* VolumetricCloudDefaultMaterial1.glsl */
     SKYHIP_MATERIAL_MINIMAL = 2,  /* VolumetricCloudMaterialMinimal.glsl */
-    SKYHIP_MATERIAL_VOXEL = 3     /* VolumetricCloudMaterialVoxel.glsl (R8 3D texture with mips) */
+    SKYHIP_MATERIAL_VOXEL = 3,    /* VolumetricCloudMaterialVoxel.glsl (R8 3D texture with mips) */
+    /* The voxel material's SampleSigmaT on an FP16 density grid (level 0 only,
+     * LINEAR, CLAMP_TO_BORDER(0)); the voxel block's uLodBias and uSampleLodK
+     * are ignored.  No reference counterpart: it stands in for the OpenVDB-loaded
+     * volume (VolumetricCloudVoxelMaterial.cpp:39-75). */
+    SKYHIP_MATERIAL_VOXEL_F16 = 4
 } SkyhipCloudMaterialKind;
 
 typedef struct SkyhipCloudMaterial {
@@ -296,6 +301,9 @@ typedef struct SkyhipCloudMaterial {
     const uint8_t* voxel_texture; /* R8 3D, voxel_dim[0] x voxel_dim[1] x voxel_dim[2] (VOXEL only) */
     int32_t voxel_dim[3];
     int32_t voxel_levels;
+    const uint16_t* voxel_grid;   /* FP16 3D, x fastest, voxel_grid_dim[0..2] (VOXEL_F16 only) */
+    int32_t voxel_grid_dim[3];
+    int32_t voxel_grid_padding;
 } SkyhipCloudMaterial;
 
 /* NoiseCreateInfo, src/SkyRendering/VolumetricCloudDefaultMaterial.h:57-62 and
@@ -514,6 +522,112 @@ int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* 
 int skyhip_cloud_sample_sigma_t(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
                                 const SkyhipCloudMaterial* material, const float* positions_xyz, int32_t count,
                                 float* out_sigma_t);
+
+/* ------------------------------------------------------------------------- */
+/* Voxel-cloud path tracer                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* VolumetricCloud::PathTracing::InitParam (src/SkyRendering/VolumetricCloud.h:158-168) plus
+ * the constants the reference bakes into the shader source (VolumetricCloud.cpp:506-519). */
+typedef enum SkyhipPrng { SKYHIP_PRNG_WANG_HASH = 0, SKYHIP_PRNG_PCG_HASH = 1 } SkyhipPrng;
+typedef enum SkyhipEnvironmentLighting {
+    SKYHIP_ENV_OFF = 0,
+    SKYHIP_ENV_CONST_ENVIRONMENT_MAP = 1,
+    SKYHIP_ENV_GROUND_SINGLE_BOUNCE = 2,
+    SKYHIP_ENV_GROUND_MULTI_BOUNCE = 3
+} SkyhipEnvironmentLighting;
+
+typedef struct SkyhipPathTracingParameters {
+    int32_t max_bounces;            /* kMaxBounces, 128 */
+    float region_box_half_width;    /* kCloudHalfWidth, 100 (km) */
+    int32_t importance_sampling;    /* IMPORTANCE_SAMPLING, 1 */
+    float forward_phase_g;          /* kForwardPhaseG, 0.85 */
+    float back_phase_g;             /* kBackPhaseG, -0.15 */
+    float forward_scattering_ratio; /* kForwardScatteringRatio, 0.7 */
+    int32_t prng;                   /* SkyhipPrng, PCG_HASH */
+    int32_t environment_lighting;   /* SkyhipEnvironmentLighting, GROUND_MULTI_BOUNCE */
+    float sigma_t_max;              /* kSigmaTMax = material GetSigmaTMax() (1/km) */
+    float albedo;                   /* single-scattering albedo; the reference's is 1 */
+    float model3[9];                /* kModelMatrix3, column-major: cloud local frame -> atmosphere frame */
+} SkyhipPathTracingParameters;
+
+/* Device inputs of the path-trace pass: the LUTs it reads (VolumetricCloud.cpp:546-555)
+ * and the optional brick majorants.
+ *   sky_view_*:      the sky-view LUT pair (RGBA32F) of the atmosphere renderer whose block is
+ *                    passed with the call; the environment term samples it directly, which is
+ *                    level 0 of the reference's environment-luminance cube up to resampling
+ *                    (AtmosphereRenderer.glsl:248-275);
+ *   brick_majorants: NULL to track with the global sigma_t_max as the reference does, or the
+ *                    float[ceil(dim/8)^3] output of skyhip_voxel_grid_majorants for a
+ *                    VOXEL_F16 material (scaled by the voxel block's uDensity on use). */
+typedef struct SkyhipPathTracingInputs {
+    const float* transmittance;
+    int32_t transmittance_width;
+    int32_t transmittance_height;
+    const float* sky_view_luminance;
+    const float* sky_view_transmittance;
+    int32_t sky_view_width;
+    int32_t sky_view_height;
+    const float* ap_luminance;
+    const float* ap_transmittance;
+    int32_t ap_width;
+    int32_t ap_height;
+    int32_t ap_depth;
+    int32_t padding;
+    const float* brick_majorants;
+} SkyhipPathTracingInputs;
+
+/* The synthetic voxel-cloud density of the path-tracing workload (no reference
+ * counterpart: the reference loads an OpenVDB file).  Per texel centre p in [0,1]^3:
+ *   f       = sum_{o<octaves} 0.5^o PerlinNoise(p, base_frequency 2^o, seed) / sum 0.5^o
+ *   falloff = max(1 - |(p - ellipsoid_center) / ellipsoid_radii|^2, 0)
+ *   density = max(exp(lognormal_sigma * fbm_inv_std * f) * falloff - threshold, 0)
+ * stored as FP16 (round to nearest even); PerlinNoise is Noise.glsl:28-60. */
+typedef struct SkyhipVoxelGridCreateInfo {
+    uint32_t seed;
+    uint32_t base_frequency;
+    int32_t octaves;
+    float fbm_inv_std;
+    float lognormal_sigma;
+    float threshold;
+    float ellipsoid_center[3];
+    float ellipsoid_radii[3];
+} SkyhipVoxelGridCreateInfo;
+
+/* Fills out_grid (device FP16[dim_x * dim_y * dim_z], x fastest).  Dims 8 .. 1024. */
+int skyhip_voxel_grid_generate(SkyhipContext* ctx, const SkyhipVoxelGridCreateInfo* info, uint16_t* out_grid,
+                               int32_t dim_x, int32_t dim_y, int32_t dim_z);
+
+/* Majorant per 8^3 brick of an FP16 grid: the maximum over the brick's texels and a
+ * one-texel apron on every side (clamped to the grid), which bounds trilinear
+ * CLAMP_TO_BORDER(0) samples anywhere in the brick.  out_majorants: device
+ * float[ceil(dim_x/8) * ceil(dim_y/8) * ceil(dim_z/8)], x fastest. */
+int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
+                                int32_t dim_z, float* out_majorants);
+
+/* One or more frames of VolumetricCloud::PathTracing::Render's trace pass
+ * (src/SkyRendering/VolumetricCloud.cpp:537-566, VolumetricCloudPathTracing.comp:255-284):
+ * for frame ids first_frame .. first_frame + frames - 1 (the reference's frame_cnt_, from 1)
+ * every pixel of region = {x0, y0, x1, y1} traces one path per frame and adds (L, alpha) to
+ * accumulation_rgba (RGBA32F width x height, in the frame order), and rendered_mask
+ * (R8, may be NULL) gets 1.  The random sequence of each sample is the reference's
+ * (seed = PRNG(PRNG(PRNG(x) + y) + frame)); with brick majorants the tracking draws differ
+ * but the estimator keeps its expectation.
+ *   work_counters: NULL, or device uint64[5] accumulating samples, density lookups,
+ *                  majorant segments, path vertices (volume and ground events) and shadow rays
+ *                  (a counting kernel variant; leave NULL when timing). */
+int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                            const SkyhipAtmosphereRenderBufferData* render, const SkyhipCloudCommonBufferData* common,
+                            const SkyhipCloudMaterial* material, const SkyhipPathTracingParameters* params,
+                            const SkyhipPathTracingInputs* inputs, const int32_t region[4], uint32_t first_frame,
+                            int32_t frames, float* accumulation_rgba, uint8_t* rendered_mask, int32_t width,
+                            int32_t height, unsigned long long* work_counters);
+
+/* The display pass (VolumetricCloudPathTracing.comp:288-296): for every pixel,
+ * avg = accumulation / (mask ? frame_id : frame_id - 1) and display.rgb = display.rgb * avg.a + avg.rgb
+ * (display_rgba16f: RGBA16F width x height, read and written; rendered_mask may be NULL = all 1). */
+int skyhip_cloud_path_trace_display(SkyhipContext* ctx, const float* accumulation_rgba, const uint8_t* rendered_mask,
+                                    uint32_t frame_id, void* display_rgba16f, int32_t width, int32_t height);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -248,3 +248,118 @@ def cloud_raymarch_forced(atm, common, cloud, inputs: HostCloudInputs, width, pi
                                            C.byref(inputs.images), width, _ptr(px), len(px), _ptr(fs), _ptr(rgba),
                                            _ptr(dist), _ptr(steps), _ptr(margin), _ptr(step_margin))
     return rgba, dist, steps, margin, step_margin
+
+
+# ---------------------------------------------------------------------------
+# Voxel-cloud path tracer
+# ---------------------------------------------------------------------------
+_PT_ARGS = [P(_abi.AtmosphereBufferData), P(_abi.AtmosphereRenderBufferData), P(_abi.CloudCommonBufferData),
+            P(_abi.CloudMaterial), P(_abi.PathTracingParametersC), P(_abi.PathTracingInputsC), i32, i32, vp, C.c_long,
+            C.c_uint32, i32, vp]
+register("oracle_path_trace_f64", None, _PT_ARGS)
+register("oracle_path_trace_f32", None, _PT_ARGS)
+register("oracle_pt_hg_sample_f64", None, [P(_abi.PathTracingParametersC), C.c_uint32, vp, vp, P(f64),
+                                           P(C.c_uint32)])
+register("oracle_pt_transmittance_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
+                                              P(_abi.PathTracingParametersC), C.c_uint32, vp, vp])
+register("oracle_pt_sigma_t_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
+                                        P(_abi.PathTracingParametersC), f64, f64, f64])
+register("oracle_voxel_grid_f16", None, [P(_abi.VoxelGridCreateInfoC), i32, i32, i32, vp])
+register("oracle_voxel_grid_density_f32", None, [P(_abi.VoxelGridCreateInfoC), i32, i32, i32, vp, C.c_long, vp])
+register("oracle_brick_majorants", None, [vp, i32, i32, i32, vp])
+register("oracle_float_to_half", C.c_uint16, [f32])
+register("oracle_half_to_float", f32, [C.c_uint16])
+
+
+class HostPathTraceInputs:
+    """Host copies of the path tracer's inputs, kept alive for the oracle call.
+    grid: uint16 FP16 bits [z, y, x] (VOXEL_F16 material), T / sv_l / sv_t: RGBA32F 2D LUTs,
+    ap_l / ap_t: RGBA32F [d, h, w, 4]."""
+
+    def __init__(self, material, T, sv_l, sv_t, ap_l, ap_t, grid=None, cloud_map=None, detail=None,
+                 displacement=None, voxel=None):
+        arrs = [np.ascontiguousarray(x) if x is not None else None
+                for x in (T, sv_l, sv_t, ap_l, ap_t, grid, cloud_map, detail, displacement, voxel)]
+        T, sv_l, sv_t, ap_l, ap_t, grid, cloud_map, detail, displacement, voxel = arrs
+        self.keep = [x for x in arrs if x is not None]
+        m = _abi.CloudMaterial()
+        C.memmove(C.byref(m), C.byref(material), C.sizeof(m))
+        m.cloud_map = _ptr(cloud_map)
+        m.detail = _ptr(detail)
+        m.displacement = _ptr(displacement)
+        m.voxel_texture = _ptr(voxel)
+        m.voxel_grid = _ptr(grid)
+        self.material = m
+        i = _abi.PathTracingInputsC()
+        i.transmittance, i.transmittance_width, i.transmittance_height = _ptr(T), T.shape[1], T.shape[0]
+        i.sky_view_luminance, i.sky_view_transmittance = _ptr(sv_l), _ptr(sv_t)
+        i.sky_view_width, i.sky_view_height = sv_l.shape[1], sv_l.shape[0]
+        i.ap_luminance, i.ap_transmittance = _ptr(ap_l), _ptr(ap_t)
+        i.ap_depth, i.ap_height, i.ap_width = ap_l.shape[0], ap_l.shape[1], ap_l.shape[2]
+        i.brick_majorants = None
+        self.inputs = i
+
+
+def path_trace(atm, render, common, params, inputs: HostPathTraceInputs, width, height, pixels, first_frame, frames,
+               precision="f64"):
+    """Samples of the reference tracer (single majorant): array [len(pixels), frames, 4]."""
+    px = np.ascontiguousarray(pixels, dtype=np.int32)
+    out = np.empty((len(px), frames, 4), _dt(precision))
+    getattr(lib(), f"oracle_path_trace_{precision}")(
+        C.byref(atm), C.byref(render), C.byref(common), C.byref(inputs.material), C.byref(params),
+        C.byref(inputs.inputs), width, height, _ptr(px), len(px), first_frame, frames, _ptr(out))
+    return out
+
+
+def pt_hg_sample(params, seed, d):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.empty(3, np.float64)
+    vop = C.c_double()
+    s = C.c_uint32()
+    lib().oracle_pt_hg_sample_f64(C.byref(params), seed, _ptr(d), _ptr(out), C.byref(vop), C.byref(s))
+    return out, vop.value, s.value
+
+
+def pt_transmittance(common, material, params, seed, o, d):
+    o = np.ascontiguousarray(o, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    return lib().oracle_pt_transmittance_f64(C.byref(common), C.byref(material), C.byref(params), seed, _ptr(o),
+                                             _ptr(d))
+
+
+def pt_sigma_t(common, material, params, p):
+    return lib().oracle_pt_sigma_t_f64(C.byref(common), C.byref(material), C.byref(params), *map(float, p))
+
+
+def voxel_grid_f16(ci, dims):
+    """FP16 bits [z, y, x] of the workload grid."""
+    dx, dy, dz = dims
+    out = np.empty((dz, dy, dx), np.uint16)
+    lib().oracle_voxel_grid_f16(C.byref(ci), dx, dy, dz, _ptr(out))
+    return out
+
+
+def voxel_grid_density(ci, dims, xyz):
+    xyz = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+    out = np.empty(len(xyz), np.float32)
+    lib().oracle_voxel_grid_density_f32(C.byref(ci), *dims, _ptr(xyz), len(xyz), _ptr(out))
+    return out
+
+
+def brick_majorants(grid):
+    g = np.ascontiguousarray(grid, dtype=np.uint16)
+    dz, dy, dx = g.shape
+    out = np.empty(((dz + 7) // 8, (dy + 7) // 8, (dx + 7) // 8), np.float32)
+    lib().oracle_brick_majorants(_ptr(g), dx, dy, dz, _ptr(out))
+    return out
+
+
+register("oracle_pt_lambert_sample_f64", None, [C.c_uint32, vp, vp, P(C.c_uint32)])
+
+
+def pt_lambert_sample(seed, n):
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    out = np.empty(3, np.float64)
+    s = C.c_uint32()
+    lib().oracle_pt_lambert_sample_f64(seed, _ptr(n), _ptr(out), C.byref(s))
+    return out, s.value

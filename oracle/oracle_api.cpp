@@ -9,6 +9,7 @@
 
 #include "oracle_cloud.h"
 #include "oracle_noise.h"
+#include "oracle_pathtrace.h"
 
 using namespace oracle;
 
@@ -91,6 +92,41 @@ void cloud_raymarch(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudComm
         for (int k = 0; k < 4; ++k) out_rgba[i * 4 + k] = O(o.rgba[k]);
         out_dist[i] = O(o.distance);
         if (out_steps) out_steps[i] = o.steps;
+    }
+}
+
+
+template <class S>
+PathTracer<S> make_tracer(const Atmosphere<S>& A, const RenderBlock<S>& R, const SkyhipCloudCommonBufferData* common,
+                          const SkyhipCloudMaterial* material, const SkyhipPathTracingParameters* params,
+                          const SkyhipPathTracingInputs* in) {
+    PathTracer<S> P(A, R, *common, *material, *params);
+    P.T = in->transmittance, P.tw = in->transmittance_width, P.th = in->transmittance_height;
+    P.sv_l = in->sky_view_luminance, P.sv_t = in->sky_view_transmittance;
+    P.sw = in->sky_view_width, P.sh = in->sky_view_height;
+    P.ap_l = in->ap_luminance, P.ap_t = in->ap_transmittance;
+    P.aw = in->ap_width, P.ah = in->ap_height, P.ad = in->ap_depth;
+    return P;
+}
+
+// Samples of the listed pixels for frames first_frame .. first_frame + frames - 1:
+// out[(i * frames + f) * 4 + c].
+template <class S, class O>
+void path_trace(const SkyhipAtmosphereBufferData* atm, const SkyhipAtmosphereRenderBufferData* render,
+                const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                const SkyhipPathTracingParameters* params, const SkyhipPathTracingInputs* in, int width, int height,
+                const int32_t* pixels, long count, uint32_t first_frame, int frames, O* out) {
+    Atmosphere<S> A(*atm);
+    RenderBlock<S> R(*render);
+    PathTracer<S> P = make_tracer<S>(A, R, common, material, params, in);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < count; ++i) {
+        int p = pixels[i];
+        for (int f = 0; f < frames; ++f) {
+            S o[4];
+            P.sample(p % width, p / width, width, height, first_frame + uint32_t(f), o);
+            for (int c = 0; c < 4; ++c) out[(i * frames + f) * 4 + c] = O(o[c]);
+        }
     }
 }
 
@@ -293,5 +329,99 @@ double oracle_cloud_shadow_f64(const SkyhipCloudCommonBufferData* common, const 
     CloudCtx<double> Cc(common, cloud, material);
     return Cc.SampleShadow({x, y, z});
 }
+
+// ---------------------------------------------------------------------------
+// Voxel-cloud path tracer (VolumetricCloudPathTracing.comp), global majorant.
+// ---------------------------------------------------------------------------
+void oracle_path_trace_f64(const SkyhipAtmosphereBufferData* atm, const SkyhipAtmosphereRenderBufferData* render,
+                           const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                           const SkyhipPathTracingParameters* params, const SkyhipPathTracingInputs* in, int width,
+                           int height, const int32_t* pixels, long count, uint32_t first_frame, int frames,
+                           double* out) {
+    path_trace<double>(atm, render, common, material, params, in, width, height, pixels, count, first_frame, frames,
+                       out);
+}
+void oracle_path_trace_f32(const SkyhipAtmosphereBufferData* atm, const SkyhipAtmosphereRenderBufferData* render,
+                           const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                           const SkyhipPathTracingParameters* params, const SkyhipPathTracingInputs* in, int width,
+                           int height, const int32_t* pixels, long count, uint32_t first_frame, int frames,
+                           float* out) {
+    path_trace<float>(atm, render, common, material, params, in, width, height, pixels, count, first_frame, frames,
+                      out);
+}
+
+// Pieces of the tracer for closed-form checks.
+void oracle_pt_hg_sample_f64(const SkyhipPathTracingParameters* params, uint32_t seed, const double* d, double* out_dir,
+                             double* out_value_over_pdf, uint32_t* out_seed) {
+    static const SkyhipAtmosphereBufferData zero_atm = {};
+    static const SkyhipAtmosphereRenderBufferData zero_render = {};
+    static const SkyhipCloudCommonBufferData zero_common = {};
+    static const SkyhipCloudMaterial zero_material = {};
+    Atmosphere<double> A(zero_atm);
+    RenderBlock<double> R(zero_render);
+    PathTracer<double> P(A, R, zero_common, zero_material, *params);
+    double vop;
+    V3<double> r = P.hg_sample(seed, {d[0], d[1], d[2]}, vop);
+    out_dir[0] = r.x, out_dir[1] = r.y, out_dir[2] = r.z;
+    *out_value_over_pdf = vop;
+    *out_seed = seed;
+}
+void oracle_pt_lambert_sample_f64(uint32_t seed, const double* n, double* out_dir, uint32_t* out_seed) {
+    static const SkyhipAtmosphereBufferData zero_atm = {};
+    static const SkyhipAtmosphereRenderBufferData zero_render = {};
+    static const SkyhipCloudCommonBufferData zero_common = {};
+    static const SkyhipCloudMaterial zero_material = {};
+    static const SkyhipPathTracingParameters params = {};
+    Atmosphere<double> A(zero_atm);
+    RenderBlock<double> R(zero_render);
+    PathTracer<double> P(A, R, zero_common, zero_material, params);
+    V3<double> r = P.lambert_sample(seed, {n[0], n[1], n[2]});
+    out_dir[0] = r.x, out_dir[1] = r.y, out_dir[2] = r.z;
+    *out_seed = seed;
+}
+double oracle_pt_transmittance_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                                   const SkyhipPathTracingParameters* params, uint32_t seed, const double* o,
+                                   const double* d) {
+    static const SkyhipAtmosphereBufferData zero_atm = {};
+    static const SkyhipAtmosphereRenderBufferData zero_render = {};
+    Atmosphere<double> A(zero_atm);
+    RenderBlock<double> R(zero_render);
+    PathTracer<double> P(A, R, *common, *material, *params);
+    return P.transmittance_estimation(seed, {o[0], o[1], o[2]}, {d[0], d[1], d[2]});
+}
+double oracle_pt_sigma_t_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                             const SkyhipPathTracingParameters* params, double x, double y, double z) {
+    static const SkyhipAtmosphereBufferData zero_atm = {};
+    static const SkyhipAtmosphereRenderBufferData zero_render = {};
+    Atmosphere<double> A(zero_atm);
+    RenderBlock<double> R(zero_render);
+    PathTracer<double> P(A, R, *common, *material, *params);
+    return P.sigma({x, y, z});
+}
+
+// The FP16 density grid of the path-tracing workload, and its brick majorants.
+void oracle_voxel_grid_f16(const SkyhipVoxelGridCreateInfo* ci, int dx, int dy, int dz, uint16_t* out) {
+    const int dim[3] = {dx, dy, dz};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int z = 0; z < dz; ++z)
+        for (int y = 0; y < dy; ++y)
+            for (int x = 0; x < dx; ++x)
+                out[(size_t(z) * dy + y) * dx + x] = float_to_half_rne(VoxelGridDensity(x, y, z, dim, *ci));
+}
+void oracle_voxel_grid_density_f32(const SkyhipVoxelGridCreateInfo* ci, int dx, int dy, int dz, const int32_t* xyz,
+                                   long count, float* out) {
+    const int dim[3] = {dx, dy, dz};
+    for (long i = 0; i < count; ++i) out[i] = VoxelGridDensity(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], dim, *ci);
+}
+void oracle_brick_majorants(const uint16_t* grid, int dx, int dy, int dz, float* out) {
+    const int dim[3] = {dx, dy, dz};
+    int bx = (dx + 7) / 8, by = (dy + 7) / 8, bz = (dz + 7) / 8;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < bz; ++z)
+        for (int y = 0; y < by; ++y)
+            for (int x = 0; x < bx; ++x) out[(size_t(z) * by + y) * bx + x] = BrickMajorant(grid, dim, x, y, z);
+}
+uint16_t oracle_float_to_half(float x) { return float_to_half_rne(x); }
+float oracle_half_to_float(uint16_t h) { return half_to_float(h); }
 
 }  // extern "C"

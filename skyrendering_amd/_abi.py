@@ -166,7 +166,7 @@ class CloudVoxelBufferData(C.Structure):
     ]
 
 
-MATERIAL_DEFAULT0, MATERIAL_DEFAULT1, MATERIAL_MINIMAL, MATERIAL_VOXEL = 0, 1, 2, 3
+MATERIAL_DEFAULT0, MATERIAL_DEFAULT1, MATERIAL_MINIMAL, MATERIAL_VOXEL, MATERIAL_VOXEL_F16 = 0, 1, 2, 3, 4
 
 
 class CloudMaterial(C.Structure):
@@ -178,11 +178,40 @@ class CloudMaterial(C.Structure):
         ("detail", vp), ("detail_size", i32), ("detail_levels", i32),
         ("displacement", vp), ("displacement_size", i32), ("displacement_levels", i32),
         ("voxel_texture", vp), ("voxel_dim", _arr(i32, 3)), ("voxel_levels", i32),
+        ("voxel_grid", vp), ("voxel_grid_dim", _arr(i32, 3)), ("voxel_grid_padding", i32),
     ]
 
 
 class NoiseCreateInfo(C.Structure):
     _fields_ = [("seed", u32), ("base_frequency", u32), ("remap_min", f32), ("remap_max", f32)]
+
+
+PRNG_WANG_HASH, PRNG_PCG_HASH = 0, 1
+ENV_OFF, ENV_CONST_ENVIRONMENT_MAP, ENV_GROUND_SINGLE_BOUNCE, ENV_GROUND_MULTI_BOUNCE = 0, 1, 2, 3
+
+
+class PathTracingParametersC(C.Structure):
+    _fields_ = [
+        ("max_bounces", i32), ("region_box_half_width", f32), ("importance_sampling", i32),
+        ("forward_phase_g", f32), ("back_phase_g", f32), ("forward_scattering_ratio", f32), ("prng", i32),
+        ("environment_lighting", i32), ("sigma_t_max", f32), ("albedo", f32), ("model3", _arr(f32, 9)),
+    ]
+
+
+class PathTracingInputsC(C.Structure):
+    _fields_ = [
+        ("transmittance", vp), ("transmittance_width", i32), ("transmittance_height", i32),
+        ("sky_view_luminance", vp), ("sky_view_transmittance", vp), ("sky_view_width", i32),
+        ("sky_view_height", i32), ("ap_luminance", vp), ("ap_transmittance", vp), ("ap_width", i32),
+        ("ap_height", i32), ("ap_depth", i32), ("padding", i32), ("brick_majorants", vp),
+    ]
+
+
+class VoxelGridCreateInfoC(C.Structure):
+    _fields_ = [
+        ("seed", u32), ("base_frequency", u32), ("octaves", i32), ("fbm_inv_std", f32), ("lognormal_sigma", f32),
+        ("threshold", f32), ("ellipsoid_center", _arr(f32, 3)), ("ellipsoid_radii", _arr(f32, 3)),
+    ]
 
 
 assert C.sizeof(AtmosphereBufferData) == 128
@@ -192,6 +221,9 @@ assert C.sizeof(CloudBufferData) == 64
 assert C.sizeof(CloudMaterialCommonBufferData) == 64
 assert C.sizeof(CloudMaterial1BufferData) == 32
 assert C.sizeof(CloudVoxelBufferData) == 32
+assert C.sizeof(PathTracingParametersC) == 76
+assert C.sizeof(PathTracingInputsC) == 80
+assert C.sizeof(VoxelGridCreateInfoC) == 48
 
 P = C.POINTER
 _PROTOTYPES = {

@@ -38,25 +38,6 @@ struct CloudArgs {
     float light_weight[16];
 };
 
-// Trilinear CLAMP_TO_EDGE fetch of an RGBA32F 3D LUT at texel-space coordinates.
-__device__ inline F3 lut3d_trilinear(const float4* __restrict__ lut, int w, int h, int d, float fx, float fy,
-                                     float fz) {
-    fx = fminf(fmaxf(fx, 0.0f), float(w - 1));
-    fy = fminf(fmaxf(fy, 0.0f), float(h - 1));
-    fz = fminf(fmaxf(fz, 0.0f), float(d - 1));
-    int x0 = int(fx), y0 = int(fy), z0 = int(fz);
-    int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1), z1 = min(z0 + 1, d - 1);
-    float ax = fx - float(x0), ay = fy - float(y0), az = fz - float(z0);
-    F3 r = f3(0.0f);
-    for (int k = 0; k < 8; ++k) {
-        int xx = (k & 1) ? x1 : x0, yy = (k & 2) ? y1 : y0, zz = (k & 4) ? z1 : z0;
-        float wgt = ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az);
-        float4 v = lut[(zz * h + yy) * w + xx];
-        r += F3{v.x, v.y, v.z} * wgt;
-    }
-    return r;
-}
-
 struct MarchAcc {
     float transmittance, transmittance_sum, weighted_t_sum;
     float sun_env_r, sun_env_g;
@@ -90,18 +71,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// FP64 unprojection of (u, v, depth) through uInvMVP, relative to the camera.
-__device__ inline F3 unproject(const float* m, float x, float y, float z, F3 cam, float* frag_dist) {
-    double v[4] = {x, y, z, 1.0}, o[4];
-    for (int row = 0; row < 4; ++row)
-        o[row] = double(m[row]) * v[0] + double(m[4 + row]) * v[1] + double(m[8 + row]) * v[2] +
-                 double(m[12 + row]) * v[3];
-    double px = o[0] / o[3] - double(cam.x), py = o[1] / o[3] - double(cam.y), pz = o[2] / o[3] - double(cam.z);
-    double len = sqrt(px * px + py * py + pz * pz);
-    *frag_dist = float(len);
-    return {float(px / len), float(py / len), float(pz / len)};
 }
 
 // Per-lane state of one ray's march (two shell segments).
@@ -430,6 +399,16 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
                 return fail(SKYHIP_ERR_INVALID_SIZE, "bad voxel mip level count");
             make_tex3d(d.voxel_tex, m->voxel_texture, m->voxel_dim, m->voxel_levels);
             return SKYHIP_OK;
+        case SKYHIP_MATERIAL_VOXEL_F16:
+            if (!finite(&m->voxel, sizeof(m->voxel))) return fail(SKYHIP_ERR_NON_FINITE, "voxel block non-finite");
+            if (!m->voxel_grid) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "voxel grid is NULL");
+            for (int k = 0; k < 3; ++k) {
+                if (m->voxel_grid_dim[k] < 2 || m->voxel_grid_dim[k] > 4096)
+                    return fail(SKYHIP_ERR_INVALID_SIZE, "voxel grid dims outside [2, 4096]");
+                d.grid.dim[k] = m->voxel_grid_dim[k];
+            }
+            d.grid.data = m->voxel_grid;
+            return SKYHIP_OK;
         default:
             return fail(SKYHIP_ERR_INVALID_ARGUMENT, "unknown material kind %d", m->kind);
     }
@@ -448,8 +427,12 @@ static int launch_raymarch(hipStream_t st, const CloudArgs& a) {
         case SKYHIP_MATERIAL_MINIMAL:
             hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_MINIMAL, kCount>), grid, dim3(256), 0, st, a);
             break;
-        default:
+        case SKYHIP_MATERIAL_VOXEL:
             hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_VOXEL, kCount>), grid, dim3(256), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_VOXEL_F16, kCount>), grid, dim3(256), 0, st,
+                               a);
             break;
     }
     SKYHIP_LAUNCH_CHECK("cloud_raymarch_kernel");
@@ -585,8 +568,12 @@ extern "C" int skyhip_cloud_sample_sigma_t(SkyhipContext* ctx, const SkyhipCloud
             hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_MINIMAL>, grid, dim3(256), 0, ctx->stream, m,
                                *common, positions_xyz, count, out_sigma_t);
             break;
-        default:
+        case SKYHIP_MATERIAL_VOXEL:
             hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_VOXEL>, grid, dim3(256), 0, ctx->stream, m,
+                               *common, positions_xyz, count, out_sigma_t);
+            break;
+        default:
+            hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_VOXEL_F16>, grid, dim3(256), 0, ctx->stream, m,
                                *common, positions_xyz, count, out_sigma_t);
             break;
     }

@@ -101,6 +101,46 @@ __device__ inline float tex3d_r8(const Tex3D& t, float u, float v, float w, floa
     return float(d[t.offset[l] + ((long long)zi * sy + yi) * sx + xi]) * (1.0f / 255.0f);
 }
 
+// FP16 density grid (x fastest), sampled at level 0 with LINEAR filtering and
+// CLAMP_TO_BORDER(0).
+struct GridF16 {
+    const uint16_t* data;
+    int dim[3];
+};
+
+__device__ __forceinline__ float half_bits_to_float(uint16_t h) { return float(__builtin_bit_cast(_Float16, h)); }
+
+// Trilinear sample at normalised (u, v, w); texel-space x = u * dim - 0.5 as GL.
+__device__ inline float grid_f16_trilinear(const GridF16& g, float u, float v, float w) {
+    const int nx = g.dim[0], ny = g.dim[1], nz = g.dim[2];
+    float x = u * float(nx) - 0.5f, y = v * float(ny) - 0.5f, z = w * float(nz) - 0.5f;
+    int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
+    float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
+    float c[8];
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix + 1 < nx && iy + 1 < ny && iz + 1 < nz) {
+        const uint16_t* p = g.data + ((long long)iz * ny + iy) * nx + ix;
+        const long long plane = (long long)nx * ny;
+        c[0] = half_bits_to_float(p[0]);
+        c[1] = half_bits_to_float(p[1]);
+        c[2] = half_bits_to_float(p[nx]);
+        c[3] = half_bits_to_float(p[nx + 1]);
+        c[4] = half_bits_to_float(p[plane]);
+        c[5] = half_bits_to_float(p[plane + 1]);
+        c[6] = half_bits_to_float(p[plane + nx]);
+        c[7] = half_bits_to_float(p[plane + nx + 1]);
+    } else {
+        for (int k = 0; k < 8; ++k) {
+            int xx = ix + (k & 1), yy = iy + ((k >> 1) & 1), zz = iz + ((k >> 2) & 1);
+            bool inside = xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+            c[k] = inside ? half_bits_to_float(g.data[((long long)zz * ny + yy) * nx + xx]) : 0.0f;
+        }
+    }
+    float r = 0.0f;
+    for (int k = 0; k < 8; ++k)
+        r += ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az) * c[k];
+    return r;
+}
+
 // Material parameters as the kernels see them.
 struct MaterialDev {
     int kind;
@@ -111,6 +151,7 @@ struct MaterialDev {
     SkyhipCloudVoxelBufferData voxel;
     Tex2D cloud_map, displacement;
     Tex3D detail, voxel_tex;
+    GridF16 grid;
 };
 
 __device__ __forceinline__ float length3(F3 a) { return sqrtf(dot(a, a)); }
@@ -127,6 +168,10 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
                                        WorkCount* wc = nullptr) {
     if (kCount) wc->evals++;
     if (kMat == SKYHIP_MATERIAL_MINIMAL) return M.minimal_density;  // VolumetricCloudMaterialMinimal.glsl:6-7
+    if (kMat == SKYHIP_MATERIAL_VOXEL_F16)  // the voxel material's uv mapping on the FP16 grid, level 0
+        return grid_f16_trilinear(M.grid, pos.x * M.voxel.uSampleFrequency[0] + M.voxel.uSampleBias[0],
+                                  pos.y * M.voxel.uSampleFrequency[1] + M.voxel.uSampleBias[1], height01) *
+               M.voxel.uDensity;
     const float dist = length3(pos - cam);
     if (kMat == SKYHIP_MATERIAL_VOXEL) {  // VolumetricCloudMaterialVoxel.glsl:12-16
         float u = pos.x * M.voxel.uSampleFrequency[0] + M.voxel.uSampleBias[0];
@@ -220,6 +265,37 @@ __device__ __forceinline__ void sphere_roots(float b, float c, float& near_t, fl
         far_t = -b + s;
         near_t = (s - b) > 0.0f ? -c / (s - b) : -b - s;
     }
+}
+
+// Trilinear CLAMP_TO_EDGE fetch of an RGBA32F 3D LUT at texel-space coordinates.
+__device__ inline F3 lut3d_trilinear(const float4* __restrict__ lut, int w, int h, int d, float fx, float fy,
+                                     float fz) {
+    fx = fminf(fmaxf(fx, 0.0f), float(w - 1));
+    fy = fminf(fmaxf(fy, 0.0f), float(h - 1));
+    fz = fminf(fmaxf(fz, 0.0f), float(d - 1));
+    int x0 = int(fx), y0 = int(fy), z0 = int(fz);
+    int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1), z1 = min(z0 + 1, d - 1);
+    float ax = fx - float(x0), ay = fy - float(y0), az = fz - float(z0);
+    F3 r = f3(0.0f);
+    for (int k = 0; k < 8; ++k) {
+        int xx = (k & 1) ? x1 : x0, yy = (k & 2) ? y1 : y0, zz = (k & 4) ? z1 : z0;
+        float wgt = ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az);
+        float4 v = lut[(zz * h + yy) * w + xx];
+        r += F3{v.x, v.y, v.z} * wgt;
+    }
+    return r;
+}
+
+// FP64 unprojection of (u, v, depth) through uInvMVP, relative to the camera.
+__device__ inline F3 unproject(const float* m, float x, float y, float z, F3 cam, float* frag_dist) {
+    double v[4] = {x, y, z, 1.0}, o[4];
+    for (int row = 0; row < 4; ++row)
+        o[row] = double(m[row]) * v[0] + double(m[4 + row]) * v[1] + double(m[8 + row]) * v[2] +
+                 double(m[12 + row]) * v[3];
+    double px = o[0] / o[3] - double(cam.x), py = o[1] / o[3] - double(cam.y), pz = o[2] / o[3] - double(cam.z);
+    double len = sqrt(px * px + py * py + pz * pz);
+    *frag_dist = float(len);
+    return {float(px / len), float(py / len), float(pz / len)};
 }
 
 }  // namespace skyhip

@@ -3,6 +3,8 @@
 // volume (R8 Perlin-Worley) and the 2D displacement map (RGBA8 Perlin-FBM),
 // plus the box-filter mip chain builder.  One lane per texel.  Hashes are
 // exact uint32 arithmetic; FBM sums run in FP32 as in the shader.
+#include <hip/hip_fp16.h>
+
 #include "common.h"
 
 namespace skyhip {
@@ -177,6 +179,34 @@ __global__ __launch_bounds__(256) void downsample_kernel(const uint8_t* __restri
     dst[t] = uint8_t((sum + n / 2) / n);
 }
 
+// The path-tracing workload's voxel-cloud density (see SkyhipVoxelGridCreateInfo):
+// fBm of PerlinNoise (unremapped), lognormal transform, ellipsoidal falloff,
+// threshold, FP16 round-to-nearest-even.  One lane per texel.
+__global__ __launch_bounds__(256) void voxel_grid_kernel(SkyhipVoxelGridCreateInfo ci, uint16_t* __restrict__ out,
+                                                         int nx, int ny, int nz) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nx * ny * nz) return;
+    int x = int(i % nx), y = int((i / nx) % ny), z = int(i / ((long long)nx * ny));
+    float p[3] = {(float(x) + 0.5f) / float(nx), (float(y) + 0.5f) / float(ny), (float(z) + 0.5f) / float(nz)};
+    float res = 0.0f, a = 1.0f, sum_a = 0.0f;
+    uint32_t f = ci.base_frequency;
+    for (int o = 0; o < ci.octaves; ++o) {
+        res += perlin_noise(p[0], p[1], p[2], f, ci.seed) * a;
+        sum_a += a;
+        f *= 2u;
+        a *= 0.5f;
+    }
+    float fbm = res / sum_a;
+    float e = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        float q = (p[k] - ci.ellipsoid_center[k]) / ci.ellipsoid_radii[k];
+        e += q * q;
+    }
+    float falloff = fmaxf(1.0f - e, 0.0f);
+    float density = fmaxf(expf(ci.lognormal_sigma * ci.fbm_inv_std * fbm) * falloff - ci.threshold, 0.0f);
+    out[i] = __builtin_bit_cast(uint16_t, __float2half_rn(density));
+}
+
 int validate_noise(const SkyhipNoiseCreateInfo* ci) {
     if (ci == nullptr) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "noise create info is NULL");
     if (ci->base_frequency == 0 || ci->base_frequency > (1u << 20))
@@ -269,5 +299,23 @@ extern "C" int skyhip_generate_mipmaps(SkyhipContext* ctx, uint8_t* texture, int
         off += src_bytes;
         sx = dx, sy = dy, sz = dz;
     }
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_voxel_grid_generate(SkyhipContext* ctx, const SkyhipVoxelGridCreateInfo* info,
+                                          uint16_t* out_grid, int32_t dim_x, int32_t dim_y, int32_t dim_z) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (info == nullptr || out_grid == nullptr) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (dim_x < 8 || dim_y < 8 || dim_z < 8 || dim_x > 1024 || dim_y > 1024 || dim_z > 1024)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "grid dims outside [8, 1024]");
+    if (info->base_frequency < 1 || info->base_frequency > (1u << 16) || info->octaves < 1 || info->octaves > 12)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "base_frequency outside [1, 65536] or octaves outside [1, 12]");
+    if (!finite_block(&info->fbm_inv_std, 9)) return fail(SKYHIP_ERR_NON_FINITE, "non-finite grid parameter");
+    for (int k = 0; k < 3; ++k)
+        if (!(info->ellipsoid_radii[k] > 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "ellipsoid radii must be > 0");
+    long long n = (long long)dim_x * dim_y * dim_z;
+    hipLaunchKernelGGL(voxel_grid_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, ctx->stream, *info, out_grid,
+                       dim_x, dim_y, dim_z);
+    SKYHIP_LAUNCH_CHECK("voxel_grid_kernel");
     return SKYHIP_OK;
 }

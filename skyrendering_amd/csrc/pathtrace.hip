@@ -1,0 +1,708 @@
+// Voxel-cloud path tracer for gfx950 (VolumetricCloudPathTracing.comp):
+// delta tracking for free paths, ratio tracking for the sun's shadow rays,
+// Henyey-Greenstein mixture and Lambert sampling, ground bounces, the
+// environment term from the sky-view LUT and aerial perspective at the first
+// scattering event.
+//
+// One path per lane; a wave64 covers an 8x8 pixel tile (a 256-lane workgroup
+// covers 16x16) and loops over the requested frames, adding each frame's
+// sample to the pixel's accumulator in frame order as the reference's
+// per-frame imageLoad/imageStore does.  Tracking runs either with the
+// reference's single majorant kSigmaTMax, drawing the reference's random
+// sequence, or over the 8^3-brick majorant grid of an FP16 density volume,
+// walked with a 3D DDA in texel space: empty bricks cost one majorant fetch,
+// and inside occupied bricks the tentative-collision rate follows the local
+// maximum instead of the global one.
+#include "cloud_device.h"
+
+namespace skyhip {
+
+int build_material(const SkyhipCloudMaterial* m, MaterialDev& d);
+int validate_atmosphere(const SkyhipAtmosphereBufferData* b);
+int validate_lut(const void* p, int w, int h, const char* what);
+int validate_render(const SkyhipAtmosphereRenderBufferData* b);
+
+// Safety bound on tentative collisions per tracking call (never reached by a
+// finite majorant over a bounded region; keeps every wave's loop finite).
+constexpr int kMaxTrackingSteps = 1 << 20;
+
+struct PtArgs {
+    MaterialDev mat;
+    AtmDev A;
+    // Cloud local frame (VolumetricCloudCommon.glsl block)
+    F3 cam, sun;
+    float R, hb, ht, inv_thickness, half_width;
+    float inv_mvp[16];
+    float ap_max_distance;
+    // PathTracing::InitParam
+    int max_bounces, importance_sampling, prng, env;
+    float gf, gb, ratio, sigma_t_max, inv_sigma_t_max, albedo;
+    F3 model0, model1, model2;  // kModelMatrix3 columns
+    // Atmosphere renderer frame for the environment term
+    F3 a_up, a_front, a_sun, a_cam, a_ec;
+    float a_r, a_alt0, horizon_up, horizon_down;
+    // LUTs
+    const float4* T;
+    int tw, th;
+    const float4* sv_l;
+    const float4* sv_t;
+    int sw, sh;
+    const float4* ap_l;
+    const float4* ap_t;
+    int aw, ah, ad;
+    // Brick majorants: texel-space X = pos * tex_scale + tex_bias per axis (z on the flat height)
+    const float* majorants;
+    int nb[3];
+    float tex_scale[3], tex_bias[3], grid_n[3];
+    float majorant_scale;
+    // Output
+    int region[4];
+    uint32_t first_frame;
+    int frames;
+    int width, height;
+    float4* accum;
+    uint8_t* mask;
+    unsigned long long* counters;  // samples, density lookups, majorant segments, path vertices, shadow rays
+};
+
+struct PtCount {
+    uint32_t lookups = 0, segments = 0, vertices = 0, shadow_rays = 0;
+};
+
+__device__ __forceinline__ uint32_t wang_hash_pt(uint32_t seed) {  // Noise.glsl:1-9
+    seed = (seed ^ 61u) ^ (seed >> 16);
+    seed *= 9u;
+    seed = seed ^ (seed >> 4);
+    seed *= 0x27d4eb2du;
+    seed = seed ^ (seed >> 15);
+    return seed;
+}
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t seed) {  // Noise.glsl:11-15
+    uint32_t state = seed * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// Random01 (VolumetricCloudPathTracing.comp:44-48): the current state as a float in [0, 1], then
+// the state hashed forward.
+struct Rng {
+    uint32_t s;
+    int pcg;
+    __device__ __forceinline__ uint32_t hash(uint32_t x) const { return pcg ? pcg_hash(x) : wang_hash_pt(x); }
+    __device__ __forceinline__ float next() {
+        float r = float(s) * 2.3283064365386963e-10f;
+        s = hash(s);
+        return r;
+    }
+};
+
+// InfiniteTransmittanceIS numerator: -log(1 - xi).
+__device__ __forceinline__ float exp_sample(float xi) { return -__logf(1.0f - xi); }
+
+// CloudRegionIntersect (:57-75) on the slab +-half_width x [hb, ht].
+__device__ __forceinline__ void region(const PtArgs& a, F3 o, F3 d, float& t0, float& t1) {
+    float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    float x0 = (-a.half_width - o.x) * ix, x1 = (a.half_width - o.x) * ix;
+    float y0 = (-a.half_width - o.y) * iy, y1 = (a.half_width - o.y) * iy;
+    float z0 = (a.hb - o.z) * iz, z1 = (a.ht - o.z) * iz;
+    t0 = fmaxf(fmaxf(0.0f, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
+    t1 = fminf(fminf(1e7f, fmaxf(x0, x1)), fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
+}
+
+template <int kMat>
+__device__ __forceinline__ float sigma_at(const PtArgs& a, F3 p) {
+    // SampleSigmaT(P) with the flat height01 of :89-91
+    return sample_sigma_t<kMat>(a.mat, p, clamp01((p.z - a.hb) * a.inv_thickness), a.cam);
+}
+
+// Majorant-segment walker over the brick grid: the ray's texel-space position is
+// X(t) = org + t k per axis.  Bricks are 8 texels wide; the first and last brick of
+// the x and y axes extend one texel beyond the grid, where CLAMP_TO_BORDER samples
+// are still non-zero, so the walk covers [-1, N + 1] there and [0, N] in z.
+struct BrickWalk {
+    float org[3], k[3], inv_k[3];
+    int b[3];
+    float tn[3];
+    float t_end;
+
+    // Sets up the walk over [t, t_end] clipped to the grid's support; false if empty.
+    __device__ bool begin(const PtArgs& a, F3 o, F3 d, float& t, float t_stop) {
+        const float oo[3] = {o.x, o.y, o.z - a.hb}, dd[3] = {d.x, d.y, d.z};
+        float lo = t, hi = t_stop;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            org[i] = oo[i] * a.tex_scale[i] + a.tex_bias[i];
+            k[i] = dd[i] * a.tex_scale[i];
+            if (fabsf(k[i]) < 1e-30f) k[i] = 0.0f;  // keeps every boundary time finite
+            inv_k[i] = k[i] != 0.0f ? 1.0f / k[i] : 0.0f;
+            if (i < 2) {  // clip to [-1, N + 1]
+                float e0 = (-1.0f - org[i]) * inv_k[i], e1 = (a.grid_n[i] + 1.0f - org[i]) * inv_k[i];
+                if (k[i] == 0.0f) {
+                    if (org[i] < -1.0f || org[i] > a.grid_n[i] + 1.0f) return false;
+                } else {
+                    lo = fmaxf(lo, fminf(e0, e1));
+                    hi = fminf(hi, fmaxf(e0, e1));
+                }
+            }
+        }
+        if (!(lo < hi)) return false;
+        t = lo;
+        t_end = hi;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float x = fmaf(t, k[i], org[i]);
+            b[i] = min(max(int(floorf(x * 0.125f)), 0), a.nb[i] - 1);
+            tn[i] = next_boundary(a, i);
+        }
+        return true;
+    }
+    __device__ __forceinline__ float next_boundary(const PtArgs& a, int i) const {
+        if (k[i] > 0.0f) return b[i] < a.nb[i] - 1 ? (float(8 * (b[i] + 1)) - org[i]) * inv_k[i] : 3.0e38f;
+        if (k[i] < 0.0f) return b[i] > 0 ? (float(8 * b[i]) - org[i]) * inv_k[i] : 3.0e38f;
+        return 3.0e38f;
+    }
+    __device__ __forceinline__ float segment_end() const { return fminf(fminf(tn[0], tn[1]), fminf(tn[2], t_end)); }
+    __device__ __forceinline__ int brick(const PtArgs& a) const { return (b[2] * a.nb[1] + b[1]) * a.nb[0] + b[0]; }
+    template <int i>
+    __device__ __forceinline__ void step(const PtArgs& a) {
+        b[i] = min(max(b[i] + (k[i] > 0.0f ? 1 : -1), 0), a.nb[i] - 1);
+        tn[i] = next_boundary(a, i);
+    }
+    // Moves into the next brick along the axis whose boundary comes first (constant
+    // indices only, so the walker stays in registers).
+    __device__ __forceinline__ void advance(const PtArgs& a) {
+        if (tn[0] <= tn[1] && tn[0] <= tn[2])
+            step<0>(a);
+        else if (tn[1] <= tn[2])
+            step<1>(a);
+        else
+            step<2>(a);
+    }
+};
+
+// Delta tracking over [t0, t1] (:182-196); returns true with t at a real collision.
+template <int kMat, bool kBricks, bool kCount>
+__device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, float t1, float& t, PtCount& wc) {
+    if (!kBricks) {
+        if (a.sigma_t_max <= 0.0f) return false;
+        t = t0;
+        for (int it = 0; it < kMaxTrackingSteps; ++it) {
+            t += exp_sample(rng.next()) * a.inv_sigma_t_max;
+            if (t > t1) return false;
+            float s = sigma_at<kMat>(a, o + d * t);
+            if (kCount) wc.lookups++;
+            if (rng.next() < s * a.inv_sigma_t_max) return true;
+        }
+        return false;
+    }
+    BrickWalk w;
+    t = t0;
+    if (!w.begin(a, o, d, t, t1)) return false;
+    int it = 0;
+    for (;;) {
+        const float seg_end = w.segment_end();
+        const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+        if (kCount) wc.segments++;
+        if (mu > 0.0f) {
+            const float inv_mu = 1.0f / mu;
+            for (; it < kMaxTrackingSteps; ++it) {
+                float tt = t + exp_sample(rng.next()) * inv_mu;
+                if (!(tt < seg_end)) break;
+                t = tt;
+                float s = sigma_at<kMat>(a, o + d * t);
+                if (kCount) wc.lookups++;
+                if (rng.next() < s * inv_mu) return true;
+            }
+            if (it >= kMaxTrackingSteps) return false;
+        }
+        if (!(seg_end < w.t_end)) return false;
+        t = seg_end;
+        w.advance(a);
+    }
+}
+
+// TransmittanceEstimation (:135-151): ratio tracking toward the sun on a copy of the state.
+template <int kMat, bool kBricks, bool kCount>
+__device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) {
+    float t0, t1;
+    region(a, o, d, t0, t1);
+    float tr = 1.0f;
+    if (t0 >= t1) return tr;
+    if (kCount) wc.shadow_rays++;
+    if (!kBricks) {
+        float t = t0;
+        for (int it = 0; it < kMaxTrackingSteps; ++it) {
+            t += exp_sample(rng.next()) * a.inv_sigma_t_max;
+            if (t > t1) break;
+            float s = sigma_at<kMat>(a, o + d * t);
+            if (kCount) wc.lookups++;
+            tr *= 1.0f - fmaxf(0.0f, s * a.inv_sigma_t_max);
+        }
+        return clamp01(tr);
+    }
+    BrickWalk w;
+    float t = t0;
+    if (!w.begin(a, o, d, t, t1)) return tr;
+    int it = 0;
+    for (;;) {
+        const float seg_end = w.segment_end();
+        const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+        if (kCount) wc.segments++;
+        if (mu > 0.0f) {
+            const float inv_mu = 1.0f / mu;
+            for (; it < kMaxTrackingSteps; ++it) {
+                float tt = t + exp_sample(rng.next()) * inv_mu;
+                if (!(tt < seg_end)) break;
+                t = tt;
+                float s = sigma_at<kMat>(a, o + d * t);
+                if (kCount) wc.lookups++;
+                tr *= 1.0f - fmaxf(0.0f, s * inv_mu);
+            }
+            if (it >= kMaxTrackingSteps || tr == 0.0f) break;
+        }
+        if (!(seg_end < w.t_end)) break;
+        t = seg_end;
+        w.advance(a);
+    }
+    return clamp01(tr);
+}
+
+// GetSunIlluminance (:131-133) at a local-frame point.
+__device__ inline F3 sun_illuminance(const PtArgs& a, F3 p) {
+    // r^2 - R_b^2 with the cloud's planet radius R: x^2 + y^2 + z (z + 2R) + (R - R_b)(R + R_b)
+    float q = p.x * p.x + p.y * p.y + p.z * (p.z + 2.0f * a.R) + (a.R - a.A.rb) * (a.R + a.A.rb);
+    float r = sqrtf(fmaxf(q + a.A.rb * a.A.rb, 0.0f));
+    float alt = q / (r + a.A.rb);
+    float mu_s = (a.sun.x * p.x + a.sun.y * p.y + a.sun.z * (p.z + a.R)) / r;
+    return sun_visibility(a.A, a.T, a.tw, a.th, r, q, alt, mu_s) * a.A.solar;
+}
+
+// Environment term for a local-frame direction: the environment-luminance cube texel
+// (AtmosphereRenderer.glsl:253-273) at kModelMatrix3 * d, from the sky-view LUT directly.
+__device__ inline F3 environment(const PtArgs& a, F3 dl) {
+    F3 d = a.model0 * dl.x + a.model1 * dl.y + a.model2 * dl.z;
+    // GetCosLatLonFromViewDirection (:134-145)
+    float cos_lat = dot(a.a_up, d);
+    F3 lon = d - a.a_up * cos_lat;
+    float l2 = dot(lon, lon);
+    float cos_lon = l2 == 0.0f ? 1.0f : dot(lon * rsqrtf(l2), a.a_front);
+    // GetSkyViewTextureUvFromCosLatLon (:113-132), texel coordinate x (N - 1)
+    float lat = acosf(fminf(fmaxf(cos_lat, -1.0f), 1.0f));
+    float x_lat;
+    if (lat < a.horizon_up) {
+        x_lat = 0.5f - 0.5f * sqrtf(fmaxf(1.0f - lat / a.horizon_up, 0.0f));
+    } else {
+        x_lat = sqrtf(fmaxf((lat - a.horizon_up) / a.horizon_down, 0.0f)) * 0.5f + 0.5f;
+    }
+    float x_lon = sqrtf(fmaxf(0.5f - 0.5f * cos_lon, 0.0f));
+    F3 lum = lut_bilinear(a.sv_l, a.sw, a.sh, x_lon * float(a.sw - 1), x_lat * float(a.sh - 1));
+    if (ray_hits_ground(a.A, a.a_r, a.a_alt0, cos_lat)) {
+        F3 tr = lut_bilinear(a.sv_t, a.sw, a.sh, x_lon * float(a.sw - 1), x_lat * float(a.sh - 1));
+        float dist = distance_to_bottom(a.A, a.a_r, a.a_alt0, cos_lat);
+        // ComputeGroundLuminance (Atmosphere.glsl:297-306) at camera + d dist
+        F3 n = (a.a_cam - a.a_ec) + d * dist;
+        n = n * (1.0f / length3(n));
+        float mu_s = dot(a.a_sun, n);
+        F3 vis = sun_visibility(a.A, a.T, a.tw, a.th, a.A.rb, 0.0f, 0.0f, mu_s) * a.A.solar;
+        lum += (1.0f / kPiF) * clamp01(mu_s) * a.A.albedo * vis * tr;
+    }
+    return lum;
+}
+
+__device__ __forceinline__ float pt_phase(const PtArgs& a, float c) {  // GetPhase (:93-96)
+    float hb = henyey_greenstein(c, a.gb), hf = henyey_greenstein(c, a.gf);
+    return hb * (1.0f - a.ratio) + hf * a.ratio;
+}
+
+// CreateOrthonormalBasis (shaders/Base/Common.glsl:44-51, Duff et al. 2017)
+__device__ __forceinline__ void onb(F3 N, F3& t0, F3& t1) {
+    float s = N.z >= 0.0f ? 1.0f : -1.0f;
+    float aa = -1.0f / (s + N.z);
+    float b = N.x * N.y * aa;
+    t0 = {1.0f + s * N.x * N.x * aa, s * b, -s * N.x};
+    t1 = {b, s + N.y * N.y * aa, -N.y};
+}
+
+// GenerateHGSample (:98-117)
+__device__ inline F3 hg_sample(const PtArgs& a, Rng& rng, F3 d, float& value_over_pdf) {
+    if (a.importance_sampling) {
+        float g = rng.next() < a.ratio ? a.gf : a.gb;
+        // HenyeyGreensteinInvertcdf (VolumetricCloudCommon.glsl:65-71)
+        float xi = rng.next();
+        float t = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
+        float cos_theta = (0.5f / g) * (1.0f + g * g - t * t);
+        float sin_theta = sqrtf(clamp01(1.0f - cos_theta * cos_theta));
+        F3 t0, t1;
+        onb(d, t0, t1);
+        float sp, cp;
+        __sincosf(2.0f * kPiF * rng.next(), &sp, &cp);
+        value_over_pdf = 1.0f;
+        return t0 * (sin_theta * sp) + t1 * (sin_theta * cp) + d * cos_theta;
+    }
+    float sp, cp;  // UniformSphereSample (:50-55)
+    __sincosf(2.0f * kPiF * rng.next(), &sp, &cp);
+    float cos_theta = 1.0f - 2.0f * rng.next();
+    float sin_theta = sqrtf(clamp01(1.0f - cos_theta * cos_theta));
+    F3 dir = {cp * sin_theta, sp * sin_theta, cos_theta};
+    value_over_pdf = pt_phase(a, dot(d, dir)) * (4.0f * kPiF);
+    return dir;
+}
+
+// GenerateLambertSample (:119-129)
+__device__ inline F3 lambert_sample(Rng& rng, F3 N) {
+    float sin_theta = sqrtf(rng.next());
+    float cos_theta = sqrtf(clamp01(1.0f - sin_theta * sin_theta));
+    F3 t0, t1;
+    onb(N, t0, t1);
+    float sp, cp;
+    __sincosf(2.0f * kPiF * rng.next(), &sp, &cp);
+    return t0 * (sin_theta * sp) + t1 * (sin_theta * cp) + N * cos_theta;
+}
+
+// Trace (:160-249)
+template <int kMat, bool kBricks, bool kCount>
+__device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, float& scattered_t, PtCount& wc) {
+    F3 L = f3(0.0f), thr = f3(1.0f);
+    has_scattered = false;
+    scattered_t = 0.0f;
+    F3 o = a.cam, d = view;
+    float c0, c1;
+    region(a, o, d, c0, c1);
+    if (c0 >= c1) return L;
+    o = o + d * c0;
+    for (int istep = 0; istep < a.max_bounces && fmaxf(thr.x, fmaxf(thr.y, thr.z)) > 0.0f; ++istep) {
+        float t0, t1;
+        region(a, o, d, t0, t1);
+        if (t0 >= t1) break;
+        float t;
+        if (!delta_track<kMat, kBricks, kCount>(a, rng, o, d, t0, t1, t, wc)) {
+            if (a.env == SKYHIP_ENV_OFF || !has_scattered) break;
+            if (a.env == SKYHIP_ENV_CONST_ENVIRONMENT_MAP) {
+                L += thr * environment(a, d);
+                break;
+            }
+            float q = o.x * o.x + o.y * o.y + o.z * (o.z + 2.0f * a.R) + (a.R - a.A.rb) * (a.R + a.A.rb);
+            float r = sqrtf(fmaxf(q + a.A.rb * a.A.rb, 0.0f));
+            float alt = q / (r + a.A.rb);
+            float mu = (d.x * o.x + d.y * o.y + d.z * (o.z + a.R)) / r;
+            if (!ray_hits_ground(a.A, r, alt, mu)) {
+                L += thr * environment(a, d);
+                break;
+            }
+            o = o + d * distance_to_bottom(a.A, r, alt, mu);
+            F3 N = F3{o.x, o.y, o.z + a.R};
+            N = N * (1.0f / length3(N));
+            if (kCount) wc.vertices++;
+            F3 bsdf = a.A.albedo * ((1.0f / kPiF) * dot(N, a.sun));
+            L += thr * (ratio_track<kMat, kBricks, kCount>(a, rng, o, a.sun, wc) * sun_illuminance(a, o) * bsdf);
+            if (a.env == SKYHIP_ENV_GROUND_SINGLE_BOUNCE) break;
+            d = lambert_sample(rng, N);
+            thr *= a.A.albedo;
+        } else {
+            if (!has_scattered) scattered_t = length3(o - a.cam);  // the segment start, as :232-233
+            has_scattered = true;
+            o = o + d * t;
+            if (kCount) wc.vertices++;
+            float ph = pt_phase(a, dot(d, a.sun));
+            L += thr * (ratio_track<kMat, kBricks, kCount>(a, rng, o, a.sun, wc) * ph * sun_illuminance(a, o));
+            float vop;
+            d = hg_sample(a, rng, d, vop);
+            thr = thr * (vop * a.albedo);
+        }
+    }
+    return L;
+}
+
+template <int kMat, bool kBricks, bool kCount>
+__global__ __launch_bounds__(256) void path_trace_kernel(PtArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x = a.region[0] + blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int y = a.region[1] + blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    PtCount wc;
+    const bool active = x < a.region[2] && y < a.region[3];
+    if (active) {
+        const float u = (float(x) + 0.5f) / float(a.width), v = (float(y) + 0.5f) / float(a.height);
+        float frag_dist;
+        const F3 view = unproject(a.inv_mvp, u * 2.0f - 1.0f, v * 2.0f - 1.0f, 1.0f, a.cam, &frag_dist);
+        const long long pix = (long long)y * a.width + x;
+        float4 acc = a.accum[pix];
+        Rng rng;
+        rng.pcg = a.prng;
+        const uint32_t seed_xy = rng.hash(rng.hash(uint32_t(x)) + uint32_t(y));
+        for (int f = 0; f < a.frames; ++f) {
+            rng.s = rng.hash(seed_xy + a.first_frame + uint32_t(f));  // :260
+            bool hs;
+            float st;
+            F3 L = trace<kMat, kBricks, kCount>(a, rng, view, hs, st, wc);
+            if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98), unit froxel visibility
+                float r = a.cam.z + a.R, mu = view.z;
+                float t = st;
+                if (r > a.A.rt) {
+                    float disc = r * r * (mu * mu - 1.0f) + a.A.rt * a.A.rt;
+                    t = (mu < 0.0f && disc >= 0.0f) ? t - fmaxf(-r * mu - sqrtf(disc), 0.0f) : 0.0f;
+                }
+                float w = sqrtf(t / a.ap_max_distance);
+                float fx = u * float(a.aw - 1), fy = v * float(a.ah - 1), fz = w * float(a.ad - 1);
+                F3 apt = lut3d_trilinear(a.ap_t, a.aw, a.ah, a.ad, fx, fy, fz);
+                F3 apl = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, fx, fy, fz);
+                L = L * apt + apl;
+            }
+            acc.x += L.x;
+            acc.y += L.y;
+            acc.z += L.z;
+            acc.w += hs ? 0.0f : 1.0f;
+        }
+        a.accum[pix] = acc;
+        if (a.mask) a.mask[pix] = 1;
+    }
+    if (kCount) {
+        unsigned long long v[5] = {active ? (unsigned long long)a.frames : 0ull, wc.lookups, wc.segments,
+                                   wc.vertices, wc.shadow_rays};
+        for (int i = 0; i < 5; ++i) {
+            unsigned long long s = v[i];
+            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) atomicAdd(&a.counters[i], s);
+        }
+    }
+}
+
+// Display pass (:288-296).
+__global__ __launch_bounds__(256) void path_trace_display_kernel(const float4* __restrict__ accum,
+                                                                 const uint8_t* __restrict__ mask, uint32_t frame,
+                                                                 uint16_t* __restrict__ display, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 s = accum[i];
+    bool rendered = mask ? mask[i] != 0 : true;
+    float inv = 1.0f / float(rendered ? frame : frame - 1u);
+    float4 avg = make_float4(s.x * inv, s.y * inv, s.z * inv, s.w * inv);
+    uint16_t* px = display + 4 * (long long)i;
+    float c[3];
+    for (int k = 0; k < 3; ++k) c[k] = half_bits_to_float(px[k]);
+    float out[3] = {c[0] * avg.w + avg.x, c[1] * avg.w + avg.y, c[2] * avg.w + avg.z};
+    for (int k = 0; k < 3; ++k) px[k] = __builtin_bit_cast(uint16_t, __float2half_rn(out[k]));
+}
+
+// Brick majorants of an FP16 grid: max over the brick plus a one-texel apron.
+__global__ __launch_bounds__(256) void majorant_kernel(const uint16_t* __restrict__ g, int nx, int ny, int nz,
+                                                       int bx, int by, int bz, float* __restrict__ out) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= bx * by * bz) return;
+    int x = i % bx, y = (i / bx) % by, z = i / (bx * by);
+    int x0 = max(8 * x - 1, 0), x1 = min(8 * x + 8, nx - 1);
+    int y0 = max(8 * y - 1, 0), y1 = min(8 * y + 8, ny - 1);
+    int z0 = max(8 * z - 1, 0), z1 = min(8 * z + 8, nz - 1);
+    float m = 0.0f;
+    for (int zz = z0; zz <= z1; ++zz)
+        for (int yy = y0; yy <= y1; ++yy) {
+            const uint16_t* row = g + ((long long)zz * ny + yy) * nx;
+            for (int xx = x0; xx <= x1; ++xx) m = fmaxf(m, half_bits_to_float(row[xx]));
+        }
+    out[i] = m;
+}
+
+template <bool kBricks, bool kCount>
+static void launch_path_trace(hipStream_t st, const PtArgs& a) {
+    dim3 grid(div_up(a.region[2] - a.region[0], 16), div_up(a.region[3] - a.region[1], 16));
+    switch (a.mat.kind) {
+        case SKYHIP_MATERIAL_DEFAULT0:
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT0, false, kCount>), grid, dim3(256), 0, st,
+                               a);
+            break;
+        case SKYHIP_MATERIAL_DEFAULT1:
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1, false, kCount>), grid, dim3(256), 0, st,
+                               a);
+            break;
+        case SKYHIP_MATERIAL_MINIMAL:
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_MINIMAL, false, kCount>), grid, dim3(256), 0, st,
+                               a);
+            break;
+        case SKYHIP_MATERIAL_VOXEL:
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL, false, kCount>), grid, dim3(256), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount>), grid, dim3(256), 0,
+                               st, a);
+            break;
+    }
+}
+
+}  // namespace skyhip
+
+using namespace skyhip;
+
+extern "C" int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
+                                           int32_t dim_z, float* out_majorants) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!grid || !out_majorants) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (dim_x < 2 || dim_y < 2 || dim_z < 2 || dim_x > 4096 || dim_y > 4096 || dim_z > 4096)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "grid dims outside [2, 4096]");
+    int bx = div_up(dim_x, 8), by = div_up(dim_y, 8), bz = div_up(dim_z, 8);
+    hipLaunchKernelGGL(majorant_kernel, dim3(div_up(bx * by * bz, 256)), dim3(256), 0, ctx->stream, grid, dim_x,
+                       dim_y, dim_z, bx, by, bz, out_majorants);
+    SKYHIP_LAUNCH_CHECK("majorant_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                                       const SkyhipAtmosphereRenderBufferData* render,
+                                       const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                                       const SkyhipPathTracingParameters* params, const SkyhipPathTracingInputs* in,
+                                       const int32_t region[4], uint32_t first_frame, int32_t frames,
+                                       float* accumulation_rgba, uint8_t* rendered_mask, int32_t width,
+                                       int32_t height, unsigned long long* work_counters) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!atmosphere || !render || !common || !params || !in || !region || !accumulation_rgba)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    SKYHIP_CHECK(validate_atmosphere(atmosphere));
+    SKYHIP_CHECK(validate_render(render));
+    if (!finite_block(reinterpret_cast<const float*>(common->uInvMVP), 16) || !finite_block(common->uCameraPos, 3) ||
+        !finite_block(common->uSunDirection, 3) || !std::isfinite(common->uEarthRadius) ||
+        !std::isfinite(common->uBottomAltitude) || !std::isfinite(common->uTopAltitude))
+        return fail(SKYHIP_ERR_NON_FINITE, "cloud common block has a non-finite field");
+    if (!(common->uAerialPerspectiveLutMaxDistance > 0.0f))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "aerial-perspective max distance must be > 0");
+    if (!finite_block(reinterpret_cast<const float*>(params), sizeof(*params) / 4))
+        return fail(SKYHIP_ERR_NON_FINITE, "path-tracing parameters have a non-finite field");
+    if (width < 1 || height < 1 || width > 16384 || height > 16384)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "image size outside [1, 16384]");
+    if (region[0] < 0 || region[1] < 0 || region[2] > width || region[3] > height || region[0] > region[2] ||
+        region[1] > region[3])
+        return fail(SKYHIP_ERR_INVALID_SIZE, "render region outside the image");
+    if (frames < 0 || frames > 65536) return fail(SKYHIP_ERR_INVALID_SIZE, "frames outside [0, 65536]");
+    if (params->max_bounces < 0 || params->max_bounces > 4096)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "max_bounces outside [0, 4096]");
+    if (params->prng != SKYHIP_PRNG_WANG_HASH && params->prng != SKYHIP_PRNG_PCG_HASH)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "unknown prng %d", params->prng);
+    if (params->environment_lighting < SKYHIP_ENV_OFF || params->environment_lighting > SKYHIP_ENV_GROUND_MULTI_BOUNCE)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "unknown environment lighting %d", params->environment_lighting);
+    if (params->importance_sampling &&
+        (params->forward_phase_g == 0.0f || params->back_phase_g == 0.0f || fabsf(params->forward_phase_g) >= 1.0f ||
+         fabsf(params->back_phase_g) >= 1.0f))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "HG importance sampling needs 0 < |g| < 1");
+    if (!(params->region_box_half_width > 0.0f) || !(common->uTopAltitude > common->uBottomAltitude))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "empty cloud region");
+    SKYHIP_CHECK(validate_lut(in->transmittance, in->transmittance_width, in->transmittance_height, "transmittance"));
+    SKYHIP_CHECK(validate_lut(in->sky_view_luminance, in->sky_view_width, in->sky_view_height, "sky-view luminance"));
+    SKYHIP_CHECK(validate_lut(in->sky_view_transmittance, in->sky_view_width, in->sky_view_height,
+                              "sky-view transmittance"));
+    SKYHIP_CHECK(validate_lut(in->ap_luminance, in->ap_width, in->ap_height, "aerial-perspective luminance"));
+    SKYHIP_CHECK(validate_lut(in->ap_transmittance, in->ap_width, in->ap_height, "aerial-perspective transmittance"));
+    if (in->ap_depth < 2 || in->ap_depth > 1024) return fail(SKYHIP_ERR_INVALID_SIZE, "aerial-perspective depth");
+
+    PtArgs a;
+    std::memset(&a, 0, sizeof(a));
+    SKYHIP_CHECK(build_material(material, a.mat));
+    const bool bricks = in->brick_majorants != nullptr;
+    if (bricks && a.mat.kind != SKYHIP_MATERIAL_VOXEL_F16)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "brick majorants need a VOXEL_F16 material");
+    if (!bricks && !(params->sigma_t_max >= 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "sigma_t_max < 0");
+    if (!(params->albedo >= 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "albedo < 0");
+
+    a.A = make_atm(*atmosphere);
+    a.cam = f3(common->uCameraPos);
+    a.sun = f3(common->uSunDirection);
+    a.R = common->uEarthRadius;
+    a.hb = common->uBottomAltitude;
+    a.ht = common->uTopAltitude;
+    a.inv_thickness = 1.0f / (common->uTopAltitude - common->uBottomAltitude);
+    a.half_width = params->region_box_half_width;
+    std::memcpy(a.inv_mvp, common->uInvMVP, sizeof(a.inv_mvp));
+    a.ap_max_distance = common->uAerialPerspectiveLutMaxDistance;
+    a.max_bounces = params->max_bounces;
+    a.importance_sampling = params->importance_sampling;
+    a.prng = params->prng;
+    a.env = params->environment_lighting;
+    a.gf = params->forward_phase_g;
+    a.gb = params->back_phase_g;
+    a.ratio = params->forward_scattering_ratio;
+    a.sigma_t_max = params->sigma_t_max;
+    a.inv_sigma_t_max = params->sigma_t_max > 0.0f ? 1.0f / params->sigma_t_max : 0.0f;
+    a.albedo = params->albedo;
+    a.model0 = {params->model3[0], params->model3[1], params->model3[2]};
+    a.model1 = {params->model3[3], params->model3[4], params->model3[5]};
+    a.model2 = {params->model3[6], params->model3[7], params->model3[8]};
+    a.a_up = f3(render->up_direction);
+    a.a_front = f3(render->front_direction);
+    a.a_sun = f3(render->sun_direction);
+    a.a_cam = f3(render->camera_position);
+    a.a_ec = f3(render->earth_center);
+    {
+        // Camera radius and horizon angles of the sky-view parameterisation
+        // (GetHorizonDownAngleFromR, AtmosphereRenderer.glsl:74-78), in FP64.
+        double r = render->camera_earth_center_distance, rb = atmosphere->bottom_radius;
+        double alt0 = r - rb;
+        a.a_r = float(r);
+        a.a_alt0 = float(alt0);
+        double down = std::acos(std::sqrt(std::fmax(alt0 * (r + rb), 0.0)) / r);
+        a.horizon_down = float(down);
+        a.horizon_up = float(3.14159265358979323846 - down);
+    }
+    a.T = reinterpret_cast<const float4*>(in->transmittance);
+    a.tw = in->transmittance_width;
+    a.th = in->transmittance_height;
+    a.sv_l = reinterpret_cast<const float4*>(in->sky_view_luminance);
+    a.sv_t = reinterpret_cast<const float4*>(in->sky_view_transmittance);
+    a.sw = in->sky_view_width;
+    a.sh = in->sky_view_height;
+    a.ap_l = reinterpret_cast<const float4*>(in->ap_luminance);
+    a.ap_t = reinterpret_cast<const float4*>(in->ap_transmittance);
+    a.aw = in->ap_width;
+    a.ah = in->ap_height;
+    a.ad = in->ap_depth;
+    if (bricks) {
+        const SkyhipCloudVoxelBufferData& vb = material->voxel;
+        a.majorants = in->brick_majorants;
+        for (int k = 0; k < 3; ++k) {
+            a.nb[k] = div_up(material->voxel_grid_dim[k], 8);
+            a.grid_n[k] = float(material->voxel_grid_dim[k]);
+        }
+        // X = (pos * freq + bias) * N ; Z = (z - hb) / (ht - hb) * N (BrickWalk subtracts hb)
+        a.tex_scale[0] = vb.uSampleFrequency[0] * a.grid_n[0];
+        a.tex_bias[0] = vb.uSampleBias[0] * a.grid_n[0];
+        a.tex_scale[1] = vb.uSampleFrequency[1] * a.grid_n[1];
+        a.tex_bias[1] = vb.uSampleBias[1] * a.grid_n[1];
+        a.tex_scale[2] = a.inv_thickness * a.grid_n[2];
+        a.tex_bias[2] = 0.0f;
+        a.majorant_scale = vb.uDensity;
+        if (!(vb.uDensity >= 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "voxel density < 0");
+    }
+    for (int k = 0; k < 4; ++k) a.region[k] = region[k];
+    a.first_frame = first_frame;
+    a.frames = frames;
+    a.width = width;
+    a.height = height;
+    a.accum = reinterpret_cast<float4*>(accumulation_rgba);
+    a.mask = rendered_mask;
+    a.counters = work_counters;
+    if (region[2] == region[0] || region[3] == region[1] || frames == 0) return SKYHIP_OK;
+    if (bricks) {
+        if (work_counters)
+            launch_path_trace<true, true>(ctx->stream, a);
+        else
+            launch_path_trace<true, false>(ctx->stream, a);
+    } else {
+        if (work_counters)
+            launch_path_trace<false, true>(ctx->stream, a);
+        else
+            launch_path_trace<false, false>(ctx->stream, a);
+    }
+    SKYHIP_LAUNCH_CHECK("path_trace_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_cloud_path_trace_display(SkyhipContext* ctx, const float* accumulation_rgba,
+                                               const uint8_t* rendered_mask, uint32_t frame_id,
+                                               void* display_rgba16f, int32_t width, int32_t height) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!accumulation_rgba || !display_rgba16f) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (width < 1 || height < 1 || width > 16384 || height > 16384)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "image size outside [1, 16384]");
+    if (frame_id < 1) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "frame_id starts at 1");
+    int n = width * height;
+    hipLaunchKernelGGL(path_trace_display_kernel, dim3(div_up(n, 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(accumulation_rgba), rendered_mask, frame_id,
+                       reinterpret_cast<uint16_t*>(display_rgba16f), n);
+    SKYHIP_LAUNCH_CHECK("path_trace_display_kernel");
+    return SKYHIP_OK;
+}

@@ -164,3 +164,36 @@ def uniform_weather_map(size=512, seeds=(2, 3)) -> np.ndarray:
 
 def uniform_blue_noise(size=128, seed=11) -> np.ndarray:
     return np.random.default_rng(seed).integers(0, 65536, size=(size, size), dtype=np.uint16)
+
+
+# ---------------------------------------------------------------------------
+# Voxel-cloud path-tracing workload: 1024x1024, 64 samples per pixel (64 frames of one
+# sample), at most 32 bounces, single-scattering albedo 0.99, Henyey-Greenstein g = 0.85
+# (forward lobe only), PCGHash chain, ground multi-bounce environment with the sky term
+# from the 192x108 sky-view LUT of the Earth preset, sun zenith 70 degrees.  Density:
+# a 512^3 FP16 grid (seeded 5-octave Perlin fBm, lognormal sigma 1.2, ellipsoidal
+# falloff, threshold 1) spanning 8 x 8 km over the 1.5-4 km cloud layer, centred 6 km
+# ahead of the camera, extinction = grid value x 8 / km.  Tracking: 8^3-brick majorants.
+# Region box half width 14 km (encloses the grid; the reference's default is 100 km).
+# ---------------------------------------------------------------------------
+def path_trace_workload():
+    from .cloud import VolumetricCloud
+    from .pathtrace import PathTracingInitParam, VoxelGridCreateInfo
+    cloud = VolumetricCloud(bottom_altitude=1.5, thickness=2.5)
+    init = PathTracingInitParam(max_bounces=32, region_box_half_width=14.0, importance_sampling=True,
+                                forward_phase_g=0.85, back_phase_g=-0.15, forward_scattering_ratio=1.0,
+                                prng="PCGHash", environment_lighting="GROUND_MULTI_BOUNCE", albedo=0.99)
+    return {"cloud": cloud, "camera": bench_camera(aspect=1.0), "width": 1024, "height": 1024, "spp": 64,
+            "sun_theta": 70.0, "sun_phi": 180.0, "atmosphere": earth_100km(), "grid": VoxelGridCreateInfo(),
+            "grid_dims": (512, 512, 512), "grid_extent_km": 8.0, "grid_distance_km": 6.0, "density": 8.0,
+            "init": init}
+
+
+def grid_center_ahead(model, camera: Camera, distance_km: float):
+    """Local-frame (x, y) point `distance_km` ahead of the camera's horizontal heading.
+    model: the cloud's column-major local-to-world matrix (VolumetricCloud::Update)."""
+    m = np.asarray(model, dtype=np.float64).reshape(4, 4)  # m[col][row]
+    front, _, _ = camera.basis()
+    local = np.array([sum(m[c][r] * front[r] for r in range(3)) for c in range(3)])
+    h = local[:2] / np.linalg.norm(local[:2])
+    return (float(h[0] * distance_km), float(h[1] * distance_km))
