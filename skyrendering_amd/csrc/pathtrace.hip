@@ -25,6 +25,9 @@ int validate_render(const SkyhipAtmosphereRenderBufferData* b);
 // Safety bound on tentative collisions per tracking call (never reached by a
 // finite majorant over a bounded region; keeps every wave's loop finite).
 constexpr int kMaxTrackingSteps = 1 << 20;
+// Safety bound on majorant segments per walk (a walk crosses at most the sum of the
+// brick-grid dimensions, 384 at the largest grid).
+constexpr int kMaxSegments = 4096;
 
 struct PtArgs {
     MaterialDev mat;
@@ -199,7 +202,7 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
     t = t0;
     if (!w.begin(a, o, d, t, t1)) return false;
     int it = 0;
-    for (;;) {
+    for (int seg = 0; seg < kMaxSegments; ++seg) {
         const float seg_end = w.segment_end();
         const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
         if (kCount) wc.segments++;
@@ -219,6 +222,7 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
         t = seg_end;
         w.advance(a);
     }
+    return false;
 }
 
 // TransmittanceEstimation (:135-151): ratio tracking toward the sun on a copy of the state.
@@ -244,7 +248,7 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
     float t = t0;
     if (!w.begin(a, o, d, t, t1)) return tr;
     int it = 0;
-    for (;;) {
+    for (int seg = 0; seg < kMaxSegments; ++seg) {
         const float seg_end = w.segment_end();
         const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
         if (kCount) wc.segments++;
