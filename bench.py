@@ -10,8 +10,11 @@ around exactly --steps passes after --warmup passes.
 
 Printed as one JSON line.  Also reported: the Earth LUT set (transmittance 256x64/40,
 multiple scattering 32x32x64/30, sky view 192x108/30, aerial perspective 32^3/50.867)
-in ms per set over 16 sun angles; the roofline of the raymarch kernel; and the CPU
-oracle's FP32 OpenMP path on a bounded sample of the same frame.
+in ms per set over 16 sun angles; the roofline of the raymarch kernel; the voxel-cloud
+path tracer (1024x1024, 64 frames of one sample, brick majorants, with the single-majorant
+tracker timed beside it); the batched LUT build (256 atmosphere sets x 8 suns, mixed sizes,
+one hipGraph launch, autotuned); and for each the CPU oracle's FP32 OpenMP path on a
+bounded sample of the same work.
 """
 import argparse
 import json
@@ -48,6 +51,7 @@ def parse():
     ap.add_argument("--lut-rounds", type=int, default=20)
     ap.add_argument("--pt-frames", type=int, default=64, help="path-tracer frames (1 sample each) timed")
     ap.add_argument("--no-pathtrace", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     return ap.parse_args()
 
 
@@ -121,6 +125,70 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
         "single_majorant_density_lookups_per_frame": ref_counts["lookups"],
         "brick_speedup_vs_single_majorant": round(ref_ms / ms, 2),
     }
+
+
+FLOP_PER_INTEGRATOR_STEP = 150  # sky-view / MS step: 2 LUT fetches, ~7 transcendentals, RGB accumulation
+
+
+def bench_lut_batch(ctx, launches=5):
+    """The batched LUT workload (256 atmosphere sets x 8 suns, mixed sizes): autotuned, then
+    `launches` graph launches timed with HIP events."""
+    from skyrendering_amd import presets
+    from skyrendering_amd.batch import LutBatch
+    sets, views = presets.batched_lut_workload()
+    b = LutBatch(ctx, sets, views)
+    b.launch()
+    ctx.synchronize()
+    ctx.record(8)
+    b.launch()
+    ctx.record(9)
+    default_ms = ctx.elapsed_ms(8, 9)
+    tuned, tuned_pass_ms = b.autotune(repetitions=2)
+    b.launch()
+    ctx.synchronize()
+    ctx.record(8)
+    for _ in range(launches):
+        b.launch()
+    ctx.record(9)
+    ms = ctx.elapsed_ms(8, 9) / launches
+    steps = b.integrator_steps
+    return b, sets, views, {
+        "workload": "256 atmosphere sets x 8 sun zeniths, T 128x32|256x64 (40 steps), MS 32x32x64 (30), "
+                    "SV 96x54|192x108|384x216 (30), one hipGraph launch",
+        "ms_per_launch": round(ms, 3), "default_tuning_ms": round(default_ms, 3),
+        "integrator_steps": steps, "gsteps_per_s": round(steps / (ms * 1e-3) / 1e9, 2),
+        "tuning": tuned, "tuned_pass_sum_ms": round(tuned_pass_ms, 3),
+        "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak", "unit": "TFLOP/s",
+                     "flop_per_step": FLOP_PER_INTEGRATOR_STEP,
+                     "achieved": round(steps * FLOP_PER_INTEGRATOR_STEP / (ms * 1e-3) / 1e12, 3),
+                     "peak": FP32_PEAK_TFLOPS,
+                     "frac": round(steps * FLOP_PER_INTEGRATOR_STEP / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+    }
+
+
+def cpu_baseline_lut_batch(sets, views, target_seconds):
+    """The oracle's FP32 LUT passes (OpenMP over texels) on the first sets of the batch and
+    their views, until about `target_seconds`; integrator steps per second."""
+    import oracle
+    steps, t_total, done_sets, done_views = 0, 0.0, 0, 0
+    for s, (atm, (tw, th), (mw, mh)) in enumerate(sets):
+        t0 = time.perf_counter()
+        T = oracle.transmittance_lut(atm, tw, th, "f32")
+        M = oracle.multiscattering_lut(atm, T, mw, mh, "f32")
+        steps += tw * th * 40 + mw * mh * 64 * 30
+        for v, (vs, render, (sw, sh)) in enumerate(views):
+            if vs != s:
+                continue
+            oracle.sky_view_lut(atm, render, T, M, sw, sh, precision="f32")
+            steps += sw * sh * 30
+            done_views += 1
+        t_total += time.perf_counter() - t0
+        done_sets += 1
+        if t_total >= target_seconds:
+            break
+    return {"value": round(steps / t_total / 1e9, 4), "unit": "Gsteps/s", "cores": oracle.max_threads(),
+            "kind": "port", "sample": f"first {done_sets} sets and their {done_views} views of the batch "
+                                      f"({steps} integrator steps), oracle FP32 OpenMP, {t_total:.1f} s"}
 
 
 def cpu_baseline_pathtrace(scene, target_seconds):
@@ -226,10 +294,15 @@ def main():
     pt_scene = None
     if not args.no_pathtrace:
         pt_scene, out["path_trace"] = bench_pathtrace(ctx, args.pt_frames)
+    batch = None
+    if not args.no_batch:
+        batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         if pt_scene is not None:
             out["path_trace"]["cpu_baseline"] = cpu_baseline_pathtrace(pt_scene, args.cpu_seconds / 2)
+        if batch is not None:
+            out["lut_batch"]["cpu_baseline"] = cpu_baseline_lut_batch(b_sets, b_views, args.cpu_seconds / 2)
     print(json.dumps(out))
 
 

@@ -629,6 +629,75 @@ int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmosphereBufferData
 int skyhip_cloud_path_trace_display(SkyhipContext* ctx, const float* accumulation_rgba, const uint8_t* rendered_mask,
                                     uint32_t frame_id, void* display_rgba16f, int32_t width, int32_t height);
 
+/* ------------------------------------------------------------------------- */
+/* Batched LUT builds (no reference counterpart: the reference builds one     */
+/* atmosphere's LUTs per frame, Atmosphere::UpdateLuts and                    */
+/* AtmosphereRenderer::Render; these build many at once)                      */
+/* ------------------------------------------------------------------------- */
+
+/* One atmosphere set: its block and the sizes of its transmittance and
+ * multiple-scattering LUTs (each built exactly as skyhip_transmittance_lut /
+ * skyhip_multiscattering_lut build them). */
+typedef struct SkyhipLutBatchSet {
+    SkyhipAtmosphereBufferData atmosphere;
+    int32_t transmittance_width;
+    int32_t transmittance_height;
+    int32_t multiscattering_width;
+    int32_t multiscattering_height;
+} SkyhipLutBatchSet;
+
+/* One view of a set (a sun direction and camera): its renderer block and the
+ * sky-view LUT size (built exactly as skyhip_sky_view_lut builds it, without dither). */
+typedef struct SkyhipLutBatchView {
+    int32_t set;
+    int32_t sky_view_width;
+    int32_t sky_view_height;
+    int32_t padding;
+    SkyhipAtmosphereRenderBufferData render;
+} SkyhipLutBatchView;
+
+/* Launch shape of the three persistent passes: workgroup sizes (64/128/256; multiple
+ * scattering: waves of 64 lanes per workgroup, 1/2/4/8), texels per tile, resident
+ * workgroups per CU, and whether sky-view tiles of the largest LUTs go first. */
+typedef struct SkyhipLutBatchTuning {
+    int32_t transmittance_block;
+    int32_t transmittance_tile;
+    int32_t transmittance_blocks_per_cu;
+    int32_t multiscattering_waves;
+    int32_t multiscattering_blocks_per_cu;
+    int32_t sky_view_block;
+    int32_t sky_view_tile;
+    int32_t sky_view_blocks_per_cu;
+    int32_t sky_view_largest_first;
+} SkyhipLutBatchTuning;
+
+typedef struct SkyhipLutBatch SkyhipLutBatch;
+
+/* Uploads the sets and views (host arrays), allocates every output LUT in device pools
+ * and captures the build into one hipGraph (default tuning).  Synchronous. */
+int skyhip_lut_batch_create(SkyhipContext* ctx, const SkyhipLutBatchSet* sets, int32_t set_count,
+                            const SkyhipLutBatchView* views, int32_t view_count, SkyhipLutBatch** out_batch);
+int skyhip_lut_batch_destroy(SkyhipLutBatch* batch);
+/* The whole build (every set's T and MS, then every view's sky view): one hipGraphLaunch
+ * on the context's stream. */
+int skyhip_lut_batch_launch(SkyhipLutBatch* batch);
+int skyhip_lut_batch_get_tuning(const SkyhipLutBatch* batch, SkyhipLutBatchTuning* out_tuning);
+/* Re-captures the graph with the given tuning.  Synchronous. */
+int skyhip_lut_batch_set_tuning(SkyhipLutBatch* batch, const SkyhipLutBatchTuning* tuning);
+/* Times each pass's candidate launch shapes on this batch (median of `repetitions` runs,
+ * two coordinate-descent sweeps), keeps the fastest and re-captures the graph.  out_ms:
+ * the sum of the three passes' best times.  Synchronous; leaves valid outputs. */
+int skyhip_lut_batch_autotune(SkyhipLutBatch* batch, int32_t repetitions, SkyhipLutBatchTuning* out_best,
+                              float* out_ms);
+/* Device pointers to a set's / view's output LUTs (RGBA32F, the sizes given at creation). */
+int skyhip_lut_batch_set_outputs(SkyhipLutBatch* batch, int32_t set, const float** transmittance_rgba,
+                                 const float** multiscattering_rgba);
+int skyhip_lut_batch_view_outputs(SkyhipLutBatch* batch, int32_t view, const float** luminance_rgba,
+                                  const float** sky_transmittance_rgba);
+/* Integrator steps of one launch: T texels x transmittance loop iterations + MS texels x 64
+ * directions x MS iterations + sky-view texels x sky-view iterations. */
+int skyhip_lut_batch_integrator_steps(const SkyhipLutBatch* batch, unsigned long long* out_steps);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -214,7 +214,27 @@ class VoxelGridCreateInfoC(C.Structure):
     ]
 
 
+class LutBatchSetC(C.Structure):
+    _fields_ = [("atmosphere", AtmosphereBufferData), ("transmittance_width", i32), ("transmittance_height", i32),
+                ("multiscattering_width", i32), ("multiscattering_height", i32)]
+
+
+class LutBatchViewC(C.Structure):
+    _fields_ = [("set", i32), ("sky_view_width", i32), ("sky_view_height", i32), ("padding", i32),
+                ("render", AtmosphereRenderBufferData)]
+
+
+class LutBatchTuningC(C.Structure):
+    _fields_ = [(n, i32) for n in (
+        "transmittance_block", "transmittance_tile", "transmittance_blocks_per_cu", "multiscattering_waves",
+        "multiscattering_blocks_per_cu", "sky_view_block", "sky_view_tile", "sky_view_blocks_per_cu",
+        "sky_view_largest_first")]
+
+
 assert C.sizeof(AtmosphereBufferData) == 128
+assert C.sizeof(LutBatchSetC) == 144
+assert C.sizeof(LutBatchViewC) == 336
+assert C.sizeof(LutBatchTuningC) == 36
 assert C.sizeof(AtmosphereRenderBufferData) == 320
 assert C.sizeof(CloudCommonBufferData) == 384
 assert C.sizeof(CloudBufferData) == 64

@@ -80,6 +80,12 @@ class Context:
         a = np.ascontiguousarray(array)
         return DeviceBuffer(self, max(a.nbytes, 1)).upload(a)
 
+    def download_ptr(self, ptr: int, dtype, shape) -> np.ndarray:
+        """Copies `shape` elements from a device pointer the library handed out."""
+        out = np.empty(shape, dtype=dtype)
+        _abi.check(_abi.lib().skyhip_memcpy_d2h(self.handle, out.ctypes.data, ptr, out.nbytes))
+        return out
+
     def synchronize(self):
         _abi.check(_abi.lib().skyhip_synchronize(self.handle))
 

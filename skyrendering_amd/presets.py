@@ -197,3 +197,49 @@ def grid_center_ahead(model, camera: Camera, distance_km: float):
     local = np.array([sum(m[c][r] * front[r] for r in range(3)) for c in range(3)])
     h = local[:2] / np.linalg.norm(local[:2])
     return (float(h[0] * distance_km), float(h[1] * distance_km))
+
+
+# ---------------------------------------------------------------------------
+# Batched LUT workload: 256 atmosphere sets drawn with numpy PCG64 seed 5 from the
+# 100 km Earth preset (Rayleigh and Mie scattering scales x U[0.25, 4] each, the Mie
+# absorption scale with the Mie factor, Rayleigh and Mie scale heights x U[0.5, 2],
+# bottom radius U[3000, 7000] km, thickness fixed at 100 km); 8 sun zenith angles per
+# set stratified over [0, 95] degrees (seed 6), azimuth 180, camera 1 km above the
+# ground; per set a sky-view size from {96x54, 192x108, 384x216} and a transmittance
+# size from {128x32, 256x64} (seed 7); multiple scattering 32x32; 40 / 30 / 30 steps.
+# ---------------------------------------------------------------------------
+BATCH_SKY_VIEW_SIZES = ((96, 54), (192, 108), (384, 216))
+BATCH_TRANSMITTANCE_SIZES = ((128, 32), (256, 64))
+
+
+def batched_lut_workload(n_sets=256, suns_per_set=8):
+    """Returns (sets, views) for skyrendering_amd.batch.LutBatch."""
+    r5 = np.random.default_rng(5)
+    r6 = np.random.default_rng(6)
+    r7 = np.random.default_rng(7)
+    cam = bench_camera()
+    vp = cam.view_projection()
+    sets, views = [], []
+    for s in range(n_sets):
+        p = earth_100km()
+        ray, mie = r5.uniform(0.25, 4.0), r5.uniform(0.25, 4.0)
+        hr, hm = r5.uniform(0.5, 2.0), r5.uniform(0.5, 2.0)
+        rb = r5.uniform(3000.0, 7000.0)
+        p.rayleigh_scattering_scale *= ray
+        p.mie_scattering_scale *= mie
+        p.mie_absorption_scale *= mie
+        p.rayleigh_exponential_distribution *= hr
+        p.mie_exponential_distribution *= hm
+        p.bottom_radius = rb
+        sv = BATCH_SKY_VIEW_SIZES[int(r7.integers(0, 3))]
+        tsz = BATCH_TRANSMITTANCE_SIZES[int(r7.integers(0, 2))]
+        sets.append((p.buffer_data(), tsz, (32, 32)))
+        u = r6.random(suns_per_set)
+        for k in range(suns_per_set):
+            theta = 95.0 * (k + u[k]) / suns_per_set
+            rp = AtmosphereRenderParameters(sun_direction_theta=float(theta), sun_direction_phi=180.0,
+                                            sky_view_lut_steps=30.0, aerial_perspective_lut_steps=50.867000579833984,
+                                            aerial_perspective_lut_max_distance=203.0, camera_position=cam.position,
+                                            view_projection=vp)
+            views.append((s, rp.buffer_data(rb), sv))
+    return sets, views
