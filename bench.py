@@ -13,8 +13,10 @@ multiple scattering 32x32x64/30, sky view 192x108/30, aerial perspective 32^3/50
 in ms per set over 16 sun angles; the roofline of the raymarch kernel; the voxel-cloud
 path tracer (1024x1024, 64 frames of one sample, brick majorants, with the single-majorant
 tracker timed beside it); the batched LUT build (256 atmosphere sets x 8 suns, mixed sizes,
-one hipGraph launch, autotuned); and for each the CPU oracle's FP32 OpenMP path on a
-bounded sample of the same work.
+one hipGraph launch, autotuned); the learned sky-view surrogate (trained here on 2^22
+samples, then 64 panoramas at 1920x1080 rendered by the plain integrator, by the surrogate
+with its per-tile residual check and fallback, and by sky-view LUT sampling); and for each
+the CPU oracle on a bounded sample of the same work.
 """
 import argparse
 import json
@@ -52,6 +54,8 @@ def parse():
     ap.add_argument("--pt-frames", type=int, default=64, help="path-tracer frames (1 sample each) timed")
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-epochs", type=int, default=200)
     return ap.parse_args()
 
 
@@ -191,6 +195,86 @@ def cpu_baseline_lut_batch(sets, views, target_seconds):
                                       f"({steps} integrator steps), oracle FP32 OpenMP, {t_total:.1f} s"}
 
 
+SURROGATE_FLOP_PER_SAMPLE = 2 * (7 * 64 + 3 * 64 * 64 + 64 * 3)  # 25,856 multiply-adds x 2
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_surrogate(ctx, epochs=200, width=1920, height=1080, n_views=64):
+    """Trains the sky-view surrogate on 2^22 plain-integrator samples (1024 atmospheres), then
+    renders n_views panoramas three ways: plain integrator per pixel, surrogate with the per-tile
+    residual check and fallback (tolerance 1e-2), and the sky-view LUT sampled bilinearly."""
+    from skyrendering_amd import surrogate as sg
+    t0 = time.perf_counter()
+    atm = sg.SkyAtmospheres(ctx, sg.draw_scales(1024, 41))
+    data = sg.SkyTrainingSet(ctx, atm, 1 << 22, seed=42)
+    ctx.synchronize()
+    data_s = time.perf_counter() - t0
+    model = sg.SkySurrogate(ctx, seed=43)
+    model.set_output_bias(data.targets_host.mean(axis=0))
+    t0 = time.perf_counter()
+    hist = model.train(data, epochs=epochs, batch=8192, lr=2e-3, lr_final=2e-5)
+    ctx.synchronize()
+    train_s = time.perf_counter() - t0
+    scales, views = sg.evaluation_views(n_views)
+    eatm = sg.SkyAtmospheres(ctx, scales, views=views)
+
+    def timed(fn, reps=3):
+        fn()
+        ctx.synchronize()
+        ctx.record(10)
+        for _ in range(reps):
+            fn()
+        ctx.record(11)
+        return ctx.elapsed_ms(10, 11) / reps
+
+    out = {"workload": f"{n_views} panoramas {width}x{height} (camera 0-10 km, sun zenith 0-90 deg, Rayleigh / "
+                       f"Mie / ozone scales U[0.25, 4]), 30-step plain integrator as label and judge",
+           "train_samples": data.n, "train_atmospheres": 1024, "epochs": epochs,
+           "train_data_s": round(data_s, 2), "train_s": round(train_s, 2), "final_loss": hist[-1]}
+    pano = sg.SkyPanoramas(ctx, eatm, views, width, height)
+    out["plain_ms"] = round(timed(pano.render_plain), 3)
+    out["surrogate_ms"] = round(timed(lambda: pano.render_surrogate(model)), 3)
+    out["fallback_tile_fraction"] = round(pano.fallback_tiles() / (pano.tiles * n_views), 4)
+    out["network_pass_ms"] = round(timed(lambda: pano.render_surrogate(model, tolerance=1e30)), 3)
+    out["lut_ms"] = round(timed(pano.render_lut), 3)
+    out["lut_build_ms"] = round(timed(eatm.batch.launch), 3)
+    out["speedup_vs_plain"] = round(out["plain_ms"] / out["surrogate_ms"], 3)
+    samples = width * height * n_views
+    achieved = samples * SURROGATE_FLOP_PER_SAMPLE / (out["network_pass_ms"] * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "unit": "TFLOP/s", "kernel": "sky_render_surrogate_kernel (+ probe check)",
+                       "flop_per_sample": SURROGATE_FLOP_PER_SAMPLE, "achieved": round(achieved, 2),
+                       "peak": BF16_PEAK_TFLOPS, "frac": round(achieved / BF16_PEAK_TFLOPS, 4)}
+    del pano
+    # Errors at a quarter resolution (the same views), against the plain pass and the FP64 oracle.
+    import oracle
+    small = sg.SkyPanoramas(ctx, eatm, views, width // 4, height // 4)
+    small.render_plain()
+    plain = small.image()[..., :3].astype(np.float64)
+    errs = {}
+    for name, fn in (("surrogate", lambda: small.render_surrogate(model)),
+                     ("network_only", lambda: small.render_surrogate(model, tolerance=1e30)),
+                     ("lut", small.render_lut)):
+        fn()
+        e = sg.relative_error(small.image()[..., :3].astype(np.float64), plain, 1e-6)
+        errs[name] = {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)), "max": float(e.max()),
+                      "frac_over_1e-2": float(np.mean(e > 1e-2))}
+    out["error_vs_plain"] = errs
+    blocks, T, MS = eatm.host_luts()
+    rng = np.random.default_rng(5)
+    v = rng.integers(0, n_views, 512)
+    xs, ys = rng.integers(0, small.width, 512), rng.integers(0, small.height, 512)
+    q = np.concatenate([small.pixel_queries(int(vi), [x], [y]) for vi, x, y in zip(v, xs, ys)])
+    t0 = time.perf_counter()
+    ref = oracle.sky_radiance(blocks, T, MS, 30.0, q)
+    cpu_s = time.perf_counter() - t0
+    e = sg.relative_error(plain[v, ys, xs], ref, 1e-6)
+    out["plain_vs_fp64_max"] = float(e.max())
+    out["cpu_baseline"] = {"value": round(len(q) / cpu_s / 1e6, 5), "unit": "Msamples/s (plain integrator)",
+                           "cores": oracle.max_threads(), "kind": "port",
+                           "sample": f"{len(q)} random pixels of the panoramas, oracle FP64 OpenMP, {cpu_s:.2f} s"}
+    return out
+
+
 def cpu_baseline_pathtrace(scene, target_seconds):
     """The oracle's FP32 path tracer (the reference's single-majorant tracking, OpenMP over
     pixels) on evenly spaced whole rows at 1 sample per pixel; samples per second."""
@@ -297,6 +381,8 @@ def main():
     batch = None
     if not args.no_batch:
         batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
+    if not args.no_surrogate:
+        out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         if pt_scene is not None:

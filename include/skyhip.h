@@ -698,6 +698,98 @@ int skyhip_lut_batch_view_outputs(SkyhipLutBatch* batch, int32_t view, const flo
  * directions x MS iterations + sky-view texels x sky-view iterations. */
 int skyhip_lut_batch_integrator_steps(const SkyhipLutBatch* batch, unsigned long long* out_steps);
 
+/* ------------------------------------------------------------------------- */
+/* Learned sky-view surrogate (no reference counterpart: an addition that     */
+/* replaces the per-pixel sky raymarch of AtmosphereRenderer.glsl:372-379     */
+/* with a small MLP, checked at run time against that raymarch)               */
+/* ------------------------------------------------------------------------- */
+
+/* A sky sample: view from `altitude` (km above the ground) along a direction with
+ * zenith cosine mu_view, sun zenith cosine mu_sun and azimuth difference delta_phi
+ * in [0, pi], in atmosphere `atmosphere` of a SkyhipSkyLuts set. */
+typedef struct SkyhipSkyQuery {
+    int32_t atmosphere;
+    float altitude;
+    float mu_view;
+    float mu_sun;
+    float delta_phi;
+    float pad[3];
+} SkyhipSkyQuery;
+
+/* A panorama view: camera altitude and sun zenith cosine in one atmosphere. */
+typedef struct SkyhipSkyView {
+    int32_t atmosphere;
+    float altitude;
+    float mu_sun;
+    float pad;
+} SkyhipSkyView;
+
+/* Device LUTs of several atmospheres with uniform sizes: atmosphere a's blocks at
+ * atmospheres[a], its transmittance LUT at transmittance + a * tw * th * 4 floats and its
+ * multiple-scattering LUT at multiscattering + a * mw * mh * 4 floats (the pool layout of a
+ * LUT batch whose sets share one size).  steps: integrator steps per ray. */
+typedef struct SkyhipSkyLuts {
+    const SkyhipAtmosphereBufferData* atmospheres;
+    const float* transmittance;
+    const float* multiscattering;
+    int32_t transmittance_width;
+    int32_t transmittance_height;
+    int32_t multiscattering_width;
+    int32_t multiscattering_height;
+    float steps;
+    int32_t atmosphere_count; /* query / view atmosphere indices are clamped to [0, count - 1] */
+} SkyhipSkyLuts;
+
+/* 13187: W1 (64x7), b1, W2..W4 (64x64), b2..b4, W5 (3x64), b5, each W out-major. */
+int skyhip_surrogate_parameter_count(void);
+/* The plain integrator per query (ComputeScatteredLuminance with the multiple-scattering
+ * term, Atmosphere.glsl:220-295, along the query's ray): out_rgb device float[3 * count]. */
+int skyhip_sky_radiance(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyQuery* queries, int32_t count,
+                        float* out_rgb);
+/* Surrogate inputs (8 floats per query, the 8th 0): 2 sqrt(altitude / thickness) - 1,
+ * mu_view, mu_sun, 2 delta_phi / pi - 1, and the atmosphere's natural-log Rayleigh, Mie and
+ * ozone scale factors (log_scales: device float[3 * atmospheres]) divided by ln 4. */
+int skyhip_surrogate_features(SkyhipContext* ctx, const SkyhipSkyQuery* queries, int32_t count,
+                              const float* log_scales, int32_t atmosphere_count, float thickness,
+                              float* out_features);
+/* Inference (BF16 MFMA, FP32 accumulate): out device float[3 * count] log radiance. */
+int skyhip_surrogate_forward(SkyhipContext* ctx, const float* params, const float* features, int32_t count,
+                             float* out_log_radiance);
+/* Workspace of skyhip_surrogate_gradient for `count` samples. */
+size_t skyhip_surrogate_workspace_bytes(int32_t count);
+/* Gradient of loss = loss_scale * sum_samples w * sum_rgb (f(x) - target)^2 over `count`
+ * samples (targets: device float[3 * count] log radiance; weights: device float[count], or
+ * NULL for w = 1) by the BF16 MFMA forward and backward kernels; out_grad: device
+ * float[13187]; out_loss: device float (may be NULL). */
+int skyhip_surrogate_gradient(SkyhipContext* ctx, const float* params, const float* features, const float* targets,
+                              const float* weights, int32_t count, float loss_scale, float* out_grad,
+                              float* out_loss, void* workspace, size_t workspace_bytes);
+/* One Adam step (step counts from 1) on device params / moments. */
+int skyhip_surrogate_adam(SkyhipContext* ctx, float* params, const float* grad, float* m, float* v, float lr,
+                          float beta1, float beta2, float eps, int32_t step);
+/* Equirectangular panoramas (width x height per view; pixel (x, y): zenith angle
+ * pi (y + 0.5) / height, azimuth from the sun 2 pi (x + 0.5) / width - pi) of RGB sky
+ * radiance (out_rgba device float[4 * width * height * view_count]) by the surrogate,
+ * with a residual check per 16 x 16 tile: the plain integrator at four probe pixels, and
+ * if any probe's max_c |surrogate - plain| / max(plain, radiance_floor) exceeds
+ * `tolerance` the whole tile is recomputed by the plain integrator.  tile_flags: device
+ * uint32[view_count * ceil(width/16) * ceil(height/16)] (1 = fell back); counters: device
+ * uint32[1], the number of tiles that fell back. */
+int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* params, const SkyhipSkyLuts* luts,
+                                const SkyhipSkyView* views, int32_t view_count, const float* log_scales,
+                                float thickness, int32_t width, int32_t height, float tolerance,
+                                float radiance_floor, float* out_rgba, uint32_t* tile_flags, uint32_t* counters);
+/* The same panoramas by the plain integrator at every pixel. */
+int skyhip_sky_render_plain(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyView* views,
+                            int32_t view_count, int32_t width, int32_t height, float* out_rgba);
+/* The same panoramas by the reference's method without the surrogate: each view's sky-view
+ * LUT (sky_view_luminance: device array of view_count device pointers to RGBA32F LUTs of
+ * sky_view_width x sky_view_height, built for the view's altitude and sun) sampled
+ * bilinearly at the pixel's direction (AtmosphereRenderer.glsl:113-145, 372-379). */
+int skyhip_sky_render_lut(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyView* views,
+                          int32_t view_count, const float* const* sky_view_luminance, int32_t sky_view_width,
+                          int32_t sky_view_height, int32_t width, int32_t height, float* out_rgba);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -363,3 +363,19 @@ def pt_lambert_sample(seed, n):
     s = C.c_uint32()
     lib().oracle_pt_lambert_sample_f64(seed, _ptr(n), _ptr(out), C.byref(s))
     return out, s.value
+
+
+register("oracle_sky_radiance_f64", None, [vp, vp, i32, i32, vp, i32, i32, f32, vp, C.c_long, vp])
+
+
+def sky_radiance(atmospheres, T, MS, steps, queries):
+    """FP64 sky radiance of queries [n, 5] (atmosphere, altitude, mu_view, mu_sun, delta_phi);
+    atmospheres: a ctypes array of AtmosphereBufferData; T [a, th, tw, 4] / MS [a, mh, mw, 4]
+    float32 LUT stacks.  Returns [n, 3]."""
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    MS = np.ascontiguousarray(MS, dtype=np.float32)
+    q = np.ascontiguousarray(queries, dtype=np.float64)
+    out = np.empty((len(q), 3), np.float64)
+    lib().oracle_sky_radiance_f64(atmospheres, _ptr(T), T.shape[2], T.shape[1], _ptr(MS), MS.shape[2], MS.shape[1],
+                                  float(steps), _ptr(q), len(q), _ptr(out))
+    return out

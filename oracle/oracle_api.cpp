@@ -399,6 +399,37 @@ double oracle_pt_sigma_t_f64(const SkyhipCloudCommonBufferData* common, const Sk
     return P.sigma({x, y, z});
 }
 
+// Sky radiance of (altitude, mu_view, mu_sun, delta_phi) queries: the sky-view texel
+// arithmetic (AtmosphereRenderer.glsl:153-186) for an explicit direction, with the
+// planet centre below the camera and the sun at azimuth 0.  queries: 5 doubles each
+// (atmosphere index, altitude, mu_view, mu_sun, delta_phi); LUTs per atmosphere at a
+// uniform stride; out: 3 doubles per query.
+void oracle_sky_radiance_f64(const SkyhipAtmosphereBufferData* atms, const float* T, int tw, int th, const float* MS,
+                             int mw, int mh, float steps, const double* queries, long count, double* out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < count; ++i) {
+        const double* q = queries + 5 * i;
+        const int a = int(q[0]);
+        Atmosphere<double> A(atms[a]);
+        const float* Ta = T + size_t(a) * tw * th * 4;
+        const float* Ma = MS + size_t(a) * mw * mh * 4;
+        double h = q[1], mu_v = q[2], mu_s = q[3], dphi = q[4];
+        V3<double> c = {0.0, 0.0, -A.bottom_radius};
+        V3<double> start = {0.0, 0.0, h};
+        double sv = std::sqrt(std::fmax(1.0 - mu_v * mu_v, 0.0)), ss = std::sqrt(std::fmax(1.0 - mu_s * mu_s, 0.0));
+        V3<double> dir = {sv * std::cos(dphi), sv * std::sin(dphi), mu_v};
+        V3<double> sun = {ss, 0.0, mu_s};
+        double r = A.bottom_radius + h;
+        double dist = A.RayIntersectsGround(r, mu_v) ? A.DistanceToBottomAtmosphereBoundary(r, mu_v)
+                                                     : A.DistanceToTopAtmosphereBoundary(r, mu_v);
+        V3<double> L = {0, 0, 0}, tr;
+        if (dist > 0.0)
+            L = A.ComputeScatteredLuminance(false, Ta, tw, th, Ma, mw, mh, 0.5f, c, start, dir, sun, dist, steps, tr,
+                                            nullptr);
+        out[3 * i] = L.x, out[3 * i + 1] = L.y, out[3 * i + 2] = L.z;
+    }
+}
+
 // The FP16 density grid of the path-tracing workload, and its brick majorants.
 void oracle_voxel_grid_f16(const SkyhipVoxelGridCreateInfo* ci, int dx, int dy, int dz, uint16_t* out) {
     const int dim[3] = {dx, dy, dz};

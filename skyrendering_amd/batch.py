@@ -59,11 +59,17 @@ class LutBatch:
         self.handle = h
 
     def close(self):
-        if self.handle:
+        """Frees the batch; a batch whose context is already closed is left to process exit
+        (its memory went with the context's device state)."""
+        if self.handle and getattr(self.ctx, "handle", None) is not None:
             _abi.lib().skyhip_lut_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
-    __del__ = close
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def launch(self):
         _abi.check(_abi.lib().skyhip_lut_batch_launch(self.handle))
@@ -92,6 +98,18 @@ class LutBatch:
         n = C.c_ulonglong()
         _abi.check(_abi.lib().skyhip_lut_batch_integrator_steps(self.handle, C.byref(n)))
         return n.value
+
+    def set_output_ptrs(self, s):
+        """Device pointers (transmittance, multiscattering) of set s."""
+        t, m = C.c_void_p(), C.c_void_p()
+        _abi.check(_abi.lib().skyhip_lut_batch_set_outputs(self.handle, s, C.byref(t), C.byref(m)))
+        return t.value, m.value
+
+    def view_output_ptrs(self, v):
+        """Device pointers (luminance, transmittance) of view v's sky-view LUTs."""
+        l, t = C.c_void_p(), C.c_void_p()
+        _abi.check(_abi.lib().skyhip_lut_batch_view_outputs(self.handle, v, C.byref(l), C.byref(t)))
+        return l.value, t.value
 
     def set_outputs(self, s):
         """(transmittance, multiscattering) of set s as host arrays [h, w, 4]."""

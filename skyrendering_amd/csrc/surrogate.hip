@@ -1,0 +1,751 @@
+// Learned sky-view surrogate for gfx950: a 7 -> 64 -> 64 -> 64 -> 64 -> 3 MLP with
+// SiLU activations predicting RGB log-radiance of the sky along a view direction,
+// with BF16 MFMA operands and FP32 accumulation, trained on the GPU by the kernels
+// below and checked at run time against the plain integrator it replaces.
+//
+// Parameters (FP32 master copy, 13187 floats): for each layer l with fan-in K and
+// fan-out N, W_l as N x K row-major (out-major), then b_l[N]; layers in order.
+// Inputs (8 floats per sample, the 8th zero): 2 sqrt(h / H) - 1, mu_view, mu_sun,
+// 2 dphi / pi - 1, log(rayleigh scale) / log 4, log(mie scale) / log 4,
+// log(ozone scale) / log 4.  Output: log of the RGB radiance (solar illuminance units).
+//
+// Kernels:
+//  - sky_radiance_kernel: the plain integrator per query (label generator and judge):
+//    ComputeScatteredLuminance with the multiple-scattering term along the view ray
+//    from altitude h, exactly the sky-view texel arithmetic for an explicit direction.
+//  - surrogate_forward_kernel: inference.  One wave holds every weight in registers as
+//    MFMA A operands and streams groups of 16 samples through v_mfma_f32_16x16x32_bf16;
+//    a layer's accumulators (rows = neurons, columns = samples) become the next layer's
+//    B operand in place, the k order inside each 32-wide block being permuted to match
+//    the accumulator layout (the weights are loaded with the same permutation).
+//  - surrogate_train_kernel: forward and backward for 32 samples per workgroup with
+//    LDS-staged BF16 operands (any transpose is an LDS stride), writing the workgroup's
+//    gradient to its own slice (no atomics; the reduction order is fixed).
+//  - surrogate_reduce_kernel / surrogate_adam_kernel: gradient sum and Adam update.
+//  - sky render kernels: per-pixel panorama evaluation (surrogate), a per-tile residual
+//    check against the plain integrator at probe pixels, and the plain fallback for
+//    tiles whose residual exceeds the bound.
+#include <algorithm>
+#include <vector>
+
+#include "lut_texels.h"
+
+namespace skyhip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kParams = 13187;
+constexpr int kOffW1 = 0, kOffB1 = 448, kOffW2 = 512, kOffB2 = 4608, kOffW3 = 4672, kOffB3 = 8768, kOffW4 = 8832,
+              kOffB4 = 12928, kOffW5 = 12992, kOffB5 = 13184;
+
+// SiLU z * sigmoid(z) with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, each
+// ~1 ulp): the same function in training and inference, well inside BF16 operand rounding.
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+__device__ __forceinline__ float silu_grad(float z) {
+    float s = sigmoid(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// Sky radiance along (mu_view, dphi) from altitude h for sun zenith cosine mu_sun.
+__device__ inline F3 sky_radiance(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                  const float4* __restrict__ MS, int mw, int mh, float h, float mu_v, float mu_s,
+                                  float dphi, float steps) {
+    const float r = A.rb + h;
+    const float sin_v = sqrtf(fmaxf(1.0f - mu_v * mu_v, 0.0f)), sin_s = sqrtf(fmaxf(1.0f - mu_s * mu_s, 0.0f));
+    RayStart s{r, h, mu_v, mu_s, mu_v * mu_s + sin_v * sin_s * cosf(dphi)};
+    const bool hit = ray_hits_ground(A, r, h, mu_v);
+    float dist = hit ? distance_to_bottom(A, r, h, mu_v) : distance_to_top(A, r, h, mu_v);
+    start_from_space(A, s, dist);
+    F3 L = f3(0.0f), Tr, Lf;
+    if (dist > 0.0f) scattered_luminance<false>(A, T, tw, th, MS, mw, mh, s, dist, steps, 0.5f, L, Tr, Lf);
+    return L;
+}
+
+struct SkyLuts {
+    const SkyhipAtmosphereBufferData* atm;  // per atmosphere
+    const float4* T;                        // per atmosphere, stride tw * th
+    const float4* MS;                       // per atmosphere, stride mw * mh
+    int tw, th, mw, mh;
+    float steps;
+    int count;
+};
+
+__device__ __forceinline__ int clamp_index(int a, int count) { return min(max(a, 0), count - 1); }
+
+__device__ __forceinline__ F3 sky_radiance_q(const SkyLuts& L, const SkyhipSkyQuery& q) {
+    const int a = clamp_index(q.atmosphere, L.count);
+    const AtmDev A = make_atm(L.atm[a]);
+    return sky_radiance(A, L.T + (long long)a * L.tw * L.th, L.tw, L.th, L.MS + (long long)a * L.mw * L.mh, L.mw,
+                        L.mh, q.altitude, q.mu_view, q.mu_sun, q.delta_phi, L.steps);
+}
+
+__global__ __launch_bounds__(256) void sky_radiance_kernel(SkyLuts L, const SkyhipSkyQuery* __restrict__ q, int n,
+                                                           float* __restrict__ out_rgb) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    F3 v = sky_radiance_q(L, q[i]);
+    out_rgb[3 * i + 0] = v.x;
+    out_rgb[3 * i + 1] = v.y;
+    out_rgb[3 * i + 2] = v.z;
+}
+
+// The 7 surrogate inputs of a query (the 8th is 0).
+__device__ inline void features(const SkyhipSkyQuery& q, const float* __restrict__ log_scales, float thickness,
+                                int count, float f[8]) {
+    const float inv_log4 = 0.72134752044448170368f;  // 1 / ln 4
+    const int a = clamp_index(q.atmosphere, count);
+    f[0] = 2.0f * sqrtf(fmaxf(q.altitude / thickness, 0.0f)) - 1.0f;
+    // The view zenith and azimuth enter in the sky-view LUT's coordinates
+    // (GetSkyViewTextureUvFromCosLatLon, AtmosphereRenderer.glsl:113-132): latitude
+    // measured from the horizon with square-root spacing on both sides, longitude
+    // sqrt(0.5 - 0.5 cos dphi); the radiance is smooth in them.
+    const float rb = 6360.0f;  // the preset's ground radius (the horizon angle's scale)
+    const float h = fmaxf(q.altitude, 0.0f), r = rb + h;
+    const float horizon_down = acosf(sqrtf(h * (r + rb)) / r);
+    const float horizon_up = kPiF - horizon_down;
+    const float lat = acosf(fminf(fmaxf(q.mu_view, -1.0f), 1.0f));
+    const float x_lat = lat < horizon_up ? 0.5f - 0.5f * sqrtf(fmaxf(1.0f - lat / horizon_up, 0.0f))
+                                         : 0.5f + 0.5f * sqrtf(fmaxf((lat - horizon_up) / horizon_down, 0.0f));
+    f[1] = 2.0f * x_lat - 1.0f;
+    f[2] = q.mu_sun;
+    // The relative azimuth as the sun-view cosine it implies (monotonic in delta_phi for
+    // fixed zenith angles), the argument of both phase functions.
+    const float sv = sqrtf(fmaxf(1.0f - q.mu_view * q.mu_view, 0.0f)),
+                ss = sqrtf(fmaxf(1.0f - q.mu_sun * q.mu_sun, 0.0f));
+    f[3] = q.mu_view * q.mu_sun + sv * ss * cosf(q.delta_phi);
+    f[4] = log_scales[3 * a + 0] * inv_log4;
+    f[5] = log_scales[3 * a + 1] * inv_log4;
+    f[6] = log_scales[3 * a + 2] * inv_log4;
+    f[7] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void features_kernel(const SkyhipSkyQuery* __restrict__ q, int n,
+                                                       const float* __restrict__ log_scales, float thickness,
+                                                       int count, float* __restrict__ out) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float f[8];
+    features(q[i], log_scales, thickness, count, f);
+    for (int k = 0; k < 8; ++k) out[8 * i + k] = f[k];
+}
+
+// ---------------------------------------------------------------------------
+// Inference: weights in registers, 16 samples per MFMA column block.
+// ---------------------------------------------------------------------------
+// Neuron feeding k slot (8 q + i) of 32-wide block b when the B operand is built
+// from accumulator tiles 2b and 2b + 1 (lane group q holds rows 4q .. 4q + 3 of each).
+__device__ __forceinline__ int permuted_neuron(int b, int q, int i) {
+    return 32 * b + (i < 4 ? 4 * q + i : 16 + 4 * q + (i - 4));
+}
+
+struct MlpRegs {
+    bf16x8 w1[4];
+    bf16x8 wh[3][4][2];
+    bf16x8 w5[2];
+};
+
+__device__ inline void load_mlp(const float* __restrict__ p, MlpRegs& R, int lane) {
+    const int q = lane >> 4, r = lane & 15;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 8; ++i) {
+            int k = 8 * q + i;
+            R.w1[j][i] = (__bf16)(k < 7 ? p[kOffW1 + (16 * j + r) * 7 + k] : 0.0f);
+        }
+    const int offs[3] = {kOffW2, kOffW3, kOffW4};
+    for (int L = 0; L < 3; ++L)
+        for (int j = 0; j < 4; ++j)
+            for (int b = 0; b < 2; ++b)
+                for (int i = 0; i < 8; ++i)
+                    R.wh[L][j][b][i] = (__bf16)p[offs[L] + (16 * j + r) * 64 + permuted_neuron(b, q, i)];
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 8; ++i) R.w5[b][i] = (__bf16)(r < 3 ? p[kOffW5 + r * 64 + permuted_neuron(b, q, i)] : 0.0f);
+}
+
+// B operands of the next layer from the four accumulator tiles (bias added, SiLU).
+__device__ __forceinline__ void activate(const f32x4 c[4], bf16x8 h[2]) {
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 4; ++i) {
+            h[b][i] = (__bf16)silu(c[2 * b][i]);
+            h[b][4 + i] = (__bf16)silu(c[2 * b + 1][i]);
+        }
+}
+
+// Forward pass of 16 samples (the lane's column); x: this lane's input operand.
+// Returns the output accumulator (lanes 0..15 hold rows 0..2 in items 0..2).
+__device__ __forceinline__ f32x4 mlp_forward16(const MlpRegs& R, const float* s_bias, bf16x8 x, int q) {
+    f32x4 c[4];
+    bf16x8 h[2];
+    for (int j = 0; j < 4; ++j) {
+        const float* b = s_bias + 16 * j + 4 * q;
+        c[j] = f32x4{b[0], b[1], b[2], b[3]};
+        c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[j], x, c[j], 0, 0, 0);
+    }
+    activate(c, h);
+    for (int L = 0; L < 3; ++L) {
+        for (int j = 0; j < 4; ++j) {
+            const float* b = s_bias + 64 * (L + 1) + 16 * j + 4 * q;
+            c[j] = f32x4{b[0], b[1], b[2], b[3]};
+            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.wh[L][j][0], h[0], c[j], 0, 0, 0);
+            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.wh[L][j][1], h[1], c[j], 0, 0, 0);
+        }
+        activate(c, h);
+    }
+    const float* b = s_bias + 256 + 4 * q;
+    f32x4 y = {b[0], b[1], b[2], b[3]};
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w5[0], h[0], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w5[1], h[1], y, 0, 0, 0);
+    return y;
+}
+
+// Biases b1..b4 at 64 (l - 1), b5 at 256 (3 used, zero to 271).
+__device__ inline void load_biases(const float* __restrict__ p, float* s_bias) {
+    for (int i = threadIdx.x; i < 272; i += blockDim.x) {
+        float v = 0.0f;
+        if (i < 64) v = p[kOffB1 + i];
+        else if (i < 128) v = p[kOffB2 + i - 64];
+        else if (i < 192) v = p[kOffB3 + i - 128];
+        else if (i < 256) v = p[kOffB4 + i - 192];
+        else if (i < 259) v = p[kOffB5 + i - 256];
+        s_bias[i] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bf16x8 input_operand(const float* f, bool valid, int q) {
+    bf16x8 x;
+    for (int i = 0; i < 8; ++i) x[i] = (__bf16)((q == 0 && valid) ? f[i] : 0.0f);
+    return x;
+}
+
+__global__ __launch_bounds__(256, 2) void surrogate_forward_kernel(const float* __restrict__ params,
+                                                                const float* __restrict__ feats, int n,
+                                                                float* __restrict__ out) {
+    __shared__ float s_bias[272];
+    load_biases(params, s_bias);
+    const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+    MlpRegs R;
+    load_mlp(params, R, lane);
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g * 16 < n; g += waves) {
+        const int s = g * 16 + r;
+        float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (q == 0 && s < n)
+            for (int k = 0; k < 8; ++k) f[k] = feats[8 * s + k];
+        f32x4 y = mlp_forward16(R, s_bias, input_operand(f, s < n, q), q);
+        if (q == 0 && s < n) {
+            out[3 * s + 0] = y[0];
+            out[3 * s + 1] = y[1];
+            out[3 * s + 2] = y[2];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training: 32 samples per workgroup, LDS-staged operands.
+// ---------------------------------------------------------------------------
+constexpr int kTS = 32;  // samples per workgroup
+
+// C[M x N] = A[M x K] B[K x N] with BF16 LDS operands at element strides
+// A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn]; 16 x 16 tiles spread over
+// the workgroup's waves, K a multiple of 32.  ep(m, n, value) consumes each output once.
+template <class Ep>
+__device__ inline void lds_gemm(const __bf16* A, int sam, int sak, const __bf16* B, int sbk, int sbn, int M, int N,
+                                int K, Ep ep) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int q = lane >> 4, r = lane & 15;
+    const int tn_count = N / 16;
+    for (int t = wave; t < (M / 16) * tn_count; t += nw) {
+        const int m0 = (t / tn_count) * 16, n0 = (t % tn_count) * 16;
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 a, b;
+            for (int i = 0; i < 8; ++i) {
+                const int k = k0 + 8 * q + i;
+                a[i] = A[(m0 + r) * sam + k * sak];
+                b[i] = B[k * sbk + (n0 + r) * sbn];
+            }
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+        }
+        for (int i = 0; i < 4; ++i) ep(m0 + 4 * q + i, n0 + r, c[i]);
+    }
+}
+
+struct TrainLds {
+    __bf16 w1[64 * 32];   // [neuron][input] (inputs 7..31 zero)
+    __bf16 wh[3][64 * 64];
+    __bf16 w5[32 * 64];   // [output][neuron] (outputs 3..31 zero)
+    __bf16 x[32 * kTS];   // [input][sample]
+    float z[4][64 * kTS]; // pre-activations [neuron][sample]
+    __bf16 a[4][64 * kTS];
+    float y[16 * kTS];
+    __bf16 d5[32 * kTS];  // output deltas [output][sample] (rows 3..31 zero)
+    __bf16 d[2][64 * kTS];
+    float bias[272];
+    float loss;
+};
+
+__global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __restrict__ params,
+                                                              const float* __restrict__ feats,
+                                                              const float* __restrict__ targets,
+                                                              const float* __restrict__ weights, int n,
+                                                              float loss_scale, float* __restrict__ partial,
+                                                              float* __restrict__ partial_loss) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    TrainLds& S = *reinterpret_cast<TrainLds*>(smem);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int s0 = blockIdx.x * kTS;
+    float* G = partial + (long long)blockIdx.x * kParams;
+    // Weights to BF16 (zero padding), inputs, biases.
+    for (int i = tid; i < 64 * 32; i += nt) {
+        int m = i / 32, k = i % 32;
+        S.w1[i] = (__bf16)(k < 7 ? params[kOffW1 + m * 7 + k] : 0.0f);
+    }
+    const int offs[3] = {kOffW2, kOffW3, kOffW4};
+    for (int L = 0; L < 3; ++L)
+        for (int i = tid; i < 64 * 64; i += nt) S.wh[L][i] = (__bf16)params[offs[L] + i];
+    for (int i = tid; i < 32 * 64; i += nt) S.w5[i] = (__bf16)(i < 3 * 64 ? params[kOffW5 + i] : 0.0f);
+    for (int i = tid; i < 32 * kTS; i += nt) {
+        int k = i / kTS, s = i % kTS;
+        S.x[i] = (__bf16)((k < 8 && s0 + s < n) ? feats[8 * (s0 + s) + k] : 0.0f);
+        S.d5[i] = (__bf16)0.0f;
+    }
+    if (tid == 0) S.loss = 0.0f;
+    load_biases(params, S.bias);  // synchronises
+    // Forward.
+    lds_gemm(S.w1, 32, 1, S.x, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
+        float z = c + S.bias[m];
+        S.z[0][m * kTS + s] = z;
+        S.a[0][m * kTS + s] = (__bf16)silu(z);
+    });
+    __syncthreads();
+    for (int L = 0; L < 3; ++L) {
+        lds_gemm(S.wh[L], 64, 1, S.a[L], kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
+            float z = c + S.bias[64 * (L + 1) + m];
+            S.z[L + 1][m * kTS + s] = z;
+            S.a[L + 1][m * kTS + s] = (__bf16)silu(z);
+        });
+        __syncthreads();
+    }
+    lds_gemm(S.w5, 64, 1, S.a[3], kTS, 1, 16, kTS, 64,
+             [&](int m, int s, float c) { S.y[m * kTS + s] = c + S.bias[256 + m]; });
+    __syncthreads();
+    // Loss loss_scale * sum w (y - t)^2 and output deltas.
+    for (int i = tid; i < 3 * kTS; i += nt) {
+        int m = i / kTS, s = i % kTS;
+        float d = 0.0f, w = 0.0f;
+        if (s0 + s < n) {
+            d = S.y[m * kTS + s] - targets[3 * (s0 + s) + m];
+            w = weights ? weights[s0 + s] : 1.0f;
+        }
+        atomicAdd(&S.loss, w * d * d * loss_scale);
+        S.d5[m * kTS + s] = (__bf16)(2.0f * w * d * loss_scale);
+    }
+    __syncthreads();
+    // Output layer gradient: dW5 = d5 a4^T, db5 = sum d5.
+    lds_gemm(S.d5, kTS, 1, S.a[3], 1, kTS, 16, 64, kTS, [&](int m, int k, float c) {
+        if (m < 3) G[kOffW5 + m * 64 + k] = c;
+    });
+    for (int m = tid; m < 3; m += nt) {
+        float sum = 0.0f;
+        for (int s = 0; s < kTS; ++s) sum += float(S.d5[m * kTS + s]);
+        G[kOffB5 + m] = sum;
+    }
+    // delta4 = (W5^T d5) * silu'(z4)
+    lds_gemm(S.w5, 1, 64, S.d5, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
+        S.d[0][m * kTS + s] = (__bf16)(c * silu_grad(S.z[3][m * kTS + s]));
+    });
+    __syncthreads();
+    // Hidden layers 4, 3, 2 (weights wh[2], wh[1], wh[0]).
+    const int woffs[3] = {kOffW2, kOffW3, kOffW4}, boffs[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
+    int cur = 0;
+    for (int L = 2; L >= 0; --L) {
+        const __bf16* dl = S.d[cur];
+        // dW = delta a_prev^T ; a_prev = a[L] (input of wh[L])
+        lds_gemm(dl, kTS, 1, S.a[L], 1, kTS, 64, 64, kTS,
+                 [&](int m, int k, float c) { G[woffs[L] + m * 64 + k] = c; });
+        for (int m = tid; m < 64; m += nt) {
+            float sum = 0.0f;
+            for (int s = 0; s < kTS; ++s) sum += float(dl[m * kTS + s]);
+            G[boffs[L + 1] + m] = sum;
+        }
+        // delta_prev = (W^T delta) * silu'(z[L])
+        __bf16* dn = S.d[cur ^ 1];
+        lds_gemm(S.wh[L], 1, 64, dl, kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
+            dn[m * kTS + s] = (__bf16)(c * silu_grad(S.z[L][m * kTS + s]));
+        });
+        __syncthreads();
+        cur ^= 1;
+    }
+    // First layer: dW1 = delta1 x^T, db1.
+    const __bf16* d1 = S.d[cur];
+    lds_gemm(d1, kTS, 1, S.x, 1, kTS, 64, 32, kTS, [&](int m, int k, float c) {
+        if (k < 7) G[kOffW1 + m * 7 + k] = c;
+    });
+    for (int m = tid; m < 64; m += nt) {
+        float sum = 0.0f;
+        for (int s = 0; s < kTS; ++s) sum += float(d1[m * kTS + s]);
+        G[kOffB1 + m] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) partial_loss[blockIdx.x] = S.loss;
+}
+
+// grad[p] = sum over workgroups of partial[wg][p] (fixed order); loss likewise.
+__global__ __launch_bounds__(256) void surrogate_reduce_kernel(const float* __restrict__ partial,
+                                                               const float* __restrict__ partial_loss, int groups,
+                                                               float* __restrict__ grad, float* __restrict__ loss) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < kParams) {
+        float s = 0.0f;
+        for (int g = 0; g < groups; ++g) s += partial[(long long)g * kParams + p];
+        grad[p] = s;
+    }
+    if (p == 0 && loss) {
+        float s = 0.0f;
+        for (int g = 0; g < groups; ++g) s += partial_loss[g];
+        *loss = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void surrogate_adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                                                             float* __restrict__ m, float* __restrict__ v, float lr,
+                                                             float beta1, float beta2, float eps, float bc1,
+                                                             float bc2) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kParams) return;
+    float gi = g[p];
+    float mi = beta1 * m[p] + (1.0f - beta1) * gi;
+    float vi = beta2 * v[p] + (1.0f - beta2) * gi * gi;
+    m[p] = mi;
+    v[p] = vi;
+    w[p] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Panorama rendering: surrogate, residual check, plain fallback.
+// ---------------------------------------------------------------------------
+struct SkyRenderArgs {
+    SkyLuts luts;
+    const SkyhipSkyView* views;  // per view
+    const float* log_scales;     // per atmosphere
+    float thickness;
+    int width, height, tiles_x, tiles_y;
+    float4* out;                 // [view][pixel] RGB radiance (w unused)
+    uint32_t* flags;             // [view][tile]: 1 if the tile fell back
+    uint32_t* counters;          // [0]: tiles that fell back
+    float tolerance, floor;
+};
+
+// Query of pixel (x, y) of an equirectangular panorama around the camera's zenith:
+// zenith angle pi (y + 0.5) / H, azimuth from the sun 2 pi (x + 0.5) / W - pi.
+__device__ inline SkyhipSkyQuery panorama_query(const SkyhipSkyView& v, int x, int y, int w, int h) {
+    SkyhipSkyQuery q;
+    q.atmosphere = v.atmosphere;
+    q.altitude = v.altitude;
+    q.mu_sun = v.mu_sun;
+    q.mu_view = cosf(kPiF * (float(y) + 0.5f) / float(h));
+    q.delta_phi = fabsf(2.0f * kPiF * (float(x) + 0.5f) / float(w) - kPiF);
+    q.pad[0] = q.pad[1] = q.pad[2] = 0.0f;
+    return q;
+}
+
+// Surrogate pass: each wave renders 16-pixel runs of one row (lanes 0..15 own a pixel each).
+__global__ __launch_bounds__(256, 2) void sky_render_surrogate_kernel(const float* __restrict__ params,
+                                                                   SkyRenderArgs a, int n_views) {
+    __shared__ float s_bias[272];
+    load_biases(params, s_bias);
+    const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+    MlpRegs R;
+    load_mlp(params, R, lane);
+    const long long per_view = (long long)a.width * a.height;
+    const long long groups = (per_view * n_views + 15) / 16;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (long long g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < groups; g += waves) {
+        const long long idx = g * 16 + r;
+        const bool valid = idx < per_view * n_views;
+        const int view = valid ? int(idx / per_view) : 0;
+        const int pix = valid ? int(idx % per_view) : 0;
+        float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (q == 0 && valid) {
+            SkyhipSkyQuery qq = panorama_query(a.views[view], pix % a.width, pix / a.width, a.width, a.height);
+            features(qq, a.log_scales, a.thickness, a.luts.count, f);
+        }
+        f32x4 y = mlp_forward16(R, s_bias, input_operand(f, valid, q), q);
+        if (q == 0 && valid) a.out[idx] = make_float4(__expf(y[0]), __expf(y[1]), __expf(y[2]), 0.0f);
+    }
+}
+
+// Residual check: probes at the 4 quarter points of each 16 x 16 tile; a tile whose
+// largest |surrogate - plain| / max(plain, floor) exceeds the tolerance is flagged.
+__global__ __launch_bounds__(64) void sky_render_check_kernel(SkyRenderArgs a) {
+    const int tile = blockIdx.x, view = blockIdx.y;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    __shared__ float s_err[4];
+    if (threadIdx.x < 4) {
+        int x = min(tx * 16 + 4 + 8 * (threadIdx.x & 1), a.width - 1);
+        int y = min(ty * 16 + 4 + 8 * (threadIdx.x >> 1), a.height - 1);
+        SkyhipSkyQuery q = panorama_query(a.views[view], x, y, a.width, a.height);
+        F3 ref = sky_radiance_q(a.luts, q);
+        long long idx = (long long)view * a.width * a.height + (long long)y * a.width + x;
+        float4 s = a.out[idx];
+        float e = fmaxf(fmaxf(fabsf(s.x - ref.x) / fmaxf(ref.x, a.floor), fabsf(s.y - ref.y) / fmaxf(ref.y, a.floor)),
+                        fabsf(s.z - ref.z) / fmaxf(ref.z, a.floor));
+        s_err[threadIdx.x] = (e == e) ? e : 3.0e38f;  // NaN counts as a failure
+        a.out[idx] = make_float4(ref.x, ref.y, ref.z, 0.0f);  // probes keep the plain value
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float e = fmaxf(fmaxf(s_err[0], s_err[1]), fmaxf(s_err[2], s_err[3]));
+        unsigned f = e > a.tolerance ? 1u : 0u;
+        a.flags[(long long)view * a.tiles_x * a.tiles_y + tile] = f;
+        if (f) atomicAdd(&a.counters[0], 1u);
+    }
+}
+
+// Plain pass: every pixel of flagged tiles (or of all tiles when all_tiles is set).
+__global__ __launch_bounds__(256) void sky_render_plain_kernel(SkyRenderArgs a, int all_tiles) {
+    const int tile = blockIdx.x, view = blockIdx.y;
+    if (!all_tiles && a.flags[(long long)view * a.tiles_x * a.tiles_y + tile] == 0u) return;  // workgroup-uniform
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
+    if (x >= a.width || y >= a.height) return;
+    SkyhipSkyQuery q = panorama_query(a.views[view], x, y, a.width, a.height);
+    F3 v = sky_radiance_q(a.luts, q);
+    a.out[(long long)view * a.width * a.height + (long long)y * a.width + x] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// The reference's sky path without the surrogate: the view's sky-view LUT sampled
+// bilinearly at the pixel's direction (GetSkyViewTextureUvFromCosLatLon,
+// AtmosphereRenderer.glsl:113-145; the LUT's longitude is measured from the sun).
+__global__ __launch_bounds__(256) void sky_render_lut_kernel(SkyRenderArgs a, const float* const* __restrict__ sv,
+                                                             int sw, int sh) {
+    const int tile = blockIdx.x, view = blockIdx.y;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
+    if (x >= a.width || y >= a.height) return;
+    const SkyhipSkyQuery q = panorama_query(a.views[view], x, y, a.width, a.height);
+    const AtmDev A = make_atm(a.luts.atm[clamp_index(q.atmosphere, a.luts.count)]);
+    const float r = A.rb + q.altitude;
+    const float horizon_down = acosf(sqrtf(q.altitude * (r + A.rb)) / r);
+    const float horizon_up = kPiF - horizon_down;
+    const float lat = acosf(fminf(fmaxf(q.mu_view, -1.0f), 1.0f));
+    const float x_lat = lat < horizon_up ? 0.5f - 0.5f * sqrtf(fmaxf(1.0f - lat / horizon_up, 0.0f))
+                                         : 0.5f + 0.5f * sqrtf(fmaxf((lat - horizon_up) / horizon_down, 0.0f));
+    const float x_lon = sqrtf(fmaxf(0.5f - 0.5f * cosf(q.delta_phi), 0.0f));
+    F3 L = lut_bilinear(reinterpret_cast<const float4*>(sv[view]), sw, sh, x_lon * float(sw - 1),
+                        x_lat * float(sh - 1));
+    a.out[(long long)view * a.width * a.height + (long long)y * a.width + x] = make_float4(L.x, L.y, L.z, 0.0f);
+}
+
+__global__ void clear_u32_kernel(uint32_t* p, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+int check_luts(const SkyhipSkyLuts* l) {
+    if (!l || !l->atmospheres || !l->transmittance || !l->multiscattering)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "sky LUT set has a NULL pointer");
+    if (l->transmittance_width < 2 || l->transmittance_height < 2 || l->multiscattering_width < 2 ||
+        l->multiscattering_height < 2 || l->transmittance_width > 4096 || l->transmittance_height > 4096 ||
+        l->multiscattering_width > 4096 || l->multiscattering_height > 4096)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "LUT sizes outside [2, 4096]");
+    if (!(l->steps > 0.0f && l->steps <= 4096.0f)) return fail(SKYHIP_ERR_INVALID_SIZE, "steps outside (0, 4096]");
+    if (l->atmosphere_count < 1) return fail(SKYHIP_ERR_INVALID_SIZE, "atmosphere_count must be >= 1");
+    return SKYHIP_OK;
+}
+
+SkyLuts make_luts(const SkyhipSkyLuts* l) {
+    SkyLuts s;
+    s.atm = l->atmospheres;
+    s.T = reinterpret_cast<const float4*>(l->transmittance);
+    s.MS = reinterpret_cast<const float4*>(l->multiscattering);
+    s.tw = l->transmittance_width;
+    s.th = l->transmittance_height;
+    s.mw = l->multiscattering_width;
+    s.mh = l->multiscattering_height;
+    s.steps = l->steps;
+    s.count = l->atmosphere_count;
+    return s;
+}
+
+}  // namespace skyhip
+
+using namespace skyhip;
+
+extern "C" int skyhip_surrogate_parameter_count(void) { return kParams; }
+
+extern "C" int skyhip_sky_radiance(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyQuery* queries,
+                                   int32_t count, float* out_rgb) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(check_luts(luts));
+    if (!queries || !out_rgb) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count < 0) return fail(SKYHIP_ERR_INVALID_SIZE, "negative count");
+    if (count == 0) return SKYHIP_OK;
+    hipLaunchKernelGGL(sky_radiance_kernel, dim3(div_up(count, 256)), dim3(256), 0, ctx->stream, make_luts(luts),
+                       queries, count, out_rgb);
+    SKYHIP_LAUNCH_CHECK("sky_radiance_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_surrogate_features(SkyhipContext* ctx, const SkyhipSkyQuery* queries, int32_t count,
+                                         const float* log_scales, int32_t atmosphere_count, float thickness,
+                                         float* out_features) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!queries || !log_scales || !out_features) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count < 0) return fail(SKYHIP_ERR_INVALID_SIZE, "negative count");
+    if (!(thickness > 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "thickness must be > 0");
+    if (atmosphere_count < 1) return fail(SKYHIP_ERR_INVALID_SIZE, "atmosphere_count must be >= 1");
+    if (count == 0) return SKYHIP_OK;
+    hipLaunchKernelGGL(features_kernel, dim3(div_up(count, 256)), dim3(256), 0, ctx->stream, queries, count,
+                       log_scales, thickness, atmosphere_count, out_features);
+    SKYHIP_LAUNCH_CHECK("features_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_surrogate_forward(SkyhipContext* ctx, const float* params, const float* features,
+                                        int32_t count, float* out_log_radiance) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!params || !features || !out_log_radiance) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count < 0) return fail(SKYHIP_ERR_INVALID_SIZE, "negative count");
+    if (count == 0) return SKYHIP_OK;
+    const int groups = int(div_up(count, 16));
+    const int blocks = std::max(1, std::min(int(div_up(groups, 4)), ctx->cu_count * 8));
+    hipLaunchKernelGGL(surrogate_forward_kernel, dim3(blocks), dim3(256), 0, ctx->stream, params, features, count,
+                       out_log_radiance);
+    SKYHIP_LAUNCH_CHECK("surrogate_forward_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" size_t skyhip_surrogate_workspace_bytes(int32_t count) {
+    if (count <= 0) return 0;
+    size_t groups = size_t((count + kTS - 1) / kTS);
+    return groups * kParams * sizeof(float) + groups * sizeof(float);
+}
+
+extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, const float* params, const float* features,
+                                         const float* targets, const float* weights, int32_t count,
+                                         float loss_scale, float* out_grad, float* out_loss, void* workspace,
+                                         size_t workspace_bytes) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!params || !features || !targets || !out_grad || !workspace)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count < 1 || count > (1 << 26)) return fail(SKYHIP_ERR_INVALID_SIZE, "count outside [1, 2^26]");
+    if (workspace_bytes < skyhip_surrogate_workspace_bytes(count))
+        return fail(SKYHIP_ERR_INVALID_SIZE, "workspace smaller than skyhip_surrogate_workspace_bytes(count)");
+    if (!std::isfinite(loss_scale)) return fail(SKYHIP_ERR_NON_FINITE, "non-finite loss scale");
+    static bool lds_ready = false;  // the training kernel's LDS footprint exceeds the 64 KiB default
+    if (!lds_ready) {
+        SKYHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&surrogate_train_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainLds))));
+        lds_ready = true;
+    }
+    const int groups = div_up(count, kTS);
+    float* partial = reinterpret_cast<float*>(workspace);
+    float* partial_loss = partial + (size_t)groups * kParams;
+    hipLaunchKernelGGL(surrogate_train_kernel, dim3(groups), dim3(256), sizeof(TrainLds), ctx->stream, params,
+                       features, targets, weights, count, loss_scale, partial, partial_loss);
+    SKYHIP_LAUNCH_CHECK("surrogate_train_kernel");
+    hipLaunchKernelGGL(surrogate_reduce_kernel, dim3(div_up(kParams, 256)), dim3(256), 0, ctx->stream, partial,
+                       partial_loss, groups, out_grad, out_loss);
+    SKYHIP_LAUNCH_CHECK("surrogate_reduce_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_surrogate_adam(SkyhipContext* ctx, float* params, const float* grad, float* m, float* v,
+                                     float lr, float beta1, float beta2, float eps, int32_t step) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!params || !grad || !m || !v) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (step < 1) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "Adam steps count from 1");
+    if (!(lr >= 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "bad Adam hyper-parameters");
+    const float bc1 = 1.0f - powf(beta1, float(step)), bc2 = 1.0f - powf(beta2, float(step));
+    hipLaunchKernelGGL(surrogate_adam_kernel, dim3(div_up(kParams, 256)), dim3(256), 0, ctx->stream, params, grad,
+                       m, v, lr, beta1, beta2, eps, bc1, bc2);
+    SKYHIP_LAUNCH_CHECK("surrogate_adam_kernel");
+    return SKYHIP_OK;
+}
+
+static int render_args(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyView* views, int32_t n_views,
+                       const float* log_scales, float thickness, int32_t width, int32_t height, float* out_rgba,
+                       uint32_t* tile_flags, uint32_t* counters, float tolerance, float floor_, SkyRenderArgs& a) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(check_luts(luts));
+    if (!views || !out_rgba || !tile_flags || !counters) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n_views < 1 || n_views > 65535) return fail(SKYHIP_ERR_INVALID_SIZE, "view count outside [1, 65535]");
+    if (width < 1 || height < 1 || width > 16384 || height > 16384)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "image size outside [1, 16384]");
+    if (!(thickness > 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "thickness must be > 0");
+    a.luts = make_luts(luts);
+    a.views = views;
+    a.log_scales = log_scales;
+    a.thickness = thickness;
+    a.width = width;
+    a.height = height;
+    a.tiles_x = div_up(width, 16);
+    a.tiles_y = div_up(height, 16);
+    a.out = reinterpret_cast<float4*>(out_rgba);
+    a.flags = tile_flags;
+    a.counters = counters;
+    a.tolerance = tolerance;
+    a.floor = floor_;
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* params, const SkyhipSkyLuts* luts,
+                                           const SkyhipSkyView* views, int32_t view_count, const float* log_scales,
+                                           float thickness, int32_t width, int32_t height, float tolerance,
+                                           float radiance_floor, float* out_rgba, unsigned* tile_flags,
+                                           unsigned* counters) {
+    SkyRenderArgs a;
+    SKYHIP_CHECK(render_args(ctx, luts, views, view_count, log_scales, thickness, width, height, out_rgba, tile_flags,
+                             counters, tolerance, radiance_floor, a));
+    if (!params || !log_scales) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!(tolerance > 0.0f) || !(radiance_floor > 0.0f))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "tolerance and radiance floor must be > 0");
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(256), 0, st, counters, 1);
+    const long long groups = ((long long)width * height * view_count + 15) / 16;
+    const int blocks = int(std::max(1LL, std::min((groups + 3) / 4, (long long)ctx->cu_count * 8)));
+    hipLaunchKernelGGL(sky_render_surrogate_kernel, dim3(blocks), dim3(256), 0, st, params, a, view_count);
+    dim3 tiles(a.tiles_x * a.tiles_y, view_count);
+    hipLaunchKernelGGL(sky_render_check_kernel, tiles, dim3(64), 0, st, a);
+    hipLaunchKernelGGL(sky_render_plain_kernel, tiles, dim3(256), 0, st, a, 0);
+    SKYHIP_LAUNCH_CHECK("sky render (surrogate)");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_sky_render_plain(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyView* views,
+                                       int32_t view_count, int32_t width, int32_t height, float* out_rgba) {
+    SkyRenderArgs a;
+    uint32_t dummy_flags = 0, dummy_counter = 0;  // never dereferenced with all_tiles set
+    SKYHIP_CHECK(render_args(ctx, luts, views, view_count, nullptr, 1.0f, width, height, out_rgba, &dummy_flags,
+                             &dummy_counter, 1.0f, 1.0f, a));
+    a.flags = nullptr;
+    a.counters = nullptr;
+    dim3 tiles(a.tiles_x * a.tiles_y, view_count);
+    hipLaunchKernelGGL(sky_render_plain_kernel, tiles, dim3(256), 0, ctx->stream, a, 1);
+    SKYHIP_LAUNCH_CHECK("sky_render_plain_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_sky_render_lut(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyView* views,
+                                     int32_t view_count, const float* const* sky_view_luminance,
+                                     int32_t sky_view_width, int32_t sky_view_height, int32_t width, int32_t height,
+                                     float* out_rgba) {
+    SkyRenderArgs a;
+    uint32_t dummy_flags = 0, dummy_counter = 0;
+    SKYHIP_CHECK(render_args(ctx, luts, views, view_count, nullptr, 1.0f, width, height, out_rgba, &dummy_flags,
+                             &dummy_counter, 1.0f, 1.0f, a));
+    if (!sky_view_luminance) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL sky-view LUT array");
+    if (sky_view_width < 2 || sky_view_height < 2 || sky_view_width > 8192 || sky_view_height > 8192)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "sky-view size outside [2, 8192]");
+    a.flags = nullptr;
+    a.counters = nullptr;
+    dim3 tiles(a.tiles_x * a.tiles_y, view_count);
+    hipLaunchKernelGGL(sky_render_lut_kernel, tiles, dim3(256), 0, ctx->stream, a, sky_view_luminance, sky_view_width,
+                       sky_view_height);
+    SKYHIP_LAUNCH_CHECK("sky_render_lut_kernel");
+    return SKYHIP_OK;
+}
