@@ -42,6 +42,25 @@ FLOP_PER_CLOUD_STEP = 72        # extinction, shadow-loop bookkeeping, phase (2 
 FLOP_PER_STEP = 8               # ray position and loop
 FLOP_PER_PIXEL = 260            # unprojection, shell intersection, composite with T-LUT and AP-LUT lookups
 
+# HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
+# bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
+# plain 256 MiB device copy on MI355X (FETCH_SIZE 0.5, WRITE_SIZE 1.0).  The counters cannot be
+# read from inside the timed run, so the figures are the profiled ones; the summaries and the
+# commands that made them are profiles/cloud_raymarch_1080p.md and
+# profiles/path_trace_voxel_1024.md.
+FETCH_COUNTS_PER_KIB = 0.5
+WRITE_COUNTS_PER_KIB = 1.0
+PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
+    "cloud_raymarch_kernel<1,false>": (7355.3, 24303.0),
+    "path_trace_kernel<4,true,false>": (4.5604e6, 1.9302e4),
+}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this bench's workload (calibrated counter figures)."""
+    fetch, write = PROFILED_KIB[kernel]
+    return int(round((fetch / FETCH_COUNTS_PER_KIB + write / WRITE_COUNTS_PER_KIB) * 1024))
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -115,6 +134,15 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     ctx.record(7)
     ref_ms = ctx.elapsed_ms(6, 7) / reference_frames
     n = scene.width * scene.height
+    # Algorithmic bytes per frame: 8 FP16 texels per trilinear density lookup, one FP32 majorant
+    # per brick segment, and the RGBA32F accumulator read and written once per pixel.
+    alg_bytes = counts["lookups"] * 16 + counts["segments"] * 4 + n * 32
+    traffic = profiled_traffic("path_trace_kernel<4,true,false>")
+    roofline = {"bound": "hbm", "unit": "GB/s", "kernel": "path_trace_kernel<4,true,false>",
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "achieved": round(alg_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_gbs": round(traffic / (ms * 1e-3) / 1e9, 1)}
     return scene, {
         "workload": f"voxel-cloud path tracing {scene.width}x{scene.height}, {w['spp']} spp (frames of 1 sample), "
                     f"512^3 FP16 fBm-lognormal grid, <= {w['init'].max_bounces} bounces, albedo {w['init'].albedo}, "
@@ -128,6 +156,7 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
         "single_majorant_ms_per_frame": round(ref_ms, 3),
         "single_majorant_density_lookups_per_frame": ref_counts["lookups"],
         "brick_speedup_vs_single_majorant": round(ref_ms / ms, 2),
+        "roofline": roofline,
     }
 
 
@@ -339,6 +368,10 @@ def cpu_baseline(scene, target_seconds):
                       f"{f', {reps} passes' if reps > 1 else ''}, oracle FP32 OpenMP, {t:.1f} s"}
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -356,7 +389,9 @@ def main():
              counts["cloud_steps"] * FLOP_PER_CLOUD_STEP + steps * FLOP_PER_STEP +
              scene.width * scene.height * FLOP_PER_PIXEL)
     achieved = flops / (ms * 1e-3) / 1e12
+    progress(f"raymarch {ms:.4f} ms per frame")
     lut_ms, lut_steps = bench_lut_set(ctx, args.lut_rounds)
+    progress(f"LUT set {lut_ms:.4f} ms")
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gsample-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -368,7 +403,9 @@ def main():
                    "density_evals_per_frame": counts["evals"], "parallelism": "1 GPU"},
         "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak; the kernel is VALU/L2-latency bound",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": profiled_traffic("cloud_raymarch_kernel<1,false>"),
+                     "traffic_gbs": round(profiled_traffic("cloud_raymarch_kernel<1,false>") / (ms * 1e-3) / 1e9, 1),
                      "kernel": "cloud_raymarch_kernel<1,false>",
                      "algorithmic_flop_per_launch": flops},
         "lut_set_ms": round(lut_ms, 4),
@@ -378,13 +415,17 @@ def main():
     pt_scene = None
     if not args.no_pathtrace:
         pt_scene, out["path_trace"] = bench_pathtrace(ctx, args.pt_frames)
+        progress(f"path tracer {out['path_trace']['ms_per_frame']} ms per frame")
     batch = None
     if not args.no_batch:
         batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
+        progress(f"LUT batch {out['lut_batch']['ms_per_launch']} ms per launch")
     if not args.no_surrogate:
         out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs)
+        progress(f"surrogate {out['surrogate']['surrogate_ms']} ms vs plain {out['surrogate']['plain_ms']} ms")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
+        progress("raymarch CPU baseline done")
         if pt_scene is not None:
             out["path_trace"]["cpu_baseline"] = cpu_baseline_pathtrace(pt_scene, args.cpu_seconds / 2)
         if batch is not None:
