@@ -262,6 +262,8 @@ register("oracle_pt_hg_sample_f64", None, [P(_abi.PathTracingParametersC), C.c_u
                                            P(C.c_uint32)])
 register("oracle_pt_transmittance_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
                                               P(_abi.PathTracingParametersC), C.c_uint32, vp, vp])
+register("oracle_pt_track_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
+                                      P(_abi.PathTracingParametersC), vp, i32, C.c_uint32, vp, vp])
 register("oracle_pt_sigma_t_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
                                         P(_abi.PathTracingParametersC), f64, f64, f64])
 register("oracle_voxel_grid_f16", None, [P(_abi.VoxelGridCreateInfoC), i32, i32, i32, vp])
@@ -274,13 +276,16 @@ register("oracle_half_to_float", f32, [C.c_uint16])
 class HostPathTraceInputs:
     """Host copies of the path tracer's inputs, kept alive for the oracle call.
     grid: uint16 FP16 bits [z, y, x] (VOXEL_F16 material), T / sv_l / sv_t: RGBA32F 2D LUTs,
-    ap_l / ap_t: RGBA32F [d, h, w, 4]."""
+    ap_l / ap_t: RGBA32F [d, h, w, 4]; majorants: float32 [bz, by, bx] brick majorants (tracking
+    over the brick grid instead of the single majorant)."""
 
     def __init__(self, material, T, sv_l, sv_t, ap_l, ap_t, grid=None, cloud_map=None, detail=None,
-                 displacement=None, voxel=None):
+                 displacement=None, voxel=None, majorants=None):
+        if majorants is not None:
+            majorants = np.asarray(majorants, np.float32)
         arrs = [np.ascontiguousarray(x) if x is not None else None
-                for x in (T, sv_l, sv_t, ap_l, ap_t, grid, cloud_map, detail, displacement, voxel)]
-        T, sv_l, sv_t, ap_l, ap_t, grid, cloud_map, detail, displacement, voxel = arrs
+                for x in (T, sv_l, sv_t, ap_l, ap_t, grid, cloud_map, detail, displacement, voxel, majorants)]
+        T, sv_l, sv_t, ap_l, ap_t, grid, cloud_map, detail, displacement, voxel, majorants = arrs
         self.keep = [x for x in arrs if x is not None]
         m = _abi.CloudMaterial()
         C.memmove(C.byref(m), C.byref(material), C.sizeof(m))
@@ -296,13 +301,14 @@ class HostPathTraceInputs:
         i.sky_view_width, i.sky_view_height = sv_l.shape[1], sv_l.shape[0]
         i.ap_luminance, i.ap_transmittance = _ptr(ap_l), _ptr(ap_t)
         i.ap_depth, i.ap_height, i.ap_width = ap_l.shape[0], ap_l.shape[1], ap_l.shape[2]
-        i.brick_majorants = None
+        i.brick_majorants = _ptr(majorants)
         self.inputs = i
 
 
 def path_trace(atm, render, common, params, inputs: HostPathTraceInputs, width, height, pixels, first_frame, frames,
                precision="f64"):
-    """Samples of the reference tracer (single majorant): array [len(pixels), frames, 4]."""
+    """Samples of the tracer (single majorant, or brick majorants when the inputs hold them):
+    array [len(pixels), frames, 4]."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     out = np.empty((len(px), frames, 4), _dt(precision))
     getattr(lib(), f"oracle_path_trace_{precision}")(
@@ -325,6 +331,16 @@ def pt_transmittance(common, material, params, seed, o, d):
     d = np.ascontiguousarray(d, dtype=np.float64)
     return lib().oracle_pt_transmittance_f64(C.byref(common), C.byref(material), C.byref(params), seed, _ptr(o),
                                              _ptr(d))
+
+
+def pt_track(common, material, params, seed, o, d, mode="delta", majorants=None):
+    """One tracking call: mode "delta" returns the collision distance (-1: none), "ratio" the
+    transmittance estimate; majorants (float32 [bz, by, bx]) selects brick-majorant tracking."""
+    o = np.ascontiguousarray(o, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    m = None if majorants is None else np.ascontiguousarray(majorants, np.float32)
+    return lib().oracle_pt_track_f64(C.byref(common), C.byref(material), C.byref(params), _ptr(m),
+                                     {"delta": 0, "ratio": 1}[mode], seed, _ptr(o), _ptr(d))
 
 
 def pt_sigma_t(common, material, params, p):

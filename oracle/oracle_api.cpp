@@ -106,6 +106,7 @@ PathTracer<S> make_tracer(const Atmosphere<S>& A, const RenderBlock<S>& R, const
     P.sw = in->sky_view_width, P.sh = in->sky_view_height;
     P.ap_l = in->ap_luminance, P.ap_t = in->ap_transmittance;
     P.aw = in->ap_width, P.ah = in->ap_height, P.ad = in->ap_depth;
+    if (in->brick_majorants && material->kind == SKYHIP_MATERIAL_VOXEL_F16) P.use_bricks(in->brick_majorants);
     return P;
 }
 
@@ -388,6 +389,26 @@ double oracle_pt_transmittance_f64(const SkyhipCloudCommonBufferData* common, co
     RenderBlock<double> R(zero_render);
     PathTracer<double> P(A, R, *common, *material, *params);
     return P.transmittance_estimation(seed, {o[0], o[1], o[2]}, {d[0], d[1], d[2]});
+}
+// One tracking call from o along d (unit) through the cloud region, with the single
+// majorant or (majorants != NULL, VOXEL_F16) the brick grid.  mode 0: delta tracking,
+// returns the collision distance from o or -1 when the ray leaves; mode 1: ratio
+// tracking, returns the transmittance estimate.
+double oracle_pt_track_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
+                           const SkyhipPathTracingParameters* params, const float* majorants, int mode, uint32_t seed,
+                           const double* o, const double* d) {
+    static const SkyhipAtmosphereBufferData zero_atm = {};
+    static const SkyhipAtmosphereRenderBufferData zero_render = {};
+    Atmosphere<double> A(zero_atm);
+    RenderBlock<double> R(zero_render);
+    PathTracer<double> P(A, R, *common, *material, *params);
+    if (majorants) P.use_bricks(majorants);
+    V3<double> oo = {o[0], o[1], o[2]}, dd = {d[0], d[1], d[2]};
+    if (mode == 1) return P.transmittance_estimation(seed, oo, dd);
+    double t0, t1, t;
+    P.region(oo, dd, t0, t1);
+    if (t0 >= t1) return -1.0;
+    return P.delta_track(seed, oo, dd, t0, t1, t) ? t : -1.0;
 }
 double oracle_pt_sigma_t_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
                              const SkyhipPathTracingParameters* params, double x, double y, double z) {

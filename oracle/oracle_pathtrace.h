@@ -6,8 +6,11 @@
 // environment term of the environment-luminance cube
 // (AtmosphereRenderer.glsl:253-273) evaluated directly from the sky-view LUT.
 // Tracking uses the reference's single majorant kSigmaTMax, so each sample
-// consumes the reference's random sequence.  Only the tests and the
-// benchmark's CPU baseline use this code.
+// consumes the reference's random sequence, or (when brick majorants are
+// given) the library's 8^3-brick majorant grid walked in texel space: the same
+// tracking estimators with a piecewise-constant majorant, drawing random
+// numbers in the same order as the library's brick tracker.  Only the tests
+// and the benchmark's CPU baseline use this code.
 #pragma once
 
 #include "oracle_cloud.h"
@@ -126,6 +129,14 @@ struct PathTracer {
     int tw = 0, th = 0, sw = 0, sh = 0, aw = 0, ah = 0, ad = 0;
     V3<S> cam, sun, box_min, box_max;
     S R, hb, ht;
+    // Brick-majorant grid (VOXEL_F16 only): one majorant per 8^3 brick, in texel space
+    // X = pos * tex_scale + tex_bias per axis (z measured from the cloud bottom).
+    const float* majorants = nullptr;
+    int nb[3] = {0, 0, 0};
+    S tex_scale[3] = {0, 0, 0}, tex_bias[3] = {0, 0, 0}, grid_n[3] = {0, 0, 0};
+    S majorant_scale = 0;
+    static constexpr int kMaxTrackingSteps = 1 << 20;  // safety bounds, as the library's tracker
+    static constexpr int kMaxSegments = 4096;
 
     PathTracer(const Atmosphere<S>& A_, const RenderBlock<S>& rb_, const SkyhipCloudCommonBufferData& cb_,
                const SkyhipCloudMaterial& m, const SkyhipPathTracingParameters& p)
@@ -138,6 +149,121 @@ struct PathTracer {
         // kCloudAABBMin/Max (VolumetricCloudPathTracing.comp:19-21)
         box_min = {-S(pp.region_box_half_width), -S(pp.region_box_half_width), hb};
         box_max = {S(pp.region_box_half_width), S(pp.region_box_half_width), ht};
+    }
+
+    void use_bricks(const float* m) {
+        majorants = m;
+        const SkyhipCloudVoxelBufferData& vb = mat.voxel;
+        for (int k = 0; k < 3; ++k) {
+            nb[k] = (mat.voxel_grid_dim[k] + 7) / 8;
+            grid_n[k] = S(mat.voxel_grid_dim[k]);
+        }
+        for (int k = 0; k < 2; ++k) {
+            tex_scale[k] = S(vb.uSampleFrequency[k]) * grid_n[k];
+            tex_bias[k] = S(vb.uSampleBias[k]) * grid_n[k];
+        }
+        tex_scale[2] = grid_n[2] / (ht - hb);
+        tex_bias[2] = 0;
+        majorant_scale = S(vb.uDensity);
+    }
+
+    // Majorant-segment walk over the brick grid.  The walk covers [-1, N + 1] texels in x
+    // and y (the first and last bricks extend one texel beyond the grid, where
+    // CLAMP_TO_BORDER samples are still non-zero); z is bounded by the cloud slab.
+    struct Walk {
+        S org[3], k[3], inv_k[3], tn[3], t_end;
+        int b[3];
+    };
+    S next_boundary(const Walk& w, int i) const {
+        if (w.k[i] > S(0)) return w.b[i] < nb[i] - 1 ? (S(8 * (w.b[i] + 1)) - w.org[i]) * w.inv_k[i] : S(3.0e38);
+        if (w.k[i] < S(0)) return w.b[i] > 0 ? (S(8 * w.b[i]) - w.org[i]) * w.inv_k[i] : S(3.0e38);
+        return S(3.0e38);
+    }
+    bool walk_begin(Walk& w, V3<S> o, V3<S> d, S& t, S t_stop) const {
+        const S oo[3] = {o.x, o.y, o.z - hb}, dd[3] = {d.x, d.y, d.z};
+        S lo = t, hi = t_stop;
+        for (int i = 0; i < 3; ++i) {
+            w.org[i] = oo[i] * tex_scale[i] + tex_bias[i];
+            w.k[i] = dd[i] * tex_scale[i];
+            if (std::fabs(w.k[i]) < S(1e-30)) w.k[i] = 0;
+            w.inv_k[i] = w.k[i] != S(0) ? S(1) / w.k[i] : S(0);
+            if (i < 2) {
+                S e0 = (S(-1) - w.org[i]) * w.inv_k[i], e1 = (grid_n[i] + S(1) - w.org[i]) * w.inv_k[i];
+                if (w.k[i] == S(0)) {
+                    if (w.org[i] < S(-1) || w.org[i] > grid_n[i] + S(1)) return false;
+                } else {
+                    lo = std::fmax(lo, std::fmin(e0, e1));
+                    hi = std::fmin(hi, std::fmax(e0, e1));
+                }
+            }
+        }
+        if (!(lo < hi)) return false;
+        t = lo;
+        w.t_end = hi;
+        for (int i = 0; i < 3; ++i) {
+            S x = t * w.k[i] + w.org[i];
+            w.b[i] = std::min(std::max(int(std::floor(x * S(0.125))), 0), nb[i] - 1);
+            w.tn[i] = next_boundary(w, i);
+        }
+        return true;
+    }
+    S segment_end(const Walk& w) const { return std::fmin(std::fmin(w.tn[0], w.tn[1]), std::fmin(w.tn[2], w.t_end)); }
+    S segment_majorant(const Walk& w) const {
+        return S(majorants[(size_t(w.b[2]) * nb[1] + w.b[1]) * nb[0] + w.b[0]]) * majorant_scale;
+    }
+    void advance(Walk& w) const {
+        int i = (w.tn[0] <= w.tn[1] && w.tn[0] <= w.tn[2]) ? 0 : (w.tn[1] <= w.tn[2] ? 1 : 2);
+        w.b[i] = std::min(std::max(w.b[i] + (w.k[i] > S(0) ? 1 : -1), 0), nb[i] - 1);
+        w.tn[i] = next_boundary(w, i);
+    }
+
+    // Delta tracking over [t0, t1] with the brick majorants; true with t at a real collision.
+    bool delta_track_bricks(uint32_t& seed, V3<S> o, V3<S> d, S t0, S t1, S& t) const {
+        Walk w;
+        t = t0;
+        if (!walk_begin(w, o, d, t, t1)) return false;
+        int it = 0;
+        for (int seg = 0; seg < kMaxSegments; ++seg) {
+            S seg_end = segment_end(w), mu = segment_majorant(w);
+            if (mu > S(0)) {
+                for (; it < kMaxTrackingSteps; ++it) {
+                    S tt = t + -std::log(S(1) - random01(seed)) / mu;
+                    if (!(tt < seg_end)) break;
+                    t = tt;
+                    if (random01(seed) < sigma(o + d * t) / mu) return true;
+                }
+                if (it >= kMaxTrackingSteps) return false;
+            }
+            if (!(seg_end < w.t_end)) return false;
+            t = seg_end;
+            advance(w);
+        }
+        return false;
+    }
+
+    // Ratio tracking with the brick majorants.
+    S ratio_track_bricks(uint32_t seed, V3<S> o, V3<S> d, S t0, S t1) const {
+        S tr = 1;
+        Walk w;
+        S t = t0;
+        if (!walk_begin(w, o, d, t, t1)) return tr;
+        int it = 0;
+        for (int seg = 0; seg < kMaxSegments; ++seg) {
+            S seg_end = segment_end(w), mu = segment_majorant(w);
+            if (mu > S(0)) {
+                for (; it < kMaxTrackingSteps; ++it) {
+                    S tt = t + -std::log(S(1) - random01(seed)) / mu;
+                    if (!(tt < seg_end)) break;
+                    t = tt;
+                    tr *= S(1) - std::fmax(S(0), sigma(o + d * t) / mu);
+                }
+                if (it >= kMaxTrackingSteps || tr == S(0)) break;
+            }
+            if (!(seg_end < w.t_end)) break;
+            t = seg_end;
+            advance(w);
+        }
+        return gclamp(tr, S(0), S(1));
     }
 
     uint32_t prng(uint32_t x) const { return pp.prng == SKYHIP_PRNG_WANG_HASH ? WangHash(x) : PCGHash(x); }
@@ -237,6 +363,7 @@ struct PathTracer {
         S t0, t1;
         region(o, d, t0, t1);
         if (t0 >= t1) return tr;
+        if (majorants) return ratio_track_bricks(seed, o, d, t0, t1);
         S smax = S(pp.sigma_t_max);
         S t = t0;
         for (;;) {
@@ -245,6 +372,20 @@ struct PathTracer {
             tr *= S(1) - std::fmax(S(0), sigma(o + d * t) / smax);
         }
         return gclamp(tr, S(0), S(1));
+    }
+
+    // Delta tracking over [t0, t1] (:182-196): true with t at a real collision.
+    bool delta_track(uint32_t& seed, V3<S> o, V3<S> d, S t0, S t1, S& t) const {
+        if (majorants) return delta_track_bricks(seed, o, d, t0, t1, t);
+        const S smax = S(pp.sigma_t_max);
+        t = t0;
+        if (smax <= S(0)) return false;
+        for (;;) {
+            t += -std::log(S(1) - random01(seed)) / smax;
+            if (t > t1) return false;
+            S s = sigma(o + d * t);
+            if (random01(seed) < s / smax) return true;
+        }
     }
 
     // SampleLuminanceFromLight (:153-158); the estimator gets the state by value.
@@ -284,26 +425,14 @@ struct PathTracer {
         region(o, d, c0, c1);
         if (c0 >= c1) return L;
         o = o + c0 * d;
-        const S smax = S(pp.sigma_t_max);
         for (int istep = 0;
              istep < pp.max_bounces && std::fmax(throughput.x, std::fmax(throughput.y, throughput.z)) > S(0);
              ++istep) {
             S t0, t1;
             region(o, d, t0, t1);
             if (t0 >= t1) break;
-            S t = t0;
-            bool scatter = false;
-            for (;;) {
-                if (smax <= S(0)) break;
-                t += -std::log(S(1) - random01(seed)) / smax;
-                if (t > t1) break;
-                S s = sigma(o + d * t);
-                if (random01(seed) < s / smax) {
-                    scatter = true;
-                    break;
-                }
-            }
-            if (!scatter) {
+            S t;
+            if (!delta_track(seed, o, d, t0, t1, t)) {
                 if (pp.environment_lighting == SKYHIP_ENV_OFF) break;
                 if (!has_scattered) break;
                 if (pp.environment_lighting == SKYHIP_ENV_CONST_ENVIRONMENT_MAP) {

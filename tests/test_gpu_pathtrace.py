@@ -9,11 +9,13 @@
   arithmetic take a different discrete branch (a tentative collision accepted on one
   side only) diverges from there on; at least 90 % of the samples must match, and the
   mismatches must not move the mean (3 sigma).
-- Brick-majorant mode (the workload's tracker) draws different random numbers, so its
-  converged mean is compared: with the FP64 oracle's (1024 vs 256 spp) through the
-  image-sum z-score (within 3) and the coverage; per pixel, at 3072 spp each, with the
-  single-majorant mode (batch means of 64 samples): image z within 3, at most 1 % of
-  the pixels with |z| > 4 (per-pixel luminance is heavy-tailed).
+- Brick-majorant mode (the workload's tracker) is compared sample by sample with the
+  oracle's brick-majorant tracking (same random sequence, same bar as above), and its
+  converged mean with the reference's single-majorant tracking, which draws different
+  random numbers: with the FP64 oracle's (1024 vs 256 spp) through the image-sum z-score
+  (within 3) and the coverage; per pixel, at 3072 spp each, with the single-majorant mode
+  (batch means of 64 samples): image z within 3, at most 1 % of the pixels with |z| > 4
+  (per-pixel luminance is heavy-tailed).
 """
 import ctypes as C
 import math
@@ -30,9 +32,10 @@ import _pt_cases as cases
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_inputs(scene):
+def _oracle_inputs(scene, bricks=False):
     T, sv_l, sv_t, ap_l, ap_t, grid = scene.host_inputs()
-    return oracle.HostPathTraceInputs(scene.material, T, sv_l, sv_t, ap_l, ap_t, grid=grid)
+    majorants = scene.grid.download_majorants() if bricks else None
+    return oracle.HostPathTraceInputs(scene.material, T, sv_l, sv_t, ap_l, ap_t, grid=grid, majorants=majorants)
 
 
 def _gpu_samples(scene, frames, first=1):
@@ -99,6 +102,27 @@ def test_single_majorant_samples_match_oracle(ctx):
     se = math.sqrt(lum_g.var() / lum_g.size + lum_o.var() / lum_o.size)
     z = (lum_g.mean() - lum_o.mean()) / se
     print(f"single majorant: {frac:.4f} of samples match ({scattered.mean():.3f} scattered, "
+          f"{match[scattered].mean():.4f} of those); mean z {z:.2f}")
+    assert scattered.mean() > 0.1
+    assert frac >= 0.9 and abs(z) < 3.0
+
+
+def test_brick_majorant_samples_match_oracle(ctx):
+    w = cases.small_grid_workload()
+    scene = PathTraceScene(ctx, 32, 32, workload=w, bricks=True)
+    frames = 4
+    gpu = _gpu_samples(scene, frames)
+    ref = oracle.path_trace(scene.atmosphere.buffer, scene.renderer.buffer, scene.common, scene.tracer.params,
+                            _oracle_inputs(scene, bricks=True), 32, 32, np.arange(32 * 32), 1, frames)
+    floor = 1e-6 * np.abs(ref[..., :3]).max()
+    err = np.abs(gpu - ref) / np.maximum(np.abs(ref), floor)
+    match = np.all(err <= 1e-3, axis=-1)
+    scattered = ref[..., 3] == 0
+    frac = float(match.mean())
+    lum_g, lum_o = gpu[..., :3].sum(-1).ravel(), ref[..., :3].sum(-1).ravel()
+    se = math.sqrt(lum_g.var() / lum_g.size + lum_o.var() / lum_o.size)
+    z = (lum_g.mean() - lum_o.mean()) / se
+    print(f"brick majorants: {frac:.4f} of samples match ({scattered.mean():.3f} scattered, "
           f"{match[scattered].mean():.4f} of those); mean z {z:.2f}")
     assert scattered.mean() > 0.1
     assert frac >= 0.9 and abs(z) < 3.0

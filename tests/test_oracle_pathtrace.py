@@ -221,3 +221,66 @@ def test_importance_and_uniform_phase_sampling_agree():
         res.append((x.mean(), x.var() / len(x)))
     (m0, v0), (m1, v1) = res
     assert abs(m0 - m1) < 4 * math.sqrt(v0 + v1), (m0, m1)
+
+
+def _optical_depth(common, mat, prm, o, d, t0, t1, n=4000):
+    """FP64 midpoint quadrature of sigma along o + t d over [t0, t1]."""
+    h = (t1 - t0) / n
+    return h * sum(oracle.pt_sigma_t(common, mat, prm, o + d * (t0 + (i + 0.5) * h)) for i in range(n))
+
+
+def _tracking_z_scores(common, mat, prm, rays, majorants, n=4000):
+    """Per ray: z-scores of the ratio-tracking mean against exp(-tau), and of the delta-tracking
+    probability of a collision before 0.3, 0.6 and 1.0 of the ray against 1 - exp(-tau(x))."""
+    seeds = [oracle.lib().oracle_pcg_hash(i + 77) for i in range(n)]
+    out = []
+    for a, b in rays:
+        o, e = np.array(a, np.float64), np.array(b, np.float64)
+        d = (e - o) / np.linalg.norm(e - o)
+        t1 = float(np.linalg.norm(e - o))
+        tau = _optical_depth(common, mat, prm, o, d, 0.0, t1)
+        tr = np.array([oracle.pt_track(common, mat, prm, s, o, d, "ratio", majorants) for s in seeds])
+        z = [(tr.mean() - math.exp(-tau)) / (max(tr.std(), 1e-3) / math.sqrt(n))]
+        tc = np.array([oracle.pt_track(common, mat, prm, s, o, d, "delta", majorants) for s in seeds])
+        for frac in (0.3, 0.6, 1.0):
+            p = 1.0 - math.exp(-_optical_depth(common, mat, prm, o, d, 0.0, frac * t1, 1000))
+            hit = np.mean((tc >= 0) & (tc <= frac * t1))
+            z.append((hit - p) / math.sqrt(max(p * (1 - p), 1e-4) / n))
+        out.append((tau, z))
+    return out
+
+
+def test_brick_tracking_is_unbiased():
+    """Delta and ratio tracking over the 8^3-brick majorant grid (and over the single
+    majorant) against exp(-optical depth) from quadrature of sigma along each ray.  The grid
+    is constant per brick with a gradient across bricks and a three-brick empty gap, so the
+    majorants vary and some bricks are empty; rays are oblique, cross the gap, and two graze
+    the half-texel shell outside the grid where CLAMP_TO_BORDER samples are still non-zero.
+    The same statistics reject majorants scaled by 0.7 and majorants mirrored along x (an
+    index error), so they have the power to see a wrong walk."""
+    dims = (64, 48, 32)
+    bz, by, bx = np.meshgrid(np.arange(4), np.arange(6), np.arange(8), indexing="ij")
+    vals = (0.25 * (bx + 2 * by + bz)).astype(np.float16)
+    vals[:, :, 2:5] = 0
+    g = np.ascontiguousarray(np.repeat(np.repeat(np.repeat(vals, 8, 0), 8, 1), 8, 2).view(np.uint16))
+    M = oracle.brick_majorants(g)
+    assert (M == 0).any() and len(np.unique(M)) > 10
+    mat = _abi.CloudMaterial()
+    mat.kind = _abi.MATERIAL_VOXEL_F16
+    mat.voxel.uSampleFrequency[0] = mat.voxel.uSampleFrequency[1] = 1.0
+    mat.voxel.uDensity = 1.0
+    mat.voxel_grid = g.ctypes.data
+    for k in range(3):
+        mat.voxel_grid_dim[k] = dims[k]
+    common = cases.straight_up_common(hb=0.0, ht=1.0)
+    prm = cases.params(float(vals.max()), half_width=3.0)
+    shell = 0.25 / dims[0]  # a quarter texel outside the grid
+    rays = [((-0.3, 0.2, 0.0), (1.2, 0.7, 1.0)), ((1.4, 0.9, 1.0), (-0.2, 0.1, 0.0)),
+            ((0.0, 0.3, 0.0), (1.0, 0.6, 1.0)), ((-shell, 0.05, 0.0), (-shell, 0.95, 1.0)),
+            ((0.1, 0.5, 0.0), (0.9, 0.5, 1.0)), ((1.0 + shell, 0.9, 1.0), (1.0 + shell, 0.1, 0.0))]
+    for majorants in (M, None):
+        for tau, z in _tracking_z_scores(common, mat, prm, rays, majorants):
+            assert tau > 0.3 and max(abs(x) for x in z) < 4.0, (majorants is None, tau, z)
+    for wrong in (M * np.float32(0.7), np.ascontiguousarray(np.flip(M, 2))):
+        zs = _tracking_z_scores(common, mat, prm, rays, wrong)
+        assert max(max(abs(x) for x in z) for _, z in zs) > 6.0
