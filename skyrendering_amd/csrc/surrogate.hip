@@ -392,20 +392,33 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
     if (tid == 0) partial_loss[blockIdx.x] = S.loss;
 }
 
-// grad[p] = sum over workgroups of partial[wg][p] (fixed order); loss likewise.
-__global__ __launch_bounds__(256) void surrogate_reduce_kernel(const float* __restrict__ partial,
-                                                               const float* __restrict__ partial_loss, int groups,
-                                                               float* __restrict__ grad, float* __restrict__ loss) {
-    int p = blockIdx.x * 256 + threadIdx.x;
+// grad[p] = sum over workgroups of partial[wg][p]; loss likewise (as parameter kParams).
+// A 1024-thread block covers 64 parameters x 16 ranges of workgroups: each thread sums
+// its range in order, then the 16 range sums are added in order, so the result does not
+// depend on scheduling.
+__global__ __launch_bounds__(1024) void surrogate_reduce_kernel(const float* __restrict__ partial,
+                                                                const float* __restrict__ partial_loss, int groups,
+                                                                float* __restrict__ grad, float* __restrict__ loss) {
+    __shared__ float s_part[16][64];
+    const int lane_p = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane_p;
+    const int chunk = (groups + 15) / 16;
+    const int g0 = part * chunk, g1 = min(groups, g0 + chunk);
+    float s = 0.0f;
     if (p < kParams) {
-        float s = 0.0f;
-        for (int g = 0; g < groups; ++g) s += partial[(long long)g * kParams + p];
-        grad[p] = s;
+        for (int g = g0; g < g1; ++g) s += partial[(long long)g * kParams + p];
+    } else if (p == kParams && loss) {
+        for (int g = g0; g < g1; ++g) s += partial_loss[g];
     }
-    if (p == 0 && loss) {
-        float s = 0.0f;
-        for (int g = 0; g < groups; ++g) s += partial_loss[g];
-        *loss = s;
+    s_part[part][lane_p] = s;
+    __syncthreads();
+    if (part == 0) {
+        float t = 0.0f;
+        for (int k = 0; k < 16; ++k) t += s_part[k][lane_p];
+        if (p < kParams)
+            grad[p] = t;
+        else if (p == kParams && loss)
+            *loss = t;
     }
 }
 
@@ -478,28 +491,33 @@ __global__ __launch_bounds__(256, 2) void sky_render_surrogate_kernel(const floa
 }
 
 // Residual check: probes at the 4 quarter points of each 16 x 16 tile; a tile whose
-// largest |surrogate - plain| / max(plain, floor) exceeds the tolerance is flagged.
-__global__ __launch_bounds__(64) void sky_render_check_kernel(SkyRenderArgs a) {
-    const int tile = blockIdx.x, view = blockIdx.y;
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    __shared__ float s_err[4];
-    if (threadIdx.x < 4) {
-        int x = min(tx * 16 + 4 + 8 * (threadIdx.x & 1), a.width - 1);
-        int y = min(ty * 16 + 4 + 8 * (threadIdx.x >> 1), a.height - 1);
-        SkyhipSkyQuery q = panorama_query(a.views[view], x, y, a.width, a.height);
-        F3 ref = sky_radiance_q(a.luts, q);
-        long long idx = (long long)view * a.width * a.height + (long long)y * a.width + x;
-        float4 s = a.out[idx];
-        float e = fmaxf(fmaxf(fabsf(s.x - ref.x) / fmaxf(ref.x, a.floor), fabsf(s.y - ref.y) / fmaxf(ref.y, a.floor)),
-                        fabsf(s.z - ref.z) / fmaxf(ref.z, a.floor));
-        s_err[threadIdx.x] = (e == e) ? e : 3.0e38f;  // NaN counts as a failure
+// largest |surrogate - plain| / max(plain, floor) exceeds the tolerance is flagged.  One
+// thread per probe: the 4 probes of a tile are 4 adjacent lanes, reduced with shuffles.
+__global__ __launch_bounds__(256) void sky_render_check_kernel(SkyRenderArgs a, int view_count) {
+    const long long tiles_per_view = (long long)a.tiles_x * a.tiles_y;
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;  // probe index
+    const long long tile_global = p >> 2;
+    const bool live = tile_global < tiles_per_view * view_count;
+    float e = 0.0f;
+    if (live) {
+        const int view = int(tile_global / tiles_per_view), tile = int(tile_global % tiles_per_view);
+        const int tx = tile % a.tiles_x, ty = tile / a.tiles_x, i = int(p & 3);
+        const int x = min(tx * 16 + 4 + 8 * (i & 1), a.width - 1);
+        const int y = min(ty * 16 + 4 + 8 * (i >> 1), a.height - 1);
+        const SkyhipSkyQuery q = panorama_query(a.views[view], x, y, a.width, a.height);
+        const F3 ref = sky_radiance_q(a.luts, q);
+        const long long idx = (long long)view * a.width * a.height + (long long)y * a.width + x;
+        const float4 s = a.out[idx];
+        e = fmaxf(fmaxf(fabsf(s.x - ref.x) / fmaxf(ref.x, a.floor), fabsf(s.y - ref.y) / fmaxf(ref.y, a.floor)),
+                  fabsf(s.z - ref.z) / fmaxf(ref.z, a.floor));
+        if (!(e == e)) e = 3.0e38f;  // NaN counts as a failure
         a.out[idx] = make_float4(ref.x, ref.y, ref.z, 0.0f);  // probes keep the plain value
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float e = fmaxf(fmaxf(s_err[0], s_err[1]), fmaxf(s_err[2], s_err[3]));
-        unsigned f = e > a.tolerance ? 1u : 0u;
-        a.flags[(long long)view * a.tiles_x * a.tiles_y + tile] = f;
+    e = fmaxf(e, __shfl_xor(e, 1));
+    e = fmaxf(e, __shfl_xor(e, 2));
+    if (live && (p & 3) == 0) {
+        const unsigned f = e > a.tolerance ? 1u : 0u;
+        a.flags[tile_global] = f;
         if (f) atomicAdd(&a.counters[0], 1u);
     }
 }
@@ -647,7 +665,7 @@ extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, const float* params
     hipLaunchKernelGGL(surrogate_train_kernel, dim3(groups), dim3(256), sizeof(TrainLds), ctx->stream, params,
                        features, targets, weights, count, loss_scale, partial, partial_loss);
     SKYHIP_LAUNCH_CHECK("surrogate_train_kernel");
-    hipLaunchKernelGGL(surrogate_reduce_kernel, dim3(div_up(kParams, 256)), dim3(256), 0, ctx->stream, partial,
+    hipLaunchKernelGGL(surrogate_reduce_kernel, dim3(div_up(kParams + 1, 64)), dim3(1024), 0, ctx->stream, partial,
                        partial_loss, groups, out_grad, out_loss);
     SKYHIP_LAUNCH_CHECK("surrogate_reduce_kernel");
     return SKYHIP_OK;
@@ -710,7 +728,8 @@ extern "C" int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* para
     const int blocks = int(std::max(1LL, std::min((groups + 3) / 4, (long long)ctx->cu_count * 8)));
     hipLaunchKernelGGL(sky_render_surrogate_kernel, dim3(blocks), dim3(256), 0, st, params, a, view_count);
     dim3 tiles(a.tiles_x * a.tiles_y, view_count);
-    hipLaunchKernelGGL(sky_render_check_kernel, tiles, dim3(64), 0, st, a);
+    const long long probes = 4LL * a.tiles_x * a.tiles_y * view_count;
+    hipLaunchKernelGGL(sky_render_check_kernel, dim3(unsigned((probes + 255) / 256)), dim3(256), 0, st, a, view_count);
     hipLaunchKernelGGL(sky_render_plain_kernel, tiles, dim3(256), 0, st, a, 0);
     SKYHIP_LAUNCH_CHECK("sky render (surrogate)");
     return SKYHIP_OK;

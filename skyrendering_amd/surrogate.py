@@ -152,14 +152,16 @@ def brightness_weights(radiance, reference=1e-2) -> np.ndarray:
     return np.sqrt(np.minimum(1.0, np.maximum(y, 0.0) / reference)).astype(np.float32)
 
 
-def random_queries(n, n_atm, thickness, seed) -> np.ndarray:
-    """Uniform samples: atmosphere, altitude [0, thickness], mu_view and mu_sun [-1, 1], delta_phi [0, pi]."""
+def random_queries(n, n_atm, thickness, seed, altitude_range=None, mu_sun_range=(-1.0, 1.0)) -> np.ndarray:
+    """Uniform samples: atmosphere, altitude in altitude_range (default [0, thickness]), mu_view
+    in [-1, 1], mu_sun in mu_sun_range, delta_phi in [0, pi]."""
     rng = np.random.default_rng(seed)
+    lo, hi = altitude_range if altitude_range is not None else (0.0, thickness)
     q = np.zeros(n, QUERY_DTYPE)
     q["atmosphere"] = rng.integers(0, n_atm, n)
-    q["altitude"] = rng.uniform(0.0, thickness, n)
+    q["altitude"] = rng.uniform(lo, hi, n)
     q["mu_view"] = rng.uniform(-1.0, 1.0, n)
-    q["mu_sun"] = rng.uniform(-1.0, 1.0, n)
+    q["mu_sun"] = rng.uniform(*mu_sun_range, n)
     q["delta_phi"] = rng.uniform(0.0, math.pi, n)
     return q
 
@@ -167,10 +169,12 @@ def random_queries(n, n_atm, thickness, seed) -> np.ndarray:
 class SkyTrainingSet:
     """Queries, surrogate inputs and plain-integrator labels (log radiance), on the device."""
 
-    def __init__(self, ctx: Context, atmospheres: SkyAtmospheres, n, seed=21):
+    def __init__(self, ctx: Context, atmospheres: SkyAtmospheres, n, seed=21, altitude_range=None,
+                 mu_sun_range=(-1.0, 1.0)):
         self.ctx, self.n = ctx, n
         self.atmospheres = atmospheres
-        self.queries_host = random_queries(n, len(atmospheres.scales), atmospheres.thickness, seed)
+        self.queries_host = random_queries(n, len(atmospheres.scales), atmospheres.thickness, seed, altitude_range,
+                                           mu_sun_range)
         self.queries = ctx.upload(self.queries_host)
         self.features = ctx.alloc(n * 32)
         _abi.check(_abi.lib().skyhip_surrogate_features(ctx.handle, self.queries.ptr, n, atmospheres.log_scales.ptr,
