@@ -160,6 +160,54 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     }
 
 
+def bench_environment(ctx, reps=20):
+    """The environment-luminance cube (128^2 x 6 RGBA16F with its mip chain) and IBL::Precompute
+    (SH9 and the 128^2, 5-roughness prefiltered cube) for the Earth preset, per update, timed
+    with HIP events; the oracle's FP32 path on the host beside it."""
+    from skyrendering_amd import Atmosphere, AtmosphereRenderer, AtmosphereRenderInitParameters, presets
+    from skyrendering_amd.atmosphere import AtmosphereRenderParameters
+    from skyrendering_amd.ibl import IBL, EnvironmentLuminance
+    p = presets.earth_100km()
+    atm = Atmosphere(ctx)
+    atm.update_luts(p)
+    cam = presets.bench_camera()
+    rp = AtmosphereRenderParameters(sun_direction_theta=45.0, sky_view_lut_steps=30.0, camera_position=cam.position,
+                                    view_projection=cam.view_projection())
+    rend = AtmosphereRenderer(ctx, AtmosphereRenderInitParameters(sky_view_lut_size=(192, 108)))
+    rend.render(atm, p, rp, aerial_perspective=False)
+    env, ibl = EnvironmentLuminance(ctx), IBL(ctx)
+
+    def update():
+        env.update(atm, rend)
+        ibl.precompute(env)
+
+    update()
+    ctx.synchronize()
+    ctx.record(12)
+    for _ in range(reps):
+        update()
+    ctx.record(13)
+    ms = ctx.elapsed_ms(12, 13) / reps
+    w, pw, n_r = env.width, ibl.resolution, ibl.roughness_count
+    counts = [max(1, int(1.0 * (1 - (l / (n_r - 1)) ** 0.3) + 64.0 * (l / (n_r - 1)) ** 0.3)) for l in range(n_r)]
+    prefilter_samples = sum(6 * (pw >> l) ** 2 * c for l, c in enumerate(counts))
+    out = {"workload": f"environment cube {w}^2 x 6 RGBA16F + {env.levels - 1} mip levels, SH9 (9 x 1024 samples), "
+                       f"prefiltered cube {pw}^2 x {n_r} roughness levels (GGX, 1-64 samples), Earth, sun 45 deg",
+           "ms_per_update": round(ms, 4), "env_texels": 6 * w * w, "prefilter_samples": prefilter_samples,
+           "bound": "launch latency (14 dependent launches of at most 98,304 threads)"}
+    import oracle
+    sv_l, sv_t = rend.sky_view()
+    T = atm.transmittance()
+    t0 = time.perf_counter()
+    cube = oracle.environment_cube(atm.buffer, rend.buffer, T, sv_l, sv_t, w, precision="f32")
+    oracle.prefiltered_radiance(cube, w, pw, n_r, precision="f32")
+    cpu_s = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms per update (environment cube + prefilter)",
+                           "cores": oracle.max_threads(), "kind": "port",
+                           "sample": "one full update, oracle FP32 OpenMP (SH9 excluded: single-threaded FP64)"}
+    return out
+
+
 FLOP_PER_INTEGRATOR_STEP = 150  # sky-view / MS step: 2 LUT fetches, ~7 transcendentals, RGB accumulation
 
 
@@ -434,6 +482,8 @@ def main():
         "lut_set_integrator_steps": lut_steps,
         "device": dev,
     }
+    out["environment"] = bench_environment(ctx)
+    progress(f"environment cube + IBL {out['environment']['ms_per_update']} ms per update")
     pt_scene = None
     if not args.no_pathtrace:
         pt_scene, out["path_trace"] = bench_pathtrace(ctx, args.pt_frames)

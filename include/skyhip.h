@@ -454,6 +454,42 @@ int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosphereBufferDa
                               int32_t ap_height, int32_t ap_depth);
 
 /* ------------------------------------------------------------------------- */
+/* Environment-luminance cube and image-based lighting                       */
+/* ------------------------------------------------------------------------- */
+
+/* Cube maps are RGBA16F (4 x uint16 binary16 per texel) with a full mip chain stored level
+ * by level: level l holds faces +X -X +Y -Y +Z -Z, each w_l x w_l texels row-major with
+ * w_l = width >> l, texel (x, y) at face uv ((x + 0.5) / w_l, 1 - (y + 0.5) / w_l) as the
+ * reference's passes write it.  Returns the texel count of the first `levels` levels
+ * (levels = log2(width) + 1 for the whole chain). */
+long long skyhip_cube_texels(int32_t width, int32_t levels);
+
+/* The environment-luminance pass of AtmosphereRenderer::Render and its mip chain
+ * (AtmosphereRenderer.cpp:234-242, AtmosphereRenderer.glsl:248-275): each texel is the
+ * sky-view luminance along its direction, plus the lit ground times the sky-view
+ * transmittance where the direction hits the ground; glGenerateTextureMipmap is a 2x2 box.
+ * width: a power of two (the reference uses 128); out_cube: skyhip_cube_texels(width,
+ * log2(width) + 1) texels. */
+int skyhip_environment_luminance(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
+                                 const SkyhipAtmosphereRenderBufferData* render, const float* transmittance_rgba,
+                                 int32_t transmittance_width, int32_t transmittance_height,
+                                 const float* sky_view_luminance_rgba, const float* sky_view_transmittance_rgba,
+                                 int32_t sky_view_width, int32_t sky_view_height, void* out_cube_rgba16f,
+                                 int32_t width);
+
+/* IBL::Precompute (src/Base/src/IBL.cpp:25-45): the 9 spherical-harmonic radiance
+ * coefficients of the environment cube (EnvRadianceSH.comp; out_sh9_rgba: device vec4[9],
+ * a = 0) and the GGX-prefiltered radiance cube (PrefilterRadiance.comp) with roughness_count
+ * levels, level l at roughness l / (roughness_count - 1) and width prefiltered_width >> l
+ * (the reference: 128 and 5).  environment_cube: the full mip chain of
+ * skyhip_environment_luminance; out_prefiltered: skyhip_cube_texels(prefiltered_width,
+ * roughness_count) texels.  Lookups filter bilinearly within a face and linearly between
+ * levels (the anisotropy sampler at an explicit level of detail). */
+int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment_cube_rgba16f, int32_t environment_width,
+                          float* out_sh9_rgba, void* out_prefiltered_rgba16f, int32_t prefiltered_width,
+                          int32_t roughness_count);
+
+/* ------------------------------------------------------------------------- */
 /* Cloud textures: noise generators and mip chains                           */
 /* ------------------------------------------------------------------------- */
 

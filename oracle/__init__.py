@@ -395,3 +395,79 @@ def sky_radiance(atmospheres, T, MS, steps, queries):
     lib().oracle_sky_radiance_f64(atmospheres, _ptr(T), T.shape[2], T.shape[1], _ptr(MS), MS.shape[2], MS.shape[1],
                                   float(steps), _ptr(q), len(q), _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Environment-luminance cube and image-based lighting
+# ---------------------------------------------------------------------------
+_CUBE_ARGS = [P(_abi.AtmosphereBufferData), P(_abi.AtmosphereRenderBufferData), vp, i32, i32, vp, vp, i32, i32, i32, vp]
+register("oracle_environment_cube_f64", None, _CUBE_ARGS)
+register("oracle_environment_cube_f32", None, _CUBE_ARGS)
+register("oracle_cube_mips", None, [vp, i32])
+register("oracle_env_radiance_sh_f64", None, [vp, i32, vp])
+register("oracle_prefiltered_radiance_f64", None, [vp, i32, i32, i32, vp])
+register("oracle_prefiltered_radiance_f32", None, [vp, i32, i32, i32, vp])
+register("oracle_cube_sample_f64", None, [vp, i32, i32, vp, f64, vp])
+register("oracle_cube_texel_dir_f64", None, [i32, i32, i32, i32, vp])
+register("oracle_cube_face_st_f64", None, [vp, P(i32), P(f64), P(f64)])
+
+
+def cube_texels(width, levels):
+    return sum(6 * max(width >> l, 1) ** 2 for l in range(levels))
+
+
+def cube_levels(width):
+    return int(width).bit_length()
+
+
+def environment_cube(atm, render, T, sv_l, sv_t, width, precision="f64"):
+    """The environment-luminance cube with its mip chain: uint16 (binary16 bits) [texels, 4]."""
+    T, sv_l, sv_t = (np.ascontiguousarray(x, dtype=np.float32) for x in (T, sv_l, sv_t))
+    out = np.empty((cube_texels(width, cube_levels(width)), 4), np.uint16)
+    getattr(lib(), f"oracle_environment_cube_{precision}")(
+        C.byref(atm), C.byref(render), _ptr(T), T.shape[1], T.shape[0], _ptr(sv_l), _ptr(sv_t), sv_l.shape[1],
+        sv_l.shape[0], width, _ptr(out))
+    return out
+
+
+def cube_mips(cube_bits, width):
+    """Recomputes levels 1.. of a cube (uint16 [texels, 4]) from its level 0 (2x2 box)."""
+    c = np.ascontiguousarray(cube_bits, dtype=np.uint16).copy()
+    lib().oracle_cube_mips(_ptr(c), width)
+    return c
+
+
+def env_radiance_sh(cube_bits, width):
+    c = np.ascontiguousarray(cube_bits, dtype=np.uint16)
+    out = np.empty((9, 4), np.float64)
+    lib().oracle_env_radiance_sh_f64(_ptr(c), width, _ptr(out))
+    return out
+
+
+def prefiltered_radiance(cube_bits, width, prefiltered_width, roughness_count, precision="f64"):
+    c = np.ascontiguousarray(cube_bits, dtype=np.uint16)
+    out = np.empty((cube_texels(prefiltered_width, roughness_count), 4), np.uint16)
+    getattr(lib(), f"oracle_prefiltered_radiance_{precision}")(_ptr(c), width, prefiltered_width, roughness_count,
+                                                               _ptr(out))
+    return out
+
+
+def cube_sample(cube_bits, width, d, lod):
+    c = np.ascontiguousarray(cube_bits, dtype=np.uint16)
+    dd = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.empty(3, np.float64)
+    lib().oracle_cube_sample_f64(_ptr(c), width, cube_levels(width), _ptr(dd), float(lod), _ptr(out))
+    return out
+
+
+def cube_texel_dir(face, x, y, w):
+    out = np.empty(3, np.float64)
+    lib().oracle_cube_texel_dir_f64(face, x, y, w, _ptr(out))
+    return out
+
+
+def cube_face_st(d):
+    dd = np.ascontiguousarray(d, dtype=np.float64)
+    f, s, t = C.c_int32(), C.c_double(), C.c_double()
+    lib().oracle_cube_face_st_f64(_ptr(dd), C.byref(f), C.byref(s), C.byref(t))
+    return f.value, s.value, t.value

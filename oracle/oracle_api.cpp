@@ -9,6 +9,7 @@
 
 #include "oracle_cloud.h"
 #include "oracle_noise.h"
+#include "oracle_ibl.h"
 #include "oracle_pathtrace.h"
 
 using namespace oracle;
@@ -472,6 +473,42 @@ void oracle_brick_majorants(const uint16_t* grid, int dx, int dy, int dz, float*
     for (int z = 0; z < bz; ++z)
         for (int y = 0; y < by; ++y)
             for (int x = 0; x < bx; ++x) out[(size_t(z) * by + y) * bx + x] = BrickMajorant(grid, dim, x, y, z);
+}
+// Environment-luminance cube (all levels) and IBL precomputation; cubes are uint16 RGBA.
+void oracle_environment_cube_f64(const SkyhipAtmosphereBufferData* atm, const SkyhipAtmosphereRenderBufferData* render,
+                                 const float* T, int tw, int th, const float* sv_l, const float* sv_t, int sw, int sh,
+                                 int w, uint16_t* out) {
+    Atmosphere<double> A(*atm);
+    RenderBlock<double> R(*render);
+    EnvironmentCube<double>(A, R, T, tw, th, sv_l, sv_t, sw, sh, w, out);
+}
+void oracle_environment_cube_f32(const SkyhipAtmosphereBufferData* atm, const SkyhipAtmosphereRenderBufferData* render,
+                                 const float* T, int tw, int th, const float* sv_l, const float* sv_t, int sw, int sh,
+                                 int w, uint16_t* out) {
+    Atmosphere<float> A(*atm);
+    RenderBlock<float> R(*render);
+    EnvironmentCube<float>(A, R, T, tw, th, sv_l, sv_t, sw, sh, w, out);
+}
+void oracle_cube_mips(uint16_t* cube, int w) {
+    for (int l = 1; l <= Log2Int(w); ++l) CubeMipLevel(cube, w, l);
+}
+void oracle_env_radiance_sh_f64(const uint16_t* cube, int w, double* out) { EnvRadianceSH<double>(cube, w, out); }
+void oracle_prefiltered_radiance_f64(const uint16_t* env, int ew, int pw, int count, uint16_t* out) {
+    PrefilteredRadiance<double>(env, ew, pw, count, out);
+}
+void oracle_prefiltered_radiance_f32(const uint16_t* env, int ew, int pw, int count, uint16_t* out) {
+    PrefilteredRadiance<float>(env, ew, pw, count, out);
+}
+void oracle_cube_sample_f64(const uint16_t* cube, int w, int levels, const double* d, double lod, double* out) {
+    V3<double> c = CubeSampleLod<double>(cube, w, levels, {d[0], d[1], d[2]}, lod);
+    out[0] = c.x, out[1] = c.y, out[2] = c.z;
+}
+void oracle_cube_texel_dir_f64(int face, int x, int y, int w, double* out) {
+    V3<double> d = CubeTexelDir<double>(face, x, y, w);
+    out[0] = d.x, out[1] = d.y, out[2] = d.z;
+}
+void oracle_cube_face_st_f64(const double* d, int* face, double* s, double* t) {
+    CubeFaceST<double>({d[0], d[1], d[2]}, *face, *s, *t);
 }
 uint16_t oracle_float_to_half(float x) { return float_to_half_rne(x); }
 float oracle_half_to_float(uint16_t h) { return half_to_float(h); }

@@ -94,6 +94,27 @@ inline float BrickMajorant(const uint16_t* g, const int dim[3], int bx, int by, 
     return m;
 }
 
+// The environment-luminance texel along world direction d (AtmosphereRenderer.glsl:253-273):
+// the sky-view luminance, plus the lit ground times the sky-view transmittance where d hits
+// the ground.
+template <class S>
+V3<S> EnvironmentLuminance(const Atmosphere<S>& A, const RenderBlock<S>& rb, const float* T, int tw, int th,
+                           const float* sv_l, const float* sv_t, int sw, int sh, V3<S> d) {
+    S cos_lat, cos_lon;
+    GetCosLatLonFromViewDirection(rb, d, cos_lat, cos_lon);
+    S r = rb.camera_earth_center_distance;
+    S u, v;
+    GetSkyViewTextureUvFromCosLatLon(A, sw, sh, r, cos_lat, cos_lon, u, v);
+    V3<S> lum = tex2d_linear_clamp<S>(sv_l, sw, sh, u, v);
+    V3<S> tr = tex2d_linear_clamp<S>(sv_t, sw, sh, u, v);
+    if (A.RayIntersectsGround(r, cos_lat)) {
+        S dist = A.DistanceToBottomAtmosphereBoundary(r, cos_lat);
+        V3<S> ground = rb.camera_position + d * dist;
+        lum += A.ComputeGroundLuminance(false, T, tw, th, rb.earth_center, ground, rb.sun_direction) * tr;
+    }
+    return lum;
+}
+
 // Trilinear CLAMP_TO_BORDER(0) sample of the FP16 grid at normalised (u, v, w).
 template <class S>
 S SampleGridF16(const uint16_t* g, const int dim[3], S u, S v, S w) {
@@ -400,19 +421,7 @@ struct PathTracer {
         V3<S> d = {S(m[0]) * local_dir.x + S(m[3]) * local_dir.y + S(m[6]) * local_dir.z,
                    S(m[1]) * local_dir.x + S(m[4]) * local_dir.y + S(m[7]) * local_dir.z,
                    S(m[2]) * local_dir.x + S(m[5]) * local_dir.y + S(m[8]) * local_dir.z};
-        S cos_lat, cos_lon;
-        GetCosLatLonFromViewDirection(rb, d, cos_lat, cos_lon);
-        S r = rb.camera_earth_center_distance;
-        S u, v;
-        GetSkyViewTextureUvFromCosLatLon(A, sw, sh, r, cos_lat, cos_lon, u, v);
-        V3<S> lum = tex2d_linear_clamp<S>(sv_l, sw, sh, u, v);
-        V3<S> tr = tex2d_linear_clamp<S>(sv_t, sw, sh, u, v);
-        if (A.RayIntersectsGround(r, cos_lat)) {
-            S dist = A.DistanceToBottomAtmosphereBoundary(r, cos_lat);
-            V3<S> ground = rb.camera_position + d * dist;
-            lum += A.ComputeGroundLuminance(false, T, tw, th, rb.earth_center, ground, rb.sun_direction) * tr;
-        }
-        return lum;
+        return EnvironmentLuminance(A, rb, T, tw, th, sv_l, sv_t, sw, sh, d);
     }
 
     // Trace (:160-249); returns L and writes has_scattered / scattered_t.

@@ -303,7 +303,7 @@ def declared_symbols():
     import re
     hdr = os.path.join(os.path.dirname(_HERE), "include", "skyhip.h")
     text = open(hdr).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void\*|const char\*)\s+(skyhip_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void\*|const char\*|size_t|long long)\s+(skyhip_\w+)\s*\(", text, re.M)))
 
 
 def register(name, restype, argtypes):

@@ -185,8 +185,9 @@ class Atmosphere:
 
 class AtmosphereRenderer:
     """AtmosphereRenderer's LUT passes (AtmosphereRenderer.cpp:164-233): sky view and
-    aerial perspective.  The environment cube, IBL and the fullscreen composite are
-    consumers of these LUTs and are not part of this library."""
+    aerial perspective.  The environment cube and IBL that consume the sky-view LUT are
+    ibl.EnvironmentLuminance and ibl.IBL; the fullscreen composite is not part of this
+    library."""
 
     def __init__(self, ctx: Context, init: AtmosphereRenderInitParameters | None = None,
                  blue_noise: DeviceBuffer | None = None, blue_noise_size: int = 64):

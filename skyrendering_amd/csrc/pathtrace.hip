@@ -14,6 +14,7 @@
 // and inside occupied bricks the tentative-collision rate follows the local
 // maximum instead of the global one.
 #include "cloud_device.h"
+#include "sky_env.h"
 
 namespace skyhip {
 
@@ -42,8 +43,7 @@ struct PtArgs {
     float gf, gb, ratio, sigma_t_max, inv_sigma_t_max, albedo;
     F3 model0, model1, model2;  // kModelMatrix3 columns
     // Atmosphere renderer frame for the environment term
-    F3 a_up, a_front, a_sun, a_cam, a_ec;
-    float a_r, a_alt0, horizon_up, horizon_down;
+    EnvFrame sky;
     // LUTs
     const float4* T;
     int tw, th;
@@ -284,33 +284,8 @@ __device__ inline F3 sun_illuminance(const PtArgs& a, F3 p) {
 // Environment term for a local-frame direction: the environment-luminance cube texel
 // (AtmosphereRenderer.glsl:253-273) at kModelMatrix3 * d, from the sky-view LUT directly.
 __device__ inline F3 environment(const PtArgs& a, F3 dl) {
-    F3 d = a.model0 * dl.x + a.model1 * dl.y + a.model2 * dl.z;
-    // GetCosLatLonFromViewDirection (:134-145)
-    float cos_lat = dot(a.a_up, d);
-    F3 lon = d - a.a_up * cos_lat;
-    float l2 = dot(lon, lon);
-    float cos_lon = l2 == 0.0f ? 1.0f : dot(lon * rsqrtf(l2), a.a_front);
-    // GetSkyViewTextureUvFromCosLatLon (:113-132), texel coordinate x (N - 1)
-    float lat = acosf(fminf(fmaxf(cos_lat, -1.0f), 1.0f));
-    float x_lat;
-    if (lat < a.horizon_up) {
-        x_lat = 0.5f - 0.5f * sqrtf(fmaxf(1.0f - lat / a.horizon_up, 0.0f));
-    } else {
-        x_lat = sqrtf(fmaxf((lat - a.horizon_up) / a.horizon_down, 0.0f)) * 0.5f + 0.5f;
-    }
-    float x_lon = sqrtf(fmaxf(0.5f - 0.5f * cos_lon, 0.0f));
-    F3 lum = lut_bilinear(a.sv_l, a.sw, a.sh, x_lon * float(a.sw - 1), x_lat * float(a.sh - 1));
-    if (ray_hits_ground(a.A, a.a_r, a.a_alt0, cos_lat)) {
-        F3 tr = lut_bilinear(a.sv_t, a.sw, a.sh, x_lon * float(a.sw - 1), x_lat * float(a.sh - 1));
-        float dist = distance_to_bottom(a.A, a.a_r, a.a_alt0, cos_lat);
-        // ComputeGroundLuminance (Atmosphere.glsl:297-306) at camera + d dist
-        F3 n = (a.a_cam - a.a_ec) + d * dist;
-        n = n * (1.0f / length3(n));
-        float mu_s = dot(a.a_sun, n);
-        F3 vis = sun_visibility(a.A, a.T, a.tw, a.th, a.A.rb, 0.0f, 0.0f, mu_s) * a.A.solar;
-        lum += (1.0f / kPiF) * clamp01(mu_s) * a.A.albedo * vis * tr;
-    }
-    return lum;
+    const F3 d = a.model0 * dl.x + a.model1 * dl.y + a.model2 * dl.z;
+    return env_luminance(a.A, a.sky, a.T, a.tw, a.th, a.sv_l, a.sv_t, a.sw, a.sh, d);
 }
 
 __device__ __forceinline__ float pt_phase(const PtArgs& a, float c) {  // GetPhase (:93-96)
@@ -626,22 +601,7 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
     a.model0 = {params->model3[0], params->model3[1], params->model3[2]};
     a.model1 = {params->model3[3], params->model3[4], params->model3[5]};
     a.model2 = {params->model3[6], params->model3[7], params->model3[8]};
-    a.a_up = f3(render->up_direction);
-    a.a_front = f3(render->front_direction);
-    a.a_sun = f3(render->sun_direction);
-    a.a_cam = f3(render->camera_position);
-    a.a_ec = f3(render->earth_center);
-    {
-        // Camera radius and horizon angles of the sky-view parameterisation
-        // (GetHorizonDownAngleFromR, AtmosphereRenderer.glsl:74-78), in FP64.
-        double r = render->camera_earth_center_distance, rb = atmosphere->bottom_radius;
-        double alt0 = r - rb;
-        a.a_r = float(r);
-        a.a_alt0 = float(alt0);
-        double down = std::acos(std::sqrt(std::fmax(alt0 * (r + rb), 0.0)) / r);
-        a.horizon_down = float(down);
-        a.horizon_up = float(3.14159265358979323846 - down);
-    }
+    a.sky = make_env_frame(*render, *atmosphere);
     a.T = reinterpret_cast<const float4*>(in->transmittance);
     a.tw = in->transmittance_width;
     a.th = in->transmittance_height;
