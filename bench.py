@@ -208,6 +208,46 @@ def bench_environment(ctx, reps=20):
     return out
 
 
+def bench_cloud_shadow(ctx, scene, reps=20):
+    """VolumetricCloud::RenderShadow for the raymarch workload: the 512^2 shadow map (Material1,
+    6-12 steps per texel, temporal blend), the two blur passes and the shadow froxel
+    (1920/12 x 1080/12 x 128), per frame, timed with HIP events; the oracle's FP32 path beside it."""
+    from skyrendering_amd.cloud import CloudShadow, cloud_shadow_matrices
+    sun = np.array(scene.renderer.buffer.sun_direction[:], np.float32)
+    cloud_shadow_matrices(scene.cloud, scene.camera, scene.earth.bottom_radius, sun, scene.common)
+    sh = CloudShadow(ctx, (scene.width, scene.height))
+    bn_size = scene.blue_noise_host.shape[1]
+    for _ in range(2):
+        sh.render(scene.common, scene.mat_block, scene.blue_noise, bn_size)
+    ctx.synchronize()
+    ctx.record(14)
+    for _ in range(reps):
+        sh.render(scene.common, scene.mat_block, scene.blue_noise, bn_size)
+    ctx.record(15)
+    ms = ctx.elapsed_ms(14, 15) / reps
+    mw, mh = sh.map_size
+    fw, fh, fd = sh.froxel_size
+    out = {"workload": f"cloud shadow map {mw}x{mh} (Material1, temporal), 2 blur passes, shadow froxel "
+                       f"{fw}x{fh}x{fd}", "ms_per_frame": round(ms, 4),
+           "bound": "latency (4 launches; the map's 262k threads march 6-12 material samples each)"}
+    import oracle
+    import ctypes as C
+    from skyrendering_amd import _abi
+    cm, det, disp = scene.textures.download()
+    keep = [np.ascontiguousarray(x) for x in (cm, det, disp)]
+    m = _abi.CloudMaterial()
+    C.memmove(C.byref(m), C.byref(scene.mat_block), C.sizeof(m))
+    m.cloud_map, m.detail, m.displacement = (x.ctypes.data for x in keep)
+    t0 = time.perf_counter()
+    raw = oracle.cloud_shadow_map(scene.common, m, scene.blue_noise_host, mw, mh, precision="f32").astype(np.float32)
+    blurred = oracle.cloud_shadow_blur(oracle.cloud_shadow_blur(raw, 0), 1)
+    oracle.cloud_shadow_froxel(scene.common, blurred, fw, fh, fd, precision="f32")
+    cpu_s = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms per frame", "cores": oracle.max_threads(),
+                           "kind": "port", "sample": "one frame (no temporal blend), oracle FP32 OpenMP (blur serial)"}
+    return out
+
+
 FLOP_PER_INTEGRATOR_STEP = 150  # sky-view / MS step: 2 LUT fetches, ~7 transcendentals, RGB accumulation
 
 
@@ -482,6 +522,8 @@ def main():
         "lut_set_integrator_steps": lut_steps,
         "device": dev,
     }
+    out["cloud_shadow"] = bench_cloud_shadow(ctx, scene)
+    progress(f"cloud shadow {out['cloud_shadow']['ms_per_frame']} ms per frame")
     out["environment"] = bench_environment(ctx)
     progress(f"environment cube + IBL {out['environment']['ms_per_update']} ms per update")
     pt_scene = None

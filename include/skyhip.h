@@ -379,6 +379,22 @@ int skyhip_cloud_assign_buffer_data(const SkyhipVolumetricCloudParameters* cloud
                                     SkyhipCloudBufferData* out_cloud,
                                     float out_model[16]);
 
+/* The shadow-map matrices of VolumetricCloud::Update (VolumetricCloud.cpp:152-172, 219-237),
+ * written into `common` (uLightVP, uInvLightVP, uShadowMapReprojectMat): the light view looks
+ * from the local camera along -sun; the orthographic projection bounds the camera and its
+ * frustum's far-plane corners at shadow_map_max_distance in light space, padded by half a
+ * shadow-map texel.  uShadowMapReprojectMat = previous light VP * (previous model^-1 * model)
+ * * translate(additional_delta) * light VP^-1 (identity when previous_light_vp is NULL;
+ * previous_model goes with it; additional_delta may be NULL).  out_light_vp and
+ * out_light_vp_inv_model (light VP * model^-1, the atmosphere renderer's uCloudShadowMapMat,
+ * :277) may be NULL. */
+int skyhip_cloud_shadow_matrices(const SkyhipVolumetricCloudParameters* cloud, const SkyhipCamera* camera,
+                                 float bottom_radius, const float sun_direction_world[3],
+                                 int32_t shadow_map_width, int32_t shadow_map_height,
+                                 const float previous_light_vp[16], const float previous_model[16],
+                                 const float additional_delta[2], SkyhipCloudCommonBufferData* common,
+                                 float out_light_vp[16], float out_light_vp_inv_model[16]);
+
 /* gen_sample_info + CalKLod, src/SkyRendering/VolumetricCloudDefaultMaterial.cpp:24-28,88-92. */
 int skyhip_cloud_sample_info(float repeat_size, int32_t tex_max_width, float fovy_degrees,
                              float viewport_w, float viewport_h, double offset_x, double offset_y,
@@ -488,6 +504,30 @@ int skyhip_environment_luminance(SkyhipContext* ctx, const SkyhipAtmosphereBuffe
 int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment_cube_rgba16f, int32_t environment_width,
                           float* out_sh9_rgba, void* out_prefiltered_rgba16f, int32_t prefiltered_width,
                           int32_t roughness_count);
+
+/* ------------------------------------------------------------------------- */
+/* Cloud shadow map and shadow froxel                                        */
+/* ------------------------------------------------------------------------- */
+
+/* VolumetricCloud::RenderShadow's shadow-map passes (VolumetricCloud.cpp:282-316):
+ * VolumetricCloudShadowMap.comp into out_raw (RG32F width x height: mid-shell depth along
+ * -sun, transmittance), blended 1:4 into previous_raw reprojected by uShadowMapReprojectMat
+ * when previous_raw is not NULL (the reference keeps the previous frame's raw map), then the
+ * 9-tap Gaussian of VolumetricCloudShadowMapBlur.comp along x into scratch and along y into
+ * out_blurred.  The march samples the material's SampleSigmaT; blue_noise: the 64 x 64 (or
+ * larger) UNORM16 tile, read at (x & 63, y & 63).  The reference's map is 512 x 512. */
+int skyhip_cloud_shadow_map(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                            const SkyhipCloudMaterial* material, const uint16_t* blue_noise, int32_t blue_noise_size,
+                            const float* previous_raw_rg32f, float* out_raw_rg32f, float* scratch_rg32f,
+                            float* out_blurred_rg32f, int32_t width, int32_t height);
+
+/* VolumetricCloudShadowFroxel.comp (VolumetricCloud.cpp:317-324): the ray-scatter visibility
+ * froxel, R16 UNORM [depth][height][width] (the reference: viewport / 12 x 128 slices over
+ * uShadowFroxelMaxDistance), the running mean along each view ray of
+ * SampleCloudShadowTransmittance of the blurred map (LINEAR, CLAMP_TO_BORDER (1e10, 1)). */
+int skyhip_cloud_shadow_froxel(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                               const float* shadow_map_rg32f, int32_t map_width, int32_t map_height,
+                               uint16_t* out_froxel_r16, int32_t width, int32_t height, int32_t depth);
 
 /* ------------------------------------------------------------------------- */
 /* Cloud textures: noise generators and mip chains                           */

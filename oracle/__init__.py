@@ -471,3 +471,40 @@ def cube_face_st(d):
     f, s, t = C.c_int32(), C.c_double(), C.c_double()
     lib().oracle_cube_face_st_f64(_ptr(dd), C.byref(f), C.byref(s), C.byref(t))
     return f.value, s.value, t.value
+
+
+# ---------------------------------------------------------------------------
+# Cloud shadow map, blur and froxel
+# ---------------------------------------------------------------------------
+register("oracle_cloud_shadow_map_f64", None, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial), vp, i32, vp, i32,
+                                               i32, vp])
+register("oracle_cloud_shadow_map_f32", None, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial), vp, i32, vp, i32,
+                                               i32, vp])
+register("oracle_cloud_shadow_blur", None, [vp, vp, i32, i32, i32])
+register("oracle_cloud_shadow_froxel_f64", None, [P(_abi.CloudCommonBufferData), vp, i32, i32, i32, i32, i32, vp])
+register("oracle_cloud_shadow_froxel_f32", None, [P(_abi.CloudCommonBufferData), vp, i32, i32, i32, i32, i32, vp])
+
+
+def cloud_shadow_map(common, material, blue_noise, w, h, previous=None, precision="f64"):
+    """[h, w, 2] (mid-shell depth, transmittance); previous: the previous raw map [h, w, 2] float32."""
+    bn = np.ascontiguousarray(blue_noise, dtype=np.uint16)
+    prev = None if previous is None else np.ascontiguousarray(previous, dtype=np.float32)
+    out = np.empty((h, w, 2), _dt(precision))
+    getattr(lib(), f"oracle_cloud_shadow_map_{precision}")(C.byref(common), C.byref(material), _ptr(bn), bn.shape[1],
+                                                           _ptr(prev), w, h, _ptr(out))
+    return out
+
+
+def cloud_shadow_blur(rg, axis):
+    a = np.ascontiguousarray(rg, dtype=np.float32)
+    out = np.empty_like(a)
+    lib().oracle_cloud_shadow_blur(_ptr(a), _ptr(out), a.shape[1], a.shape[0], axis)
+    return out
+
+
+def cloud_shadow_froxel(common, shadow_map, w, h, d, precision="f64"):
+    m = np.ascontiguousarray(shadow_map, dtype=np.float32)
+    out = np.empty((d, h, w), np.uint16)
+    getattr(lib(), f"oracle_cloud_shadow_froxel_{precision}")(C.byref(common), _ptr(m), m.shape[1], m.shape[0], w, h,
+                                                              d, _ptr(out))
+    return out

@@ -9,6 +9,7 @@
 
 #include "oracle_cloud.h"
 #include "oracle_noise.h"
+#include "oracle_cloud_shadow.h"
 #include "oracle_ibl.h"
 #include "oracle_pathtrace.h"
 
@@ -509,6 +510,24 @@ void oracle_cube_texel_dir_f64(int face, int x, int y, int w, double* out) {
 }
 void oracle_cube_face_st_f64(const double* d, int* face, double* s, double* t) {
     CubeFaceST<double>({d[0], d[1], d[2]}, *face, *s, *t);
+}
+// Cloud shadow passes: maps are RG interleaved (depth, transmittance).
+void oracle_cloud_shadow_map_f64(const SkyhipCloudCommonBufferData* c, const SkyhipCloudMaterial* m, const uint16_t* bn,
+                                 int bn_size, const float* previous, int w, int h, double* out) {
+    CloudShadowMap<double>(*c, *m, bn, bn_size, previous, w, h, out);
+}
+void oracle_cloud_shadow_map_f32(const SkyhipCloudCommonBufferData* c, const SkyhipCloudMaterial* m, const uint16_t* bn,
+                                 int bn_size, const float* previous, int w, int h, float* out) {
+    CloudShadowMap<float>(*c, *m, bn, bn_size, previous, w, h, out);
+}
+void oracle_cloud_shadow_blur(const float* in, float* out, int w, int h, int axis) { CloudShadowBlur(in, out, w, h, axis); }
+void oracle_cloud_shadow_froxel_f64(const SkyhipCloudCommonBufferData* c, const float* map, int mw, int mh, int w, int h,
+                                    int d, uint16_t* out) {
+    CloudShadowFroxel<double>(*c, map, mw, mh, w, h, d, out);
+}
+void oracle_cloud_shadow_froxel_f32(const SkyhipCloudCommonBufferData* c, const float* map, int mw, int mh, int w, int h,
+                                    int d, uint16_t* out) {
+    CloudShadowFroxel<float>(*c, map, mw, mh, w, h, d, out);
 }
 uint16_t oracle_float_to_half(float x) { return float_to_half_rne(x); }
 float oracle_half_to_float(uint16_t h) { return half_to_float(h); }

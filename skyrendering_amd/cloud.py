@@ -278,6 +278,75 @@ class VolumetricCloud:
         return common, cloud, np.array(model[:], dtype=np.float32)
 
 
+_abi.register("skyhip_cloud_shadow_matrices", C.c_int, [
+    C.POINTER(_abi.VolumetricCloudParametersC), C.POINTER(_abi.CameraC), C.c_float, C.c_void_p, C.c_int32, C.c_int32,
+    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_void_p])
+_abi.register("skyhip_cloud_shadow_map", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.POINTER(_abi.CloudMaterial), C.c_void_p, C.c_int32,
+    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32])
+_abi.register("skyhip_cloud_shadow_froxel", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+    C.c_int32, C.c_int32])
+
+SHADOW_MAP_RESOLUTION = (512, 512)  # kShadowMapResolution, VolumetricCloud.cpp:52
+SHADOW_FROXEL_DEPTH = 128           # VolumetricCloud.cpp:135
+
+
+def cloud_shadow_matrices(cloud: "VolumetricCloud", camera: Camera, bottom_radius: float, sun_direction_world,
+                          common, previous_light_vp=None, previous_model=None, additional_delta=None,
+                          shadow_map_size=SHADOW_MAP_RESOLUTION):
+    """Fills common's light matrices as VolumetricCloud::Update does; returns (light VP,
+    light VP * model^-1) as float32 [16] column-major arrays."""
+    def arr(x, n):
+        return None if x is None else (C.c_float * n)(*[float(v) for v in np.asarray(x, np.float32).ravel()])
+    out_vp, out_vpm = (C.c_float * 16)(), (C.c_float * 16)()
+    _abi.check(_abi.lib().skyhip_cloud_shadow_matrices(
+        C.byref(cloud.to_c()), C.byref(camera.to_c()), float(bottom_radius), arr(sun_direction_world, 3),
+        shadow_map_size[0], shadow_map_size[1], arr(previous_light_vp, 16), arr(previous_model, 16),
+        arr(additional_delta, 2), C.byref(common), out_vp, out_vpm))
+    return np.array(out_vp[:], np.float32), np.array(out_vpm[:], np.float32)
+
+
+class CloudShadow:
+    """VolumetricCloud::RenderShadow (VolumetricCloud.cpp:282-325): the raw shadow maps of this
+    and the previous frame, the blurred map and the shadow froxel for a viewport."""
+
+    def __init__(self, ctx: Context, viewport, map_size=SHADOW_MAP_RESOLUTION, froxel_depth=SHADOW_FROXEL_DEPTH):
+        self.ctx = ctx
+        self.map_size = tuple(map_size)
+        mw, mh = self.map_size
+        self.raw = [ctx.alloc(mw * mh * 8), ctx.alloc(mw * mh * 8)]  # this frame's, the previous frame's
+        self.scratch = ctx.alloc(mw * mh * 8)
+        self.blurred = ctx.alloc(mw * mh * 8)
+        self.froxel_size = (viewport[0] // 12, viewport[1] // 12, froxel_depth)
+        fw, fh, fd = self.froxel_size
+        self.froxel = ctx.alloc(fw * fh * fd * 2)
+        self.frames = 0
+
+    def render(self, common, material: _abi.CloudMaterial, blue_noise: DeviceBuffer, blue_noise_size: int):
+        """One frame: the map (blended with the previous frame's after the first), blur, froxel."""
+        self.raw.reverse()
+        mw, mh = self.map_size
+        previous = self.raw[1].ptr if self.frames > 0 else None
+        _abi.check(_abi.lib().skyhip_cloud_shadow_map(
+            self.ctx.handle, C.byref(common), C.byref(material), blue_noise.ptr, blue_noise_size, previous,
+            self.raw[0].ptr, self.scratch.ptr, self.blurred.ptr, mw, mh))
+        fw, fh, fd = self.froxel_size
+        _abi.check(_abi.lib().skyhip_cloud_shadow_froxel(
+            self.ctx.handle, C.byref(common), self.blurred.ptr, mw, mh, self.froxel.ptr, fw, fh, fd))
+        self.frames += 1
+
+    RenderShadow = render
+
+    def maps(self):
+        mw, mh = self.map_size
+        return (self.raw[0].download(np.float32, (mh, mw, 2)), self.blurred.download(np.float32, (mh, mw, 2)))
+
+    def froxel_volume(self):
+        fw, fh, fd = self.froxel_size
+        return self.froxel.download(np.uint16, (fd, fh, fw))
+
+
 def make_material(material, textures: CloudTextures | None, viewport, camera: Camera, offset=(0.0, 0.0)):
     m = _abi.CloudMaterial()
     m.kind = material.kind

@@ -149,6 +149,17 @@ static void look_at(V3f eye, V3f center, V3f up, float* m) {
 }
 
 // glm::perspective (right-handed, clip z in [-1, 1])
+// glm::ortho(left, right, bottom, top, z_near, z_far): right-handed, clip z in [-1, 1].
+static void ortho(float l, float r, float b, float t, float n, float f, float* m) {
+    mat_identity(m);
+    m[0] = 2.0f / (r - l);
+    m[5] = 2.0f / (t - b);
+    m[10] = -2.0f / (f - n);
+    m[12] = -(r + l) / (r - l);
+    m[13] = -(t + b) / (t - b);
+    m[14] = -(f + n) / (f - n);
+}
+
 static void perspective(float fovy, float aspect, float z_near, float z_far, float* m) {
     float t = std::tan(fovy / 2.0f);
     std::memset(m, 0, 16 * sizeof(float));
@@ -474,6 +485,99 @@ extern "C" int skyhip_cloud_assign_buffer_data(const SkyhipVolumetricCloudParame
     b.uEnvBottomVisibility = cloud->env_bottom_visibility;
     b.uEnvSunHeightCurveExp = cloud->env_sun_height_curve_exp;
     if (out_model) std::memcpy(out_model, model, sizeof(model));
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_cloud_shadow_matrices(const SkyhipVolumetricCloudParameters* cloud, const SkyhipCamera* camera,
+                                            float bottom_radius, const float sun_direction_world[3],
+                                            int32_t shadow_map_width, int32_t shadow_map_height,
+                                            const float previous_light_vp[16], const float previous_model[16],
+                                            const float additional_delta[2], SkyhipCloudCommonBufferData* common,
+                                            float out_light_vp[16], float out_light_vp_inv_model[16]) {
+    if (!cloud || !camera || !sun_direction_world || !common) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if ((previous_light_vp == nullptr) != (previous_model == nullptr))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "previous_light_vp and previous_model go together");
+    if (shadow_map_width < 1 || shadow_map_height < 1) return fail(SKYHIP_ERR_INVALID_SIZE, "shadow map size");
+    if (!finite_block(sun_direction_world, 3) || !std::isfinite(bottom_radius) ||
+        !std::isfinite(cloud->shadow_map_max_distance) || !(cloud->shadow_map_max_distance > 0.0f))
+        return fail(SKYHIP_ERR_NON_FINITE, "non-finite sun direction, radius or shadow distance");
+    float vp[16];
+    V3f front, right, up_c;
+    {
+        float f3_[3], r3_[3], u3_[3];
+        SKYHIP_CHECK(skyhip_camera_view_projection(camera, vp, f3_, r3_, u3_));
+        front = load3(f3_);
+        right = load3(r3_);
+        up_c = load3(u3_);
+    }
+    // The local frame of VolumetricCloud::Update (VolumetricCloud.cpp:188-199), as in
+    // skyhip_cloud_assign_buffer_data.
+    const V3f earth_center = {0.0f, -bottom_radius, 0.0f};
+    const V3f camera_pos = load3(camera->position);
+    const V3f up = normalize3(camera_pos - earth_center);
+    const V3f origin = earth_center + up * bottom_radius;
+    const V3f fr = normalize3(cross3(up, V3f{1.0f, 0.0f, 0.0f}));
+    const V3f ri = normalize3(cross3(up, fr));
+    const float model[16] = {fr.x, fr.y, fr.z, 0.0f, ri.x, ri.y, ri.z, 0.0f,
+                             up.x, up.y, up.z, 0.0f, origin.x, origin.y, origin.z, 1.0f};
+    float inv_model[16];
+    if (!mat_inverse(model, inv_model)) return fail(SKYHIP_ERR_NON_FINITE, "degenerate local cloud frame");
+    auto to_local = [&](V3f p, float w) {
+        return V3f{inv_model[0] * p.x + inv_model[4] * p.y + inv_model[8] * p.z + inv_model[12] * w,
+                   inv_model[1] * p.x + inv_model[5] * p.y + inv_model[9] * p.z + inv_model[13] * w,
+                   inv_model[2] * p.x + inv_model[6] * p.y + inv_model[10] * p.z + inv_model[14] * w};
+    };
+    const V3f local_cam = to_local(camera_pos, 1.0f);
+    const V3f local_sun = normalize3(to_local(load3(sun_direction_world), 0.0f));
+    // :221-223: the light view looks from the camera along -sun.
+    const V3f light_up = (local_sun.x == 0.0f && local_sun.y == 0.0f) ? V3f{1.0f, 0.0f, 0.0f} : V3f{0.0f, 0.0f, 1.0f};
+    float light_view[16];
+    look_at(local_cam, local_cam - local_sun, light_up, light_view);
+    // GetLightProjection (:152-172): the light-space bounds of the camera and the far-plane
+    // corners at shadow_map_max_distance, padded by half a texel, as an orthographic box.
+    const float dmax = cloud->shadow_map_max_distance;
+    const float tan_half = std::tan(radians(camera->fovy) * 0.5f);
+    const V3f far_center = camera_pos + front * dmax;
+    const V3f u = up_c * (tan_half * dmax), rr = right * (camera->aspect * tan_half * dmax);
+    const V3f corners[5] = {camera_pos, far_center + u + rr, far_center + u - rr, far_center - u + rr,
+                            far_center - u - rr};
+    float min_xy[2] = {1e10f, 1e10f}, max_xy[2] = {-1e10f, -1e10f};
+    for (const V3f& cw : corners) {
+        const V3f l = to_local(cw, 1.0f);
+        const float lx = light_view[0] * l.x + light_view[4] * l.y + light_view[8] * l.z + light_view[12];
+        const float ly = light_view[1] * l.x + light_view[5] * l.y + light_view[9] * l.z + light_view[13];
+        min_xy[0] = std::min(min_xy[0], lx);
+        max_xy[0] = std::max(max_xy[0], lx);
+        min_xy[1] = std::min(min_xy[1], ly);
+        max_xy[1] = std::max(max_xy[1], ly);
+    }
+    const float pad_x = 0.5f * (max_xy[0] - min_xy[0]) / float(shadow_map_width);
+    const float pad_y = 0.5f * (max_xy[1] - min_xy[1]) / float(shadow_map_height);
+    float light_proj[16], light_vp[16], inv_light_vp[16];
+    ortho(min_xy[0] - pad_x, max_xy[0] + pad_x, min_xy[1] - pad_y, max_xy[1] + pad_y, -1.0f, 1.0f, light_proj);
+    mat_mul(light_proj, light_view, light_vp);
+    if (!mat_inverse(light_vp, inv_light_vp)) return fail(SKYHIP_ERR_NON_FINITE, "light view-projection is singular");
+    std::memcpy(common->uLightVP, light_vp, sizeof(light_vp));
+    std::memcpy(common->uInvLightVP, inv_light_vp, sizeof(inv_light_vp));
+    // :235: previous light VP * (previous model^-1 * model) * additional-delta translation * light VP^-1;
+    // identity without a previous frame.
+    if (previous_light_vp) {
+        float inv_prev_model[16], rel[16], delta[16], t0[16], t1[16];
+        if (!mat_inverse(previous_model, inv_prev_model)) return fail(SKYHIP_ERR_NON_FINITE, "previous model singular");
+        mat_mul(inv_prev_model, model, rel);
+        mat_identity(delta);
+        if (additional_delta) {
+            delta[12] = additional_delta[0];
+            delta[13] = additional_delta[1];
+        }
+        mat_mul(previous_light_vp, rel, t0);
+        mat_mul(t0, delta, t1);
+        mat_mul(t1, inv_light_vp, common->uShadowMapReprojectMat);
+    } else {
+        mat_identity(common->uShadowMapReprojectMat);
+    }
+    if (out_light_vp) std::memcpy(out_light_vp, light_vp, sizeof(light_vp));
+    if (out_light_vp_inv_model) mat_mul(light_vp, inv_model, out_light_vp_inv_model);  // :277
     return SKYHIP_OK;
 }
 
