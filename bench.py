@@ -248,6 +248,84 @@ def bench_cloud_shadow(ctx, scene, reps=20):
     return out
 
 
+# Algorithmic HBM bytes of the checkerboard / index / reconstruction / upscale passes per
+# full-resolution pixel: checkerboard reads the R32F depth and writes a quarter R32F (4 + 1);
+# the index reads the checkerboard once and writes RG32F per tile (1 + 0.5); the reconstruction,
+# per quarter of a pixel, reads the checkerboard (4), index / render / distance shared by 4 half-
+# resolution pixels (2 + 2 + 1), the RGBA16F history (8) and writes RGBA16F (8); the upscale reads
+# depth (4), checkerboard and reconstruction shared by 4 (1 + 2) and reads + writes the RGBA16F HDR
+# pixel (16).
+RECON_BYTES_PER_PIXEL = (4 + 1) + (1 + 0.5) + (4 + 2 + 2 + 1 + 8 + 8) / 4 + (4 + 1 + 2 + 16)
+
+
+def bench_cloud_reconstruct(ctx, scene, reps=50):
+    """VolumetricCloud::Render's passes around the quarter-resolution raymarch at the raymarch
+    workload's viewport: checkerboard depth, shading index, temporal reconstruction and the
+    upscale composite (timed together, HIP events), and the whole frame including the
+    quarter-resolution raymarch of the indexed texels; the oracle's FP32 passes beside it."""
+    from skyrendering_amd import presets
+    from skyrendering_amd.cloud import CloudReconstruction, cloud_reproject_matrix
+    w, h = scene.width, scene.height
+    c = scene.common
+    cloud_reproject_matrix(scene.camera, scene.earth.bottom_radius, c)
+    depth = presets.synthetic_depth_buffer(c.uLinearDepthParam, w, h)
+    d_depth = ctx.upload(depth)
+    hdr0 = np.concatenate([np.full((h, w, 3), 2.0, np.float32), np.ones((h, w, 1), np.float32)], -1)
+    d_hdr = ctx.upload(hdr0.astype(np.float16).view(np.uint16))
+    rec = CloudReconstruction(ctx, (w, h))
+    a, r = scene.atmosphere, scene.renderer
+    aw, ah = r.init.aerial_perspective_lut_size
+
+    def frame():
+        rec.render(a.buffer, c, scene.cloud_block, scene.mat_block, a.transmittance_texture, a.transmittance_size,
+                   r.aerial_perspective_luminance_texture, r.aerial_perspective_transmittance_texture,
+                   (aw, ah, r.init.aerial_perspective_lut_depth), scene.blue_noise, scene.blue_noise_host.shape[0],
+                   d_depth, d_hdr)
+
+    def passes():
+        rec.checkerboard_depth(d_depth)
+        rec.shading_index(c)
+        rec.reconstruct_pass(c)
+        rec.upscale(c, d_depth, d_hdr)
+        rec.reconstruct.reverse()
+
+    for _ in range(3):
+        frame()
+    ctx.synchronize()
+    ctx.record(16)
+    for _ in range(reps):
+        passes()
+    ctx.record(17)
+    for _ in range(reps):
+        frame()
+    ctx.record(18)
+    ctx.synchronize()
+    pass_ms = ctx.elapsed_ms(16, 17) / reps
+    frame_ms = ctx.elapsed_ms(17, 18) / reps
+    alg = RECON_BYTES_PER_PIXEL * w * h
+    gbs = alg / (pass_ms * 1e-3) / 1e9
+    out = {"workload": f"cloud checkerboard {w // 2}x{h // 2}, shading index {w // 4}x{h // 4}, temporal "
+                       f"reconstruction {w // 2}x{h // 2} RGBA16F, upscale composite {w}x{h}; frame = these + the "
+                       f"quarter-resolution raymarch ({w // 4}x{h // 4}) of the indexed texels",
+           "passes_ms": round(pass_ms, 4), "frame_ms": round(frame_ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_frame": int(alg)}}
+    import oracle
+    hdr_bits = hdr0.astype(np.float16).view(np.uint16)
+    render = rec.render_texture.download(np.uint16, (h // 4, w // 4, 4))
+    dist = rec.distance.download(np.float32, (h // 4, w // 4))
+    prev = rec.reconstruct[0].download(np.uint16, (h // 2, w // 2, 4))
+    t0 = time.perf_counter()
+    cb = oracle.cloud_checkerboard_depth(depth)
+    idx = oracle.cloud_shading_index(c, cb, "f32")
+    recon = oracle.cloud_reconstruct(c, cb, idx, render, dist, prev, w, h, "f32")
+    oracle.cloud_upscale(c, cb, depth, recon, hdr_bits, "f32")
+    cpu_s = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms per frame (passes only)", "cores": 1,
+                           "kind": "port", "sample": "one frame of the four passes, oracle FP32, single thread"}
+    return out
+
+
 FLOP_PER_INTEGRATOR_STEP = 150  # sky-view / MS step: 2 LUT fetches, ~7 transcendentals, RGB accumulation
 
 
@@ -524,6 +602,9 @@ def main():
     }
     out["cloud_shadow"] = bench_cloud_shadow(ctx, scene)
     progress(f"cloud shadow {out['cloud_shadow']['ms_per_frame']} ms per frame")
+    out["cloud_reconstruct"] = bench_cloud_reconstruct(ctx, scene)
+    progress(f"cloud checkerboard/index/reconstruct/upscale {out['cloud_reconstruct']['passes_ms']} ms, "
+             f"quarter-resolution frame {out['cloud_reconstruct']['frame_ms']} ms")
     out["environment"] = bench_environment(ctx)
     progress(f"environment cube + IBL {out['environment']['ms_per_update']} ms per update")
     pt_scene = None

@@ -530,6 +530,58 @@ int skyhip_cloud_shadow_froxel(SkyhipContext* ctx, const SkyhipCloudCommonBuffer
                                uint16_t* out_froxel_r16, int32_t width, int32_t height, int32_t depth);
 
 /* ------------------------------------------------------------------------- */
+/* Cloud checkerboard, shading index, temporal reconstruction and upscale     */
+/* ------------------------------------------------------------------------- */
+
+/* uReprojectMat of VolumetricCloud::Update (VolumetricCloud.cpp:204-214, 234): previous MVP *
+ * translate(delta_local.xy), delta_local = inverse(model)'s rotation applied to camera
+ * position - previous_camera_position (world), plus additional_delta (the material's offset
+ * of this frame; may be NULL).  previous_mvp and previous_camera_position are both NULL on the
+ * first frame: uReprojectMat is then this frame's MVP.  out_mvp (may be NULL) receives this
+ * frame's camera view-projection * model, to pass as previous_mvp next frame.  Only
+ * common->uReprojectMat is written. */
+int skyhip_cloud_reproject_matrix(const SkyhipCamera* camera, float bottom_radius, const float previous_mvp[16],
+                                  const float previous_camera_position[3], const float additional_delta[2],
+                                  SkyhipCloudCommonBufferData* common, float out_mvp[16]);
+
+/* CheckerboardGen.comp (VolumetricCloud.cpp:330-341): out (R32F, width/2 x height/2) = the
+ * max of each 2x2 block of depth (R32F width x height) where the block's x and y have the
+ * same parity, its min otherwise.  width and height: multiples of 4 in [4, 16384]. */
+int skyhip_cloud_checkerboard_depth(SkyhipContext* ctx, const float* depth, int32_t width, int32_t height,
+                                    float* out_checkerboard);
+
+/* VolumetricCloudIndexGen.comp (VolumetricCloud.cpp:344-354): per 2x2 tile of the checkerboard
+ * depth, the texel the quarter-resolution raymarch shades, (index, linear depth) as RG32F
+ * (width/4 x height/4); index i is the texel ((i+1)>>1 & 1, (i+2)>>1 & 1) of the tile.  The
+ * nearest texel wins where no neighbour tile's scheduled texel is within 25% of its linear
+ * depth, then the farthest, else the rotating pattern (uBaseShadingIndex + x + y) & 3.
+ * Neighbour fetches outside the checkerboard read depth 0.  width, height: the viewport. */
+int skyhip_cloud_shading_index(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                               const float* checkerboard, int32_t width, int32_t height,
+                               float* out_index_linear_depth);
+
+/* VolumetricCloudReconstruct.comp (VolumetricCloud.cpp:386-404): the half-resolution temporal
+ * reconstruction (RGBA16F width/2 x height/2) from the quarter-resolution render (RGBA16F)
+ * and cloud distance (R32F, skyhip_cloud_raymarch with the shading index), the previous
+ * frame's reconstruction reprojected by uReprojectMat (LINEAR, CLAMP_TO_EDGE) and clamped to
+ * the colour box of the depth-similar rendered neighbours, blended 4:1 with the new sample
+ * where this pixel was shaded, replaced by it where the reprojection leaves the screen
+ * (Reinhard space).  previous_rgba16f NULL (the first frame) counts as off-screen; it must
+ * not be the output. */
+int skyhip_cloud_reconstruct(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                             const float* checkerboard, const float* index_linear_depth, const void* render_rgba16f,
+                             const float* cloud_distance, const void* previous_rgba16f, void* out_rgba16f,
+                             int32_t width, int32_t height);
+
+/* VolumetricCloudUpscale.comp (VolumetricCloud.cpp:407-417): the depth-aware 2x upscale of
+ * the reconstruction (mean of the 4 diagonal half-resolution neighbours, or the
+ * depth-nearest one across a depth edge with contrasting transmittance) composited into
+ * hdr (RGBA16F width x height, in place): rgb = rgb * (T <= 0.01 ? 0 : T) + luminance. */
+int skyhip_cloud_upscale(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common, const float* checkerboard,
+                         const float* depth, const void* reconstruct_rgba16f, void* hdr_rgba16f, int32_t width,
+                         int32_t height);
+
+/* ------------------------------------------------------------------------- */
 /* Cloud textures: noise generators and mip chains                           */
 /* ------------------------------------------------------------------------- */
 

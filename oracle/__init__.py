@@ -508,3 +508,58 @@ def cloud_shadow_froxel(common, shadow_map, w, h, d, precision="f64"):
     getattr(lib(), f"oracle_cloud_shadow_froxel_{precision}")(C.byref(common), _ptr(m), m.shape[1], m.shape[0], w, h,
                                                               d, _ptr(out))
     return out
+
+
+# Volumetric cloud checkerboard / shading index / reconstruction / upscale.
+_cc = P(_abi.CloudCommonBufferData)
+register("oracle_cloud_checkerboard_depth", None, [vp, i32, i32, vp])
+register("oracle_cloud_shading_index_f64", None, [_cc, vp, i32, i32, vp])
+register("oracle_cloud_shading_index_f32", None, [_cc, vp, i32, i32, vp])
+register("oracle_cloud_reconstruct_f64", None, [_cc, vp, vp, vp, vp, vp, i32, i32, vp])
+register("oracle_cloud_reconstruct_f32", None, [_cc, vp, vp, vp, vp, vp, i32, i32, vp])
+register("oracle_cloud_upscale_f64", None, [_cc, vp, vp, vp, vp, i32, i32, vp])
+register("oracle_cloud_upscale_f32", None, [_cc, vp, vp, vp, vp, i32, i32, vp])
+
+
+def cloud_checkerboard_depth(depth):
+    """[h/2, w/2] float32 from the [h, w] depth buffer."""
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    h, w = d.shape
+    out = np.empty((h // 2, w // 2), np.float32)
+    lib().oracle_cloud_checkerboard_depth(_ptr(d), w, h, _ptr(out))
+    return out
+
+
+def cloud_shading_index(common, checkerboard, precision="f64"):
+    """[h/4, w/4, 2] (index, linear depth) from the [h/2, w/2] checkerboard."""
+    cb = np.ascontiguousarray(checkerboard, dtype=np.float32)
+    ch, cw = cb.shape
+    out = np.empty((ch // 2, cw // 2, 2), _dt(precision))
+    getattr(lib(), f"oracle_cloud_shading_index_{precision}")(C.byref(common), _ptr(cb), cw, ch, _ptr(out))
+    return out
+
+
+def cloud_reconstruct(common, checkerboard, index, render_bits, distance, previous_bits, w, h, precision="f64"):
+    """[h/2, w/2, 4]: f64 values before the RGBA16F store, or (f32) the stored RGBA16F bits."""
+    cb = np.ascontiguousarray(checkerboard, dtype=np.float32)
+    ix = np.ascontiguousarray(index, dtype=np.float32)
+    rb = np.ascontiguousarray(render_bits, dtype=np.uint16)
+    dist = np.ascontiguousarray(distance, dtype=np.float32)
+    prev = None if previous_bits is None else np.ascontiguousarray(previous_bits, dtype=np.uint16)
+    out = np.empty((h // 2, w // 2, 4), np.float64 if precision == "f64" else np.uint16)
+    getattr(lib(), f"oracle_cloud_reconstruct_{precision}")(C.byref(common), _ptr(cb), _ptr(ix), _ptr(rb), _ptr(dist),
+                                                            _ptr(prev), w, h, _ptr(out))
+    return out
+
+
+def cloud_upscale(common, checkerboard, depth, reconstruct_bits, hdr_bits, precision="f64"):
+    """[h, w, 4]: hdr composited under the upscaled reconstruction (f64 values, or f32 RGBA16F bits)."""
+    cb = np.ascontiguousarray(checkerboard, dtype=np.float32)
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    rb = np.ascontiguousarray(reconstruct_bits, dtype=np.uint16)
+    hb = np.ascontiguousarray(hdr_bits, dtype=np.uint16)
+    h, w = d.shape
+    out = np.empty((h, w, 4), np.float64 if precision == "f64" else np.uint16)
+    getattr(lib(), f"oracle_cloud_upscale_{precision}")(C.byref(common), _ptr(cb), _ptr(d), _ptr(rb), _ptr(hb), w, h,
+                                                        _ptr(out))
+    return out

@@ -10,6 +10,7 @@
 #include "oracle_cloud.h"
 #include "oracle_noise.h"
 #include "oracle_cloud_shadow.h"
+#include "oracle_cloud_reconstruct.h"
 #include "oracle_ibl.h"
 #include "oracle_pathtrace.h"
 
@@ -528,6 +529,36 @@ void oracle_cloud_shadow_froxel_f64(const SkyhipCloudCommonBufferData* c, const 
 void oracle_cloud_shadow_froxel_f32(const SkyhipCloudCommonBufferData* c, const float* map, int mw, int mh, int w, int h,
                                     int d, uint16_t* out) {
     CloudShadowFroxel<float>(*c, map, mw, mh, w, h, d, out);
+}
+void oracle_cloud_checkerboard_depth(const float* depth, int w, int h, float* out) { CheckerboardDepth(depth, w, h, out); }
+void oracle_cloud_shading_index_f64(const SkyhipCloudCommonBufferData* c, const float* cb, int cw, int ch, double* out) {
+    ShadingIndex<double>(*c, cb, cw, ch, out);
+}
+void oracle_cloud_shading_index_f32(const SkyhipCloudCommonBufferData* c, const float* cb, int cw, int ch, float* out) {
+    ShadingIndex<float>(*c, cb, cw, ch, out);
+}
+void oracle_cloud_reconstruct_f64(const SkyhipCloudCommonBufferData* c, const float* cb, const float* index,
+                                  const uint16_t* render, const float* distance, const uint16_t* previous, int w, int h,
+                                  double* out) {
+    Reconstruct<double>(*c, cb, index, render, distance, previous, w, h, out);
+}
+// FP32, stored as RGBA16F bits.
+void oracle_cloud_reconstruct_f32(const SkyhipCloudCommonBufferData* c, const float* cb, const float* index,
+                                  const uint16_t* render, const float* distance, const uint16_t* previous, int w, int h,
+                                  uint16_t* out) {
+    std::vector<float> v(size_t(w / 2) * (h / 2) * 4);
+    Reconstruct<float>(*c, cb, index, render, distance, previous, w, h, v.data());
+    for (size_t i = 0; i < v.size(); ++i) out[i] = float_to_half_rne(v[i]);
+}
+void oracle_cloud_upscale_f64(const SkyhipCloudCommonBufferData* c, const float* cb, const float* depth,
+                              const uint16_t* reconstruct, const uint16_t* hdr, int w, int h, double* out) {
+    Upscale<double>(*c, cb, depth, reconstruct, hdr, w, h, out);
+}
+void oracle_cloud_upscale_f32(const SkyhipCloudCommonBufferData* c, const float* cb, const float* depth,
+                              const uint16_t* reconstruct, const uint16_t* hdr, int w, int h, uint16_t* out) {
+    std::vector<float> v(size_t(w) * h * 4);
+    Upscale<float>(*c, cb, depth, reconstruct, hdr, w, h, v.data());
+    for (size_t i = 0; i < v.size(); ++i) out[i] = float_to_half_rne(v[i]);
 }
 uint16_t oracle_float_to_half(float x) { return float_to_half_rne(x); }
 float oracle_half_to_float(uint16_t h) { return half_to_float(h); }

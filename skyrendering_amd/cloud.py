@@ -371,3 +371,87 @@ def cloud_raymarch(ctx: Context, atmosphere_block, common, cloud, material: _abi
         depth.ptr if depth is not None else None, shading_index.ptr if shading_index is not None else None,
         out_rgba16f.ptr, out_distance.ptr, width, height, work_counters.ptr if work_counters is not None else None,
         pixel_steps.ptr if pixel_steps is not None else None))
+
+
+_abi.register("skyhip_cloud_reproject_matrix", C.c_int, [
+    C.POINTER(_abi.CameraC), C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_abi.CloudCommonBufferData),
+    C.c_void_p])
+_abi.register("skyhip_cloud_checkerboard_depth", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p])
+_abi.register("skyhip_cloud_shading_index", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p])
+_abi.register("skyhip_cloud_reconstruct", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+    C.c_void_p, C.c_int32, C.c_int32])
+_abi.register("skyhip_cloud_upscale", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+    C.c_int32])
+
+
+def cloud_reproject_matrix(camera: Camera, bottom_radius: float, common, previous_mvp=None,
+                           previous_camera_position=None, additional_delta=None):
+    """Fills common.uReprojectMat as VolumetricCloud::Update does (VolumetricCloud.cpp:204-214,
+    234); returns this frame's MVP (float32 [16], column-major), the next frame's previous_mvp."""
+    def arr(x, n):
+        return None if x is None else (C.c_float * n)(*[float(v) for v in np.asarray(x, np.float32).ravel()])
+    out = (C.c_float * 16)()
+    _abi.check(_abi.lib().skyhip_cloud_reproject_matrix(
+        C.byref(camera.to_c()), float(bottom_radius), arr(previous_mvp, 16), arr(previous_camera_position, 3),
+        arr(additional_delta, 2), C.byref(common), out))
+    return np.array(out[:], np.float32)
+
+
+class CloudReconstruction:
+    """VolumetricCloud::Render's passes for one viewport (VolumetricCloud.cpp:327-422): the
+    checkerboard depth (viewport / 2), the shading index (viewport / 4), the quarter-resolution
+    raymarch of the indexed texels, the temporal reconstruction (viewport / 2, two textures
+    that swap each frame) and the upscale composited into the HDR image."""
+
+    def __init__(self, ctx: Context, viewport):
+        w, h = int(viewport[0]), int(viewport[1])
+        if w % 4 or h % 4:
+            raise ValueError("the viewport's width and height must be multiples of 4")
+        self.ctx, self.viewport = ctx, (w, h)
+        self.checkerboard = ctx.alloc((w // 2) * (h // 2) * 4)
+        self.index = ctx.alloc((w // 4) * (h // 4) * 8)
+        self.render_texture = ctx.alloc((w // 4) * (h // 4) * 8)
+        self.distance = ctx.alloc((w // 4) * (h // 4) * 4)
+        self.reconstruct = [ctx.alloc((w // 2) * (h // 2) * 8), ctx.alloc((w // 2) * (h // 2) * 8)]
+        self.frames = 0
+
+    def checkerboard_depth(self, depth: DeviceBuffer):
+        w, h = self.viewport
+        _abi.check(_abi.lib().skyhip_cloud_checkerboard_depth(self.ctx.handle, depth.ptr, w, h, self.checkerboard.ptr))
+
+    def shading_index(self, common):
+        w, h = self.viewport
+        _abi.check(_abi.lib().skyhip_cloud_shading_index(self.ctx.handle, C.byref(common), self.checkerboard.ptr, w,
+                                                         h, self.index.ptr))
+
+    def reconstruct_pass(self, common):
+        w, h = self.viewport
+        previous = self.reconstruct[1].ptr if self.frames > 0 else None
+        _abi.check(_abi.lib().skyhip_cloud_reconstruct(
+            self.ctx.handle, C.byref(common), self.checkerboard.ptr, self.index.ptr, self.render_texture.ptr,
+            self.distance.ptr, previous, self.reconstruct[0].ptr, w, h))
+
+    def upscale(self, common, depth: DeviceBuffer, hdr: DeviceBuffer):
+        w, h = self.viewport
+        _abi.check(_abi.lib().skyhip_cloud_upscale(self.ctx.handle, C.byref(common), self.checkerboard.ptr, depth.ptr,
+                                                   self.reconstruct[0].ptr, hdr.ptr, w, h))
+
+    def render(self, atmosphere_block, common, cloud, material: _abi.CloudMaterial, transmittance: DeviceBuffer,
+               t_size, ap_l: DeviceBuffer, ap_t: DeviceBuffer, ap_size, blue_noise: DeviceBuffer, blue_noise_size,
+               depth: DeviceBuffer, hdr: DeviceBuffer, work_counters: DeviceBuffer | None = None):
+        """One frame of VolumetricCloud::Render: hdr (RGBA16F viewport) is composited in place."""
+        w, h = self.viewport
+        self.checkerboard_depth(depth)
+        self.shading_index(common)
+        cloud_raymarch(self.ctx, atmosphere_block, common, cloud, material, transmittance, t_size, ap_l, ap_t, ap_size,
+                       blue_noise, blue_noise_size, self.render_texture, self.distance, w // 4, h // 4,
+                       depth=self.checkerboard, shading_index=self.index, work_counters=work_counters)
+        self.reconstruct_pass(common)
+        self.upscale(common, depth, hdr)
+        self.reconstruct.reverse()  # :419-421
+        self.frames += 1
+
+    Render = render

@@ -581,6 +581,51 @@ extern "C" int skyhip_cloud_shadow_matrices(const SkyhipVolumetricCloudParameter
     return SKYHIP_OK;
 }
 
+extern "C" int skyhip_cloud_reproject_matrix(const SkyhipCamera* camera, float bottom_radius,
+                                             const float previous_mvp[16], const float previous_camera_position[3],
+                                             const float additional_delta[2], SkyhipCloudCommonBufferData* common,
+                                             float out_mvp[16]) {
+    if (!camera || !common) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if ((previous_mvp == nullptr) != (previous_camera_position == nullptr))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "previous_mvp and previous_camera_position go together");
+    if ((previous_mvp && (!finite_block(previous_mvp, 16) || !finite_block(previous_camera_position, 3))) ||
+        (additional_delta && !finite_block(additional_delta, 2)) || !std::isfinite(bottom_radius))
+        return fail(SKYHIP_ERR_NON_FINITE, "non-finite previous frame, delta or radius");
+    float vp[16];
+    SKYHIP_CHECK(skyhip_camera_view_projection(camera, vp, nullptr, nullptr, nullptr));
+    // The local frame of VolumetricCloud::Update (:188-202), as in skyhip_cloud_assign_buffer_data.
+    const V3f earth_center = {0.0f, -bottom_radius, 0.0f};
+    const V3f camera_pos = load3(camera->position);
+    const V3f up = normalize3(camera_pos - earth_center);
+    const V3f origin = earth_center + up * bottom_radius;
+    const V3f fr = normalize3(cross3(up, V3f{1.0f, 0.0f, 0.0f}));
+    const V3f ri = normalize3(cross3(up, fr));
+    const float model[16] = {fr.x, fr.y, fr.z, 0.0f, ri.x, ri.y, ri.z, 0.0f,
+                             up.x, up.y, up.z, 0.0f, origin.x, origin.y, origin.z, 1.0f};
+    float inv_model[16], mvp[16];
+    if (!mat_inverse(model, inv_model)) return fail(SKYHIP_ERR_NON_FINITE, "degenerate local cloud frame");
+    mat_mul(vp, model, mvp);
+    if (previous_mvp) {
+        // :204-214, 234: uReprojectMat = previous MVP * translate(delta_local.xy), delta_local the
+        // camera's move in the local frame plus the material's additional offset.
+        const V3f dw = camera_pos - load3(previous_camera_position);
+        float delta[16];
+        mat_identity(delta);
+        delta[12] = inv_model[0] * dw.x + inv_model[4] * dw.y + inv_model[8] * dw.z;
+        delta[13] = inv_model[1] * dw.x + inv_model[5] * dw.y + inv_model[9] * dw.z;
+        if (additional_delta) {
+            delta[12] += additional_delta[0];
+            delta[13] += additional_delta[1];
+        }
+        mat_mul(previous_mvp, delta, common->uReprojectMat);
+    } else {
+        // No previous frame: this frame's MVP (a static camera).
+        std::memcpy(common->uReprojectMat, mvp, sizeof(mvp));
+    }
+    if (out_mvp) std::memcpy(out_mvp, mvp, sizeof(mvp));
+    return SKYHIP_OK;
+}
+
 extern "C" int skyhip_cloud_sample_info(float repeat_size, int32_t tex_max_width, float fovy_degrees,
                                         float viewport_w, float viewport_h, double offset_x, double offset_y,
                                         SkyhipSampleInfo* out) {

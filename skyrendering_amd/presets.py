@@ -243,3 +243,23 @@ def batched_lut_workload(n_sets=256, suns_per_set=8):
                                             view_projection=vp)
             views.append((s, rp.buffer_data(rb), sv))
     return sets, views
+
+
+def synthetic_depth_buffer(linear_depth_param, width: int, height: int, seed: int = 0) -> np.ndarray:
+    """A [height, width] float32 depth buffer (NDC depth in [0, 1], row 0 at the bottom) for the
+    cloud's checkerboard / reconstruction passes: sky (depth 1) above a rippled horizon at
+    ~55% of the height, a terrain whose linear depth grows from 1.5 km to ~40 km towards it,
+    and six near boxes (0.8-1.8 km).  Depth from linear depth z: (p0 - 1 / z) / p1, the
+    inverse of DepthToLinearDepth (VolumetricCloudCommon.glsl:32-34)."""
+    rng = np.random.default_rng(seed)
+    p0, p1 = (np.float64(v) for v in linear_depth_param)
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    horizon = 0.55 * height + 0.08 * height * np.sin(x / 37.0)
+    z = 1.5 + 40.0 * (y / horizon) ** 2 + 2.0 * np.sin(x / 11.0) * np.cos(y / 7.0)
+    box = max(8, width // 40)
+    for _ in range(6):
+        x0, y0 = rng.integers(0, width - 3 * box), rng.integers(0, int(0.5 * height))
+        z[y0:y0 + rng.integers(box, 3 * box), x0:x0 + rng.integers(box, 3 * box)] = 0.8 + rng.random()
+    d = (p0 - 1.0 / z) / p1
+    d[y > horizon] = 1.0
+    return d.astype(np.float32)
