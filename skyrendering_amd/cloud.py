@@ -158,6 +158,54 @@ class VolumetricCloudDefaultMaterial1(DefaultMaterialCommon):
 
 
 @dataclass
+class VolumetricCloudDefaultMaterial0(DefaultMaterialCommon):
+    """VolumetricCloudDefaultMaterial0, VolumetricCloudDefaultMaterial.h:142-169, .cpp:180-222."""
+    detail_param: tuple = (0.4, 0.0)
+    displacement_scale: float = 1.0
+    kind = _abi.MATERIAL_DEFAULT0
+
+    @classmethod
+    def from_json(cls, data: dict):
+        m = cls()
+        m.load_json(data.get("materail_common_", {}))
+        if "detail_param_" in data:
+            m.detail_param = tuple(float(x) for x in data["detail_param_"])
+        if "displacement_scale_" in data:
+            m.displacement_scale = float(data["displacement_scale_"])
+        return m
+
+    def material0_block(self):
+        b = _abi.CloudMaterial0BufferData()  # Material0::Update (.cpp:198-205)
+        b.uDetailParam[0], b.uDetailParam[1] = self.detail_param
+        b.uDisplacementScale = self.displacement_scale
+        return b
+
+
+@dataclass
+class VolumetricCloudVoxelMaterial:
+    """VolumetricCloudVoxelMaterial, VolumetricCloudVoxelMaterial.h:10-40 (an R8 voxel volume
+    with mips, placed by base_ and width_ in km)."""
+    lod_bias: float = 2.75
+    density: float = 20.0
+    base: tuple = (0.0, 0.0)
+    width: tuple = (2.0, 2.0)
+    kind = _abi.MATERIAL_VOXEL
+
+    def voxel_block(self, viewport, camera: Camera, voxel_dim, offset=(0.0, 0.0)):
+        """VolumetricCloudVoxelMaterial::Update (VolumetricCloudVoxelMaterial.cpp:93-104)."""
+        b = _abi.CloudVoxelBufferData()
+        f32 = np.float32
+        b.uLodBias, b.uDensity = self.lod_bias, self.density
+        for k in range(2):
+            b.uSampleFrequency[k] = f32(1.0) / f32(self.width[k])
+            b.uSampleBias[k] = f32((np.float64(offset[k]) + np.float64(f32(self.base[k]))) / np.float64(f32(self.width[k])))
+        max_width = f32(max(voxel_dim))
+        tan_half = f32(np.tan(f32(np.radians(f32(camera.fovy))) * f32(0.5)))
+        b.uSampleLodK = max_width * tan_half / (f32(min(self.width)) * f32(min(viewport)))
+        return b
+
+
+@dataclass
 class VolumetricCloudMinimalMaterial:
     """VolumetricCloudMinimalMaterial (constant extinction), VolumetricCloudMinimalMaterial.h."""
     density: float = 0.5
@@ -353,9 +401,27 @@ def make_material(material, textures: CloudTextures | None, viewport, camera: Ca
     if material.kind == _abi.MATERIAL_MINIMAL:
         m.minimal.uDensity = material.density
         return m
+    if material.kind == _abi.MATERIAL_VOXEL:
+        raise ValueError("voxel materials need their volume: use voxel_material()")
     m.common = material.common_block(viewport, camera, offset)
-    m.material1 = material.material1_block()
+    if material.kind == _abi.MATERIAL_DEFAULT0:
+        m.material0 = material.material0_block()
+    else:
+        m.material1 = material.material1_block()
     textures.fill_material(m)
+    return m
+
+
+def voxel_material(material: "VolumetricCloudVoxelMaterial", voxel_texture: DeviceBuffer, voxel_dim, levels,
+                   viewport, camera: Camera, offset=(0.0, 0.0)) -> _abi.CloudMaterial:
+    """A VOXEL material over an R8 voxel mip chain (level 0 first, x fastest)."""
+    m = _abi.CloudMaterial()
+    m.kind = _abi.MATERIAL_VOXEL
+    m.voxel = material.voxel_block(viewport, camera, voxel_dim, offset)
+    m.voxel_texture = voxel_texture.ptr
+    for k in range(3):
+        m.voxel_dim[k] = int(voxel_dim[k])
+    m.voxel_levels = int(levels)
     return m
 
 
@@ -385,6 +451,11 @@ _abi.register("skyhip_cloud_reconstruct", C.c_int, [
 _abi.register("skyhip_cloud_upscale", C.c_int, [
     C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
     C.c_int32])
+_abi.register("skyhip_cloud_checkerboard_index", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p])
+_abi.register("skyhip_cloud_reconstruct_upscale", C.c_int, [
+    C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+    C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32])
 
 
 def cloud_reproject_matrix(camera: Camera, bottom_radius: float, common, previous_mvp=None,
@@ -439,18 +510,40 @@ class CloudReconstruction:
         _abi.check(_abi.lib().skyhip_cloud_upscale(self.ctx.handle, C.byref(common), self.checkerboard.ptr, depth.ptr,
                                                    self.reconstruct[0].ptr, hdr.ptr, w, h))
 
+    def checkerboard_index(self, common, depth: DeviceBuffer):
+        """checkerboard_depth + shading_index in one launch."""
+        w, h = self.viewport
+        _abi.check(_abi.lib().skyhip_cloud_checkerboard_index(self.ctx.handle, C.byref(common), depth.ptr, w, h,
+                                                              self.checkerboard.ptr, self.index.ptr))
+
+    def reconstruct_upscale(self, common, depth: DeviceBuffer, hdr: DeviceBuffer):
+        """reconstruct_pass + upscale in one launch."""
+        w, h = self.viewport
+        previous = self.reconstruct[1].ptr if self.frames > 0 else None
+        _abi.check(_abi.lib().skyhip_cloud_reconstruct_upscale(
+            self.ctx.handle, C.byref(common), self.checkerboard.ptr, self.index.ptr, self.render_texture.ptr,
+            self.distance.ptr, previous, self.reconstruct[0].ptr, depth.ptr, hdr.ptr, w, h))
+
     def render(self, atmosphere_block, common, cloud, material: _abi.CloudMaterial, transmittance: DeviceBuffer,
                t_size, ap_l: DeviceBuffer, ap_t: DeviceBuffer, ap_size, blue_noise: DeviceBuffer, blue_noise_size,
-               depth: DeviceBuffer, hdr: DeviceBuffer, work_counters: DeviceBuffer | None = None):
-        """One frame of VolumetricCloud::Render: hdr (RGBA16F viewport) is composited in place."""
+               depth: DeviceBuffer, hdr: DeviceBuffer, work_counters: DeviceBuffer | None = None, fused=True):
+        """One frame of VolumetricCloud::Render: hdr (RGBA16F viewport) is composited in place.
+        fused: the checkerboard + index and reconstruction + upscale passes run as one launch
+        each (same results bit for bit); otherwise one launch per reference pass."""
         w, h = self.viewport
-        self.checkerboard_depth(depth)
-        self.shading_index(common)
+        if fused:
+            self.checkerboard_index(common, depth)
+        else:
+            self.checkerboard_depth(depth)
+            self.shading_index(common)
         cloud_raymarch(self.ctx, atmosphere_block, common, cloud, material, transmittance, t_size, ap_l, ap_t, ap_size,
                        blue_noise, blue_noise_size, self.render_texture, self.distance, w // 4, h // 4,
                        depth=self.checkerboard, shading_index=self.index, work_counters=work_counters)
-        self.reconstruct_pass(common)
-        self.upscale(common, depth, hdr)
+        if fused:
+            self.reconstruct_upscale(common, depth, hdr)
+        else:
+            self.reconstruct_pass(common)
+            self.upscale(common, depth, hdr)
         self.reconstruct.reverse()  # :419-421
         self.frames += 1
 
