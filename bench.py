@@ -261,8 +261,10 @@ RECON_BYTES_PER_PIXEL = (4 + 1) + (1 + 0.5) + (4 + 2 + 2 + 1 + 8 + 8) / 4 + (4 +
 def bench_cloud_reconstruct(ctx, scene, reps=50):
     """VolumetricCloud::Render's passes around the quarter-resolution raymarch at the raymarch
     workload's viewport: checkerboard depth, shading index, temporal reconstruction and the
-    upscale composite (timed together, HIP events), and the whole frame including the
-    quarter-resolution raymarch of the indexed texels; the oracle's FP32 passes beside it."""
+    upscale composite, timed together (HIP events) as the library runs them (two launches:
+    checkerboard + index, reconstruction + upscale) and as one launch per reference pass, and
+    the whole frame including the quarter-resolution raymarch of the indexed texels; the
+    oracle's FP32 passes beside it."""
     from skyrendering_amd import presets
     from skyrendering_amd.cloud import CloudReconstruction, cloud_reproject_matrix
     w, h = scene.width, scene.height
@@ -282,11 +284,16 @@ def bench_cloud_reconstruct(ctx, scene, reps=50):
                    (aw, ah, r.init.aerial_perspective_lut_depth), scene.blue_noise, scene.blue_noise_host.shape[0],
                    d_depth, d_hdr)
 
-    def passes():
+    def passes_separate():
         rec.checkerboard_depth(d_depth)
         rec.shading_index(c)
         rec.reconstruct_pass(c)
         rec.upscale(c, d_depth, d_hdr)
+        rec.reconstruct.reverse()
+
+    def passes_fused():
+        rec.checkerboard_index(c, d_depth)
+        rec.reconstruct_upscale(c, d_depth, d_hdr)
         rec.reconstruct.reverse()
 
     for _ in range(3):
@@ -294,20 +301,25 @@ def bench_cloud_reconstruct(ctx, scene, reps=50):
     ctx.synchronize()
     ctx.record(16)
     for _ in range(reps):
-        passes()
+        passes_separate()
     ctx.record(17)
     for _ in range(reps):
-        frame()
+        passes_fused()
     ctx.record(18)
+    for _ in range(reps):
+        frame()
+    ctx.record(19)
     ctx.synchronize()
-    pass_ms = ctx.elapsed_ms(16, 17) / reps
-    frame_ms = ctx.elapsed_ms(17, 18) / reps
+    separate_ms = ctx.elapsed_ms(16, 17) / reps
+    pass_ms = ctx.elapsed_ms(17, 18) / reps
+    frame_ms = ctx.elapsed_ms(18, 19) / reps
     alg = RECON_BYTES_PER_PIXEL * w * h
     gbs = alg / (pass_ms * 1e-3) / 1e9
     out = {"workload": f"cloud checkerboard {w // 2}x{h // 2}, shading index {w // 4}x{h // 4}, temporal "
                        f"reconstruction {w // 2}x{h // 2} RGBA16F, upscale composite {w}x{h}; frame = these + the "
                        f"quarter-resolution raymarch ({w // 4}x{h // 4}) of the indexed texels",
-           "passes_ms": round(pass_ms, 4), "frame_ms": round(frame_ms, 4),
+           "passes_ms": round(pass_ms, 4), "passes_one_launch_each_ms": round(separate_ms, 4),
+           "frame_ms": round(frame_ms, 4),
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_frame": int(alg)}}
     import oracle

@@ -581,6 +581,22 @@ int skyhip_cloud_upscale(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* 
                          const float* depth, const void* reconstruct_rgba16f, void* hdr_rgba16f, int32_t width,
                          int32_t height);
 
+/* skyhip_cloud_checkerboard_depth then skyhip_cloud_shading_index (VolumetricCloud.cpp:330-354)
+ * in one launch, with the same outputs bit for bit.  depth: 16-byte aligned; the outputs
+ * 8-byte aligned. */
+int skyhip_cloud_checkerboard_index(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                                    const float* depth, int32_t width, int32_t height, float* out_checkerboard,
+                                    float* out_index_linear_depth);
+
+/* skyhip_cloud_reconstruct then skyhip_cloud_upscale (VolumetricCloud.cpp:386-417) in one
+ * launch, with the same outputs bit for bit: out_rgba16f receives the reconstruction (the next
+ * frame's previous_rgba16f) and hdr_rgba16f is composited in place. */
+int skyhip_cloud_reconstruct_upscale(SkyhipContext* ctx, const SkyhipCloudCommonBufferData* common,
+                                     const float* checkerboard, const float* index_linear_depth,
+                                     const void* render_rgba16f, const float* cloud_distance,
+                                     const void* previous_rgba16f, void* out_rgba16f, const float* depth,
+                                     void* hdr_rgba16f, int32_t width, int32_t height);
+
 /* ------------------------------------------------------------------------- */
 /* Cloud textures: noise generators and mip chains                           */
 /* ------------------------------------------------------------------------- */

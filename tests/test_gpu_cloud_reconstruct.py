@@ -8,9 +8,10 @@ image, identical RGBA16F bits for the reconstruction and the composite.  The ora
 path is checked beside it (relative error of the reconstruction where both take the same
 branches).  Inputs: a synthetic depth buffer (sky, a sloping terrain with ripples, near boxes),
 random quarter-resolution samples, distances and history, and a camera that moved between
-frames (sub-texel reprojection).  The chain test runs the real quarter-resolution raymarch in
-index mode (VolumetricCloud::Render) for two frames and checks each pass against the oracle on
-the GPU's own inputs.
+frames (sub-texel reprojection); the tile rows of the fused reconstruction + upscale end in a
+partial tile at this height.  The chain test runs the real quarter-resolution raymarch in index
+mode (VolumetricCloud::Render) for two frames, fused and one launch per pass, and checks each
+pass against the oracle on the GPU's own inputs.
 """
 import numpy as np
 import pytest
@@ -101,9 +102,32 @@ def test_passes_bit_exact_against_oracle(ctx):
     ctx.synchronize()
     _same_bits(d_hdr.download(np.uint16, (H, W, 4)), oracle.cloud_upscale(c2, cb, depth, got, hdr, "f32"),
                "upscale composite")
+    # The fused launches (checkerboard + index; reconstruction + upscale through LDS tiles, with
+    # a partial tile row at this height): the same bits.
+    fused = CloudReconstruction(ctx, (W, H))
+    fused.checkerboard_index(c2, d_depth)
+    ctx.synchronize()
+    _same_bits(fused.checkerboard.download(np.float32, (H // 2, W // 2)), cb, "fused checkerboard")
+    _same_bits(fused.index.download(np.float32, (H // 4, W // 4, 2)), oracle.cloud_shading_index(c2, cb, "f32"),
+               "fused shading index (base 3)")
+    fused.index.upload(idx)
+    fused.render_texture.upload(render)
+    fused.distance.upload(dist)
+    fused.reconstruct[1].upload(prev)
+    for frames, history in ((0, None), (1, prev)):
+        fused.frames = frames
+        d_hdr.upload(hdr)
+        fused.reconstruct_upscale(c2, d_depth, d_hdr)
+        ctx.synchronize()
+        got = fused.reconstruct[0].download(np.uint16, (H // 2, W // 2, 4))
+        _same_bits(got, oracle.cloud_reconstruct(c2, cb, idx, render, dist, history, W, H, "f32"),
+                   f"fused reconstruction ({'with' if history is not None else 'no'} history)")
+        _same_bits(d_hdr.download(np.uint16, (H, W, 4)), oracle.cloud_upscale(c2, cb, depth, got, hdr, "f32"),
+                   "fused upscale composite")
 
 
-def test_render_chain_two_frames(ctx):
+@pytest.mark.parametrize("fused", [True, False])
+def test_render_chain_two_frames(ctx, fused):
     scene = CloudScene(ctx, W, H)
     c = scene.common
     mvp = cloud_reproject_matrix(scene.camera, scene.earth.bottom_radius, c)
@@ -124,7 +148,7 @@ def test_render_chain_two_frames(ctx):
         rec.render(a.buffer, c, scene.cloud_block, scene.mat_block, a.transmittance_texture, a.transmittance_size,
                    r.aerial_perspective_luminance_texture, r.aerial_perspective_transmittance_texture,
                    (aw, ah, r.init.aerial_perspective_lut_depth), scene.blue_noise, scene.blue_noise_host.shape[0],
-                   d_depth, d_hdr)
+                   d_depth, d_hdr, fused=fused)
         ctx.synchronize()
         cb = rec.checkerboard.download(np.float32, (H // 2, W // 2))
         idx = rec.index.download(np.float32, (H // 4, W // 4, 2))
