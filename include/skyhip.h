@@ -529,6 +529,16 @@ int skyhip_cloud_shadow_froxel(SkyhipContext* ctx, const SkyhipCloudCommonBuffer
                                const float* shadow_map_rg32f, int32_t map_width, int32_t map_height,
                                uint16_t* out_froxel_r16, int32_t width, int32_t height, int32_t depth);
 
+/* Binds a ray-scatter-visibility froxel (R16 UNORM [depth][height][width], device memory, e.g.
+ * skyhip_cloud_shadow_froxel's output) to the context, as VolumetricCloud::Render and
+ * PathTracing::Render bind GetShadowFroxel() (VolumetricCloud.cpp:369, 543): later
+ * skyhip_cloud_raymarch and skyhip_cloud_path_trace calls on this context multiply their
+ * aerial-perspective luminance by SampleRayScatterVisibility (VolumetricCloudShadowInterface.glsl:10-13),
+ * LINEAR / CLAMP_TO_EDGE at (uv, distance * uInvShadowFroxelMaxDistance).  NULL unbinds
+ * (visibility 1, the default).  The memory must stay valid while bound. */
+int skyhip_bind_shadow_froxel(SkyhipContext* ctx, const uint16_t* froxel_r16, int32_t width, int32_t height,
+                              int32_t depth);
+
 /* ------------------------------------------------------------------------- */
 /* Cloud checkerboard, shading index, temporal reconstruction and upscale     */
 /* ------------------------------------------------------------------------- */
@@ -647,8 +657,9 @@ int skyhip_generate_mipmaps(SkyhipContext* ctx, uint8_t* texture, int32_t dims, 
  *                   uint(max(steps * dist / max_dist, 1)), each as segment 0 in bits 0-15 and
  *                   segment 1 in bits 16-31, so a checker can replay the discrete decisions
  *                   (step count, early exit at T < 0.01).
- * The cloud shadow froxel visibility (VolumetricCloudShadowInterface.glsl:10-13) is 1:
- * the cloud shadow passes are not part of this library. */
+ * The aerial-perspective luminance is multiplied by the ray-scatter visibility of the froxel
+ * bound with skyhip_bind_shadow_froxel (VolumetricCloudRender.comp:198), or by 1 when none is
+ * bound. */
 int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
                           const SkyhipCloudCommonBufferData* common, const SkyhipCloudBufferData* cloud,
                           const SkyhipCloudMaterial* material, const float* transmittance_rgba,
@@ -756,7 +767,9 @@ int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_
  * accumulation_rgba (RGBA32F width x height, in the frame order), and rendered_mask
  * (R8, may be NULL) gets 1.  The random sequence of each sample is the reference's
  * (seed = PRNG(PRNG(PRNG(x) + y) + frame)); with brick majorants the tracking draws differ
- * but the estimator keeps its expectation.
+ * but the estimator keeps its expectation.  A scattered path's aerial-perspective luminance is
+ * multiplied by the froxel bound with skyhip_bind_shadow_froxel (VolumetricCloudPathTracing.comp:277),
+ * or by 1 when none is bound.
  *   work_counters: NULL, or device uint64[5] accumulating samples, density lookups,
  *                  majorant segments, path vertices (volume and ground events) and shadow rays
  *                  (a counting kernel variant; leave NULL when timing). */

@@ -563,3 +563,20 @@ def cloud_upscale(common, checkerboard, depth, reconstruct_bits, hdr_bits, preci
     getattr(lib(), f"oracle_cloud_upscale_{precision}")(C.byref(common), _ptr(cb), _ptr(d), _ptr(rb), _ptr(hb), w, h,
                                                         _ptr(out))
     return out
+
+
+# The ray-scatter-visibility froxel bound for oracle raymarch / path-trace calls.
+register("oracle_bind_shadow_froxel", None, [vp, i32, i32, i32])
+_bound_froxel = None
+
+
+def bind_shadow_froxel(froxel):
+    """froxel: uint16 [d, h, w] R16 UNORM, or None (visibility 1)."""
+    global _bound_froxel
+    if froxel is None:
+        _bound_froxel = None
+        lib().oracle_bind_shadow_froxel(None, 0, 0, 0)
+        return
+    _bound_froxel = np.ascontiguousarray(froxel, dtype=np.uint16)
+    d, h, w = _bound_froxel.shape
+    lib().oracle_bind_shadow_froxel(_ptr(_bound_froxel), w, h, d)

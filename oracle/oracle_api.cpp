@@ -560,6 +560,9 @@ void oracle_cloud_upscale_f32(const SkyhipCloudCommonBufferData* c, const float*
     Upscale<float>(*c, cb, depth, reconstruct, hdr, w, h, v.data());
     for (size_t i = 0; i < v.size(); ++i) out[i] = float_to_half_rne(v[i]);
 }
+void oracle_bind_shadow_froxel(const uint16_t* froxel, int w, int h, int d) {
+    BoundFroxel() = FroxelBinding{froxel, froxel ? w : 0, froxel ? h : 0, froxel ? d : 0};
+}
 uint16_t oracle_float_to_half(float x) { return float_to_half_rne(x); }
 float oracle_half_to_float(uint16_t h) { return half_to_float(h); }
 

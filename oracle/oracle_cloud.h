@@ -176,6 +176,42 @@ struct CloudImages {
     int image_w, image_h;
 };
 
+// The ray-scatter-visibility froxel bound for the raymarch and the path tracer (R16 UNORM
+// [d][h][w]), or null: visibility 1.  Set by oracle_bind_shadow_froxel, as the library's
+// skyhip_bind_shadow_froxel binds it to a context.
+struct FroxelBinding {
+    const uint16_t* data = nullptr;
+    int w = 0, h = 0, d = 0;
+};
+inline FroxelBinding& BoundFroxel() {
+    static FroxelBinding f;
+    return f;
+}
+
+// SampleRayScatterVisibility (VolumetricCloudShadowInterface.glsl:10-13): texture(froxel,
+// vec3(uv, dist * inv_max)) with LINEAR / CLAMP_TO_EDGE (GetShadowFroxel, VolumetricCloud.h:71-73)
+// on R16 UNORM texels, then mix(1, tex, clamp(1 / w, 0, 1)).
+template <class S>
+S RayScatterVisibility(const FroxelBinding& f, S u, S v, S dist, S inv_max) {
+    S w = dist * inv_max;
+    S c[3] = {gclamp(u * S(f.w) - S(0.5), S(0), S(f.w - 1)), gclamp(v * S(f.h) - S(0.5), S(0), S(f.h - 1)),
+              gclamp(w * S(f.d) - S(0.5), S(0), S(f.d - 1))};
+    int n[3] = {f.w, f.h, f.d}, i0[3], i1[3];
+    S a[3];
+    for (int k = 0; k < 3; ++k) {
+        i0[k] = int(std::floor(c[k]));
+        i1[k] = std::min(i0[k] + 1, n[k] - 1);
+        a[k] = c[k] - S(i0[k]);
+    }
+    S tex = 0;
+    for (int k = 0; k < 8; ++k) {
+        int x = (k & 1) ? i1[0] : i0[0], y = (k & 2) ? i1[1] : i0[1], z = (k & 4) ? i1[2] : i0[2];
+        S wgt = ((k & 1) ? a[0] : S(1) - a[0]) * ((k & 2) ? a[1] : S(1) - a[1]) * ((k & 4) ? a[2] : S(1) - a[2]);
+        tex += S(f.data[(size_t(z) * f.h + y) * f.w + x]) / S(65535) * wgt;
+    }
+    return gmix(S(1), tex, gclamp(S(1) / w, S(0), S(1)));
+}
+
 template <class S>
 struct CloudPixelOut {
     S rgba[4];
@@ -343,7 +379,9 @@ CloudPixelOut<S> CloudRaymarchPixel(const Atmosphere<S>& A, const CloudCtx<S>& C
     S ww = S(0.5) / S(im.ad) + w * (S(1) - S(1) / S(im.ad));
     V3<S> ap_t = tex3d_linear_clamp<S>(im.ap_transmittance, im.aw, im.ah, im.ad, uu, vv, ww);
     V3<S> ap_l = tex3d_linear_clamp<S>(im.ap_luminance, im.aw, im.ah, im.ad, uu, vv, ww);
-    // SampleRayScatterVisibility with a unit froxel: mix(1, 1, x) = 1.
+    const FroxelBinding& froxel = BoundFroxel();
+    if (froxel.data)  // VolumetricCloudRender.comp:198
+        ap_l = ap_l * RayScatterVisibility<S>(froxel, u, v, average_t, S(cb.uInvShadowFroxelMaxDistance));
     luminance = luminance * ap_t + ap_l * (S(1) - ctx.transmittance);
     luminance = luminance * (S(1) / std::fmax(S(1e-5), S(1) - ctx.transmittance));
     S fade = smoothstep(S(b.uMaxVisibleDistance) * S(0.75), S(b.uMaxVisibleDistance), t1[0]);

@@ -485,7 +485,7 @@ struct PathTracer {
         bool hs;
         S st;
         V3<S> L = trace(seed, view, hs, st);
-        if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98), unit froxel visibility
+        if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98) and the froxel (:277)
             S r = cam.z + R, mu = view.z;
             S t = st;
             if (r > A.top_radius) {
@@ -500,6 +500,8 @@ struct PathTracer {
             S ww = S(0.5) / S(ad) + w * (S(1) - S(1) / S(ad));
             V3<S> apt = tex3d_linear_clamp<S>(ap_t, aw, ah, ad, uu, vv, ww);
             V3<S> apl = tex3d_linear_clamp<S>(ap_l, aw, ah, ad, uu, vv, ww);
+            const FroxelBinding& froxel = BoundFroxel();
+            if (froxel.data) apl = apl * RayScatterVisibility<S>(froxel, u, v, st, S(cb.uInvShadowFroxelMaxDistance));
             L = L * apt + apl;
         }
         out[0] = L.x, out[1] = L.y, out[2] = L.z, out[3] = hs ? S(0) : S(1);

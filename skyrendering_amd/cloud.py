@@ -336,6 +336,16 @@ _abi.register("skyhip_cloud_shadow_froxel", C.c_int, [
     C.c_void_p, C.POINTER(_abi.CloudCommonBufferData), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
     C.c_int32, C.c_int32])
 
+_abi.register("skyhip_bind_shadow_froxel", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32])
+
+
+def bind_shadow_froxel(ctx: Context, froxel: DeviceBuffer | None, size=(0, 0, 0)):
+    """skyhip_bind_shadow_froxel: froxel (R16 UNORM, size = (w, h, d)) multiplies the
+    aerial-perspective term of later raymarch and path-trace calls; None unbinds."""
+    w, h, d = size
+    _abi.check(_abi.lib().skyhip_bind_shadow_froxel(ctx.handle, froxel.ptr if froxel is not None else None, w, h, d))
+
+
 SHADOW_MAP_RESOLUTION = (512, 512)  # kShadowMapResolution, VolumetricCloud.cpp:52
 SHADOW_FROXEL_DEPTH = 128           # VolumetricCloud.cpp:135
 
@@ -385,6 +395,11 @@ class CloudShadow:
         self.frames += 1
 
     RenderShadow = render
+
+    def bind(self):
+        """Binds this froxel for the context's raymarch and path-trace calls (the reference binds
+        GetShadowFroxel() to both passes, VolumetricCloud.cpp:369, 543)."""
+        bind_shadow_froxel(self.ctx, self.froxel, self.froxel_size)
 
     def maps(self):
         mw, mh = self.map_size

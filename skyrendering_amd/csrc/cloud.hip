@@ -36,6 +36,9 @@ struct CloudArgs {
     int light_steps;
     float light_offset[16];
     float light_weight[16];
+    // The context's bound ray-scatter-visibility froxel, or null (visibility 1).
+    const uint16_t* froxel;
+    int fw, fh, fd;
 };
 
 struct MarchAcc {
@@ -294,8 +297,11 @@ __global__ __launch_bounds__(256) void cloud_raymarch_kernel(CloudArgs a) {
         const float w = sqrtf(t_ap / cb.uAerialPerspectiveLutMaxDistance);
         const F3 ap_t = lut3d_trilinear(a.ap_t, a.aw, a.ah, a.ad, u * float(a.aw - 1), v * float(a.ah - 1),
                                         w * float(a.ad - 1));
-        const F3 ap_l = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, u * float(a.aw - 1), v * float(a.ah - 1),
-                                        w * float(a.ad - 1));
+        F3 ap_l = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, u * float(a.aw - 1), v * float(a.ah - 1),
+                                  w * float(a.ad - 1));
+        if (a.froxel)  // :198
+            ap_l = ap_l * ray_scatter_visibility(a.froxel, a.fw, a.fh, a.fd, u, v, average_t,
+                                                 cb.uInvShadowFroxelMaxDistance);
         luminance = luminance * ap_t + ap_l * (1.0f - acc.transmittance);
         luminance = luminance * (1.0f / fmaxf(1e-5f, 1.0f - acc.transmittance));
         const float fade = smoothstepf(a.cloud.uMaxVisibleDistance * 0.75f, a.cloud.uMaxVisibleDistance, t1[0]);
@@ -501,6 +507,10 @@ extern "C" int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereB
     a.width = width, a.height = height;
     a.counters = work_counters;
     a.pixel_steps = work_counters ? pixel_steps : nullptr;
+    a.froxel = ctx->froxel;
+    a.fw = ctx->froxel_w, a.fh = ctx->froxel_h, a.fd = ctx->froxel_d;
+    if (a.froxel && !(common->uInvShadowFroxelMaxDistance > 0.0f && std::isfinite(common->uInvShadowFroxelMaxDistance)))
+        return fail(SKYHIP_ERR_NON_FINITE, "a shadow froxel is bound but uInvShadowFroxelMaxDistance is not > 0");
     // SampleShadow's FP32 counter `for (t = 1/n; t <= 1.0; t += 1/n)`, evaluated once here with
     // the same float arithmetic the shader uses (the iteration count depends on its rounding).
     {

@@ -286,6 +286,28 @@ __device__ inline F3 lut3d_trilinear(const float4* __restrict__ lut, int w, int 
     return r;
 }
 
+// SampleRayScatterVisibility (VolumetricCloudShadowInterface.glsl:10-13): the R16 UNORM froxel
+// (LINEAR, CLAMP_TO_EDGE: GetShadowFroxel, VolumetricCloud.h:71-73) at (uv, dist / max distance),
+// faded towards 1 beyond the froxel's far end.
+__device__ inline float ray_scatter_visibility(const uint16_t* __restrict__ fr, int fw, int fh, int fd, float u,
+                                               float v, float dist, float inv_max_dist) {
+    const float w = dist * inv_max_dist;
+    const float fx = fminf(fmaxf(u * float(fw) - 0.5f, 0.0f), float(fw - 1));
+    const float fy = fminf(fmaxf(v * float(fh) - 0.5f, 0.0f), float(fh - 1));
+    const float fz = fminf(fmaxf(w * float(fd) - 0.5f, 0.0f), float(fd - 1));
+    const int x0 = int(fx), y0 = int(fy), z0 = int(fz);
+    const int x1 = min(x0 + 1, fw - 1), y1 = min(y0 + 1, fh - 1), z1 = min(z0 + 1, fd - 1);
+    const float ax = fx - float(x0), ay = fy - float(y0), az = fz - float(z0);
+    float tex = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        const int xx = (k & 1) ? x1 : x0, yy = (k & 2) ? y1 : y0, zz = (k & 4) ? z1 : z0;
+        const float wgt = ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az);
+        tex += float(fr[(zz * fh + yy) * fw + xx]) * (1.0f / 65535.0f) * wgt;
+    }
+    const float t = fminf(fmaxf(1.0f / w, 0.0f), 1.0f);
+    return (1.0f - t) + tex * t;  // mix(1.0, tex, t)
+}
+
 // FP64 unprojection of (u, v, depth) through uInvMVP, relative to the camera.
 __device__ inline F3 unproject(const float* m, float x, float y, float z, F3 cam, float* frag_dist) {
     double v[4] = {x, y, z, 1.0}, o[4];

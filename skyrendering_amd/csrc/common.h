@@ -54,6 +54,9 @@ struct SkyhipContext {
     int wave_size = 64;
     int clock_khz = 0;
     char name[256] = {};
+    // The bound ray-scatter-visibility froxel (R16 UNORM [d][h][w]), or null: visibility 1.
+    const uint16_t* froxel = nullptr;
+    int froxel_w = 0, froxel_h = 0, froxel_d = 0;
 };
 
 namespace skyhip {

@@ -148,7 +148,6 @@ static void look_at(V3f eye, V3f center, V3f up, float* m) {
     m[14] = dot3(f, eye);
 }
 
-// glm::perspective (right-handed, clip z in [-1, 1])
 // glm::ortho(left, right, bottom, top, z_near, z_far): right-handed, clip z in [-1, 1].
 static void ortho(float l, float r, float b, float t, float n, float f, float* m) {
     mat_identity(m);
@@ -160,6 +159,7 @@ static void ortho(float l, float r, float b, float t, float n, float f, float* m
     m[14] = -(f + n) / (f - n);
 }
 
+// glm::perspective (right-handed, clip z in [-1, 1])
 static void perspective(float fovy, float aspect, float z_near, float z_far, float* m) {
     float t = std::tan(fovy / 2.0f);
     std::memset(m, 0, 16 * sizeof(float));
@@ -236,6 +236,18 @@ extern "C" int skyhip_destroy(SkyhipContext* ctx) {
 extern "C" const char* skyhip_last_error(void) { return g_last_error; }
 
 extern "C" void* skyhip_stream(SkyhipContext* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int skyhip_bind_shadow_froxel(SkyhipContext* ctx, const uint16_t* froxel_r16, int32_t width,
+                                         int32_t height, int32_t depth) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (froxel_r16 && (width < 1 || height < 1 || depth < 1 || width > 8192 || height > 8192 || depth > 1024))
+        return fail(SKYHIP_ERR_INVALID_SIZE, "froxel size out of range");
+    ctx->froxel = froxel_r16;
+    ctx->froxel_w = froxel_r16 ? width : 0;
+    ctx->froxel_h = froxel_r16 ? height : 0;
+    ctx->froxel_d = froxel_r16 ? depth : 0;
+    return SKYHIP_OK;
+}
 
 extern "C" int skyhip_synchronize(SkyhipContext* ctx) {
     SKYHIP_CHECK(check_ctx(ctx));

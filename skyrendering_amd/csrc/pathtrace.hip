@@ -58,6 +58,10 @@ struct PtArgs {
     int nb[3];
     float tex_scale[3], tex_bias[3], grid_n[3];
     float majorant_scale;
+    // The context's bound ray-scatter-visibility froxel, or null (visibility 1)
+    const uint16_t* froxel;
+    int fw, fh, fd;
+    float inv_froxel_max_distance;
     // Output
     int region[4];
     uint32_t first_frame;
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(256) void path_trace_kernel(PtArgs a) {
             bool hs;
             float st;
             F3 L = trace<kMat, kBricks, kCount>(a, rng, view, hs, st, wc);
-            if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98), unit froxel visibility
+            if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98) and the froxel visibility (:277)
                 float r = a.cam.z + a.R, mu = view.z;
                 float t = st;
                 if (r > a.A.rt) {
@@ -424,6 +428,8 @@ __global__ __launch_bounds__(256) void path_trace_kernel(PtArgs a) {
                 float fx = u * float(a.aw - 1), fy = v * float(a.ah - 1), fz = w * float(a.ad - 1);
                 F3 apt = lut3d_trilinear(a.ap_t, a.aw, a.ah, a.ad, fx, fy, fz);
                 F3 apl = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, fx, fy, fz);
+                if (a.froxel)
+                    apl = apl * ray_scatter_visibility(a.froxel, a.fw, a.fh, a.fd, u, v, st, a.inv_froxel_max_distance);
                 L = L * apt + apl;
             }
             acc.x += L.x;
@@ -614,6 +620,11 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
     a.aw = in->ap_width;
     a.ah = in->ap_height;
     a.ad = in->ap_depth;
+    a.froxel = ctx->froxel;
+    a.fw = ctx->froxel_w, a.fh = ctx->froxel_h, a.fd = ctx->froxel_d;
+    a.inv_froxel_max_distance = common->uInvShadowFroxelMaxDistance;
+    if (a.froxel && !(a.inv_froxel_max_distance > 0.0f && std::isfinite(a.inv_froxel_max_distance)))
+        return fail(SKYHIP_ERR_NON_FINITE, "a shadow froxel is bound but uInvShadowFroxelMaxDistance is not > 0");
     if (bricks) {
         const SkyhipCloudVoxelBufferData& vb = material->voxel;
         a.majorants = in->brick_majorants;

@@ -213,3 +213,31 @@ def test_material_sample_sigma_t_parity(ctx, lod_bias):
     for i in bad:
         print(f"   pos {pos32[i]} dist {dist[i]:.2f} gpu {got[i]:.6g} oracle {ref[i]:.6g}")
     assert np.mean(err > 1e-3) < 2e-3
+
+
+def test_raymarch_with_shadow_froxel_parity(ctx):
+    """The benchmark scene at 480x270 with its own cloud shadow froxel bound (the reference
+    binds GetShadowFroxel() to the render pass, VolumetricCloud.cpp:369): the aerial-perspective
+    term is scaled by SampleRayScatterVisibility (VolumetricCloudRender.comp:198).  The oracle
+    gets the same froxel bytes; the bars are the raymarch's, and the froxel must change the image."""
+    from skyrendering_amd.cloud import CloudShadow, bind_shadow_froxel, cloud_shadow_matrices
+    scene = CloudScene(ctx, 480, 270)
+    scene.run()
+    unbound, _ = scene.image()
+    sun = np.array(scene.renderer.buffer.sun_direction[:], np.float32)
+    cloud_shadow_matrices(scene.cloud, scene.camera, scene.earth.bottom_radius, sun, scene.common)
+    sh = CloudShadow(ctx, (480, 270))
+    sh.render(scene.common, scene.mat_block, scene.blue_noise, scene.blue_noise_host.shape[1])
+    sh.bind()
+    froxel = sh.froxel_volume()
+    oracle.bind_shadow_froxel(froxel)
+    try:
+        print(f"froxel {froxel.shape}: visibility {froxel.min() / 65535:.3f}-{froxel.max() / 65535:.3f}")
+        _compare(scene, 1e-2, "Material1 480x270, shadow froxel bound")
+        bound, _ = scene.image()
+    finally:
+        bind_shadow_froxel(ctx, None)
+        oracle.bind_shadow_froxel(None)
+    changed = np.abs(bound[..., :3] - unbound[..., :3]).max(-1) > 1e-3 * np.abs(unbound[..., :3]).max()
+    print(f"pixels changed by the froxel: {changed.mean():.3f}")
+    assert changed.mean() > 0.01

@@ -229,3 +229,37 @@ def test_counting_variant_and_rejections(ctx):
     assert L.skyhip_cloud_path_trace(*args(ok, 1, scene.tracer.params, minimal, scene.inputs)) == \
         _abi.ERR_INVALID_ARGUMENT  # brick majorants need a VOXEL_F16 material
     acc.free()
+
+
+def test_bound_froxel_samples_match_oracle(ctx):
+    """With a ray-scatter-visibility froxel bound (skyhip_bind_shadow_froxel), each scattered
+    sample's aerial-perspective luminance is scaled by SampleRayScatterVisibility
+    (VolumetricCloudPathTracing.comp:277): per-sample parity against the oracle with the same
+    froxel bound, and the samples differ from the unbound ones."""
+    from skyrendering_amd.cloud import bind_shadow_froxel
+    w = cases.small_grid_workload()
+    scene = PathTraceScene(ctx, 32, 32, workload=w, bricks=False)
+    frames = 2
+    unbound = _gpu_samples(scene, frames)
+    fw, fh, fd = 8, 6, 16
+    rng = np.random.default_rng(12)
+    froxel = (65535 * (0.2 + 0.8 * rng.random((fd, fh, fw)))).astype(np.uint16)
+    d_froxel = ctx.upload(froxel)
+    bind_shadow_froxel(ctx, d_froxel, (fw, fh, fd))
+    oracle.bind_shadow_froxel(froxel)
+    try:
+        gpu = _gpu_samples(scene, frames)
+        ref = oracle.path_trace(scene.atmosphere.buffer, scene.renderer.buffer, scene.common, scene.tracer.params,
+                                _oracle_inputs(scene), 32, 32, np.arange(32 * 32), 1, frames)
+    finally:
+        bind_shadow_froxel(ctx, None)
+        oracle.bind_shadow_froxel(None)
+    floor = 1e-6 * np.abs(ref[..., :3]).max()
+    err = np.abs(gpu - ref) / np.maximum(np.abs(ref), floor)
+    match = np.all(err <= 1e-3, axis=-1)
+    scattered = ref[..., 3] == 0
+    changed = np.any(np.abs(gpu - unbound) > 1e-6 * np.abs(unbound).max(), axis=-1)
+    print(f"froxel bound: {match.mean():.4f} of samples match; {changed[scattered].mean():.3f} of scattered samples "
+          f"changed by the froxel")
+    assert scattered.mean() > 0.1 and match.mean() >= 0.9
+    assert changed[scattered].mean() > 0.5 and not changed[~scattered].any()

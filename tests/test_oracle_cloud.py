@@ -129,3 +129,70 @@ def test_mip_level_selection_follows_gl_rule():
         # density = cov * clamp((1 - h01) * 100) * 1.0 * h01 with the detail term scaled to 0.
         expected_cov = (200 + 5 * level) / 255.0
         assert abs(got - expected_cov * want_h01) < 1e-9, (lam, level, got)
+
+
+def test_froxel_visibility_scales_the_aerial_perspective_term():
+    """SampleRayScatterVisibility (VolumetricCloudShadowInterface.glsl:10-13) in the composite
+    (VolumetricCloudRender.comp:198).  With unit T / AP-transmittance LUTs and a constant
+    AP luminance Lap, the pixel's luminance is S + Lap * vis * (1 - T) (no fade), so with L1 the
+    unbound image L = L1 - (1 - vis) Lap (1 - T).  vis = mix(1, tex, min(1, 1 / w)), w = distance /
+    max distance: a constant froxel c gives tex = c; a froxel ramping along z gives tex = w
+    (trilinear of texel centres (z + 0.5) / d), one along x gives tex = u; beyond the far end
+    (w > 1) the constant froxel's effect falls as 1 / w."""
+    width, height = 33, 19
+    cam, common, blk = _blocks(width=width, height=height)
+    mat = _abi.CloudMaterial()
+    mat.kind = _abi.MATERIAL_MINIMAL
+    mat.minimal.uDensity = float(np.float32(0.05))
+    T, apl, apt, bn = _unit_images(width, height)
+    lap = 0.375
+    apl[..., :3] = lap
+    inputs = oracle.HostCloudInputs(mat, T=T, ap_l=apl, ap_t=apt, blue_noise=bn, image_size=(width, height))
+    atm = presets.earth_100km().buffer_data()
+    px = np.arange(width * height)
+    u = ((px % width) + 0.5) / width
+    fd, fh, fw = 16, 5, 11
+
+    def render(froxel, inv_max=None):
+        if inv_max is not None:
+            common.uInvShadowFroxelMaxDistance = inv_max
+        oracle.bind_shadow_froxel(froxel)
+        try:
+            return oracle.cloud_raymarch(atm, common, blk, inputs, width, px)
+        finally:
+            oracle.bind_shadow_froxel(None)
+
+    base_inv = common.uInvShadowFroxelMaxDistance
+    L1, dist, _ = render(None, base_inv)
+    one_minus_t = 1.0 - L1[:, 3]
+    sel = one_minus_t > 1e-3
+    assert sel.sum() > 50
+    w = dist * np.float64(np.float32(base_inv))
+    assert w[sel].max() < 1.0 - 0.5 / fd and w[sel].min() > 0.5 / fd
+    ones = np.full((fd, fh, fw), 65535, np.uint16)
+    np.testing.assert_allclose(render(ones)[0], L1, rtol=1e-12, atol=1e-15)
+
+    def expect(vis):
+        return L1[:, :3] - ((1.0 - vis) * lap * one_minus_t)[:, None]
+
+    c = 20000
+    got = render(np.full((fd, fh, fw), c, np.uint16))[0]
+    np.testing.assert_allclose(got[sel, :3], expect(np.full(len(px), c / 65535))[sel], rtol=1e-9, atol=1e-12)
+    ramp_z = np.round((np.arange(fd) + 0.5) / fd * 65535).astype(np.uint16)[:, None, None] * np.ones((1, fh, fw), np.uint16)
+    got = render(ramp_z)[0]
+    np.testing.assert_allclose(got[sel, :3], expect(w)[sel], rtol=0, atol=lap * 1e-5)
+    ramp_x = np.round((np.arange(fw) + 0.5) / fw * 65535).astype(np.uint16)[None, None, :] * np.ones((fd, fh, 1), np.uint16)
+    got = render(ramp_x)[0]
+    inner = sel & (u > 0.5 / fw) & (u < 1 - 0.5 / fw)
+    np.testing.assert_allclose(got[inner, :3], expect(u)[inner], rtol=0, atol=lap * 1e-5)
+    # Beyond the froxel's far end: w > 1 everywhere with a 1 km froxel depth.
+    near_inv = float(np.float32(1.0))
+    L1b, distb, _ = render(None, near_inv)
+    wb = distb * near_inv
+    selb = (1.0 - L1b[:, 3] > 1e-3) & (wb > 1.0)
+    assert selb.sum() > 50
+    got = render(np.zeros((fd, fh, fw), np.uint16), near_inv)[0]
+    vis = 1.0 - 1.0 / wb
+    np.testing.assert_allclose(got[selb, :3], (L1b[:, :3] - ((1.0 - vis) * lap * (1.0 - L1b[:, 3]))[:, None])[selb],
+                               rtol=1e-9, atol=1e-12)
+    common.uInvShadowFroxelMaxDistance = base_inv
