@@ -15,8 +15,11 @@ path tracer (1024x1024, 64 frames of one sample, brick majorants, with the singl
 tracker timed beside it); the batched LUT build (256 atmosphere sets x 8 suns, mixed sizes,
 one hipGraph launch, autotuned); the learned sky-view surrogate (trained here on 2^22
 samples, then 64 panoramas at 1920x1080 rendered by the plain integrator, by the surrogate
-with its per-tile residual check and fallback, and by sky-view LUT sampling); and for each
-the CPU oracle on a bounded sample of the same work.
+with its per-tile residual check and fallback, and by sky-view LUT sampling); the cloud
+shadow passes (512^2 map, blur, 160x90x128 froxel); the quarter-resolution cloud pipeline
+(checkerboard depth, shading index, temporal reconstruction, upscale composite at 1080p, and
+the frame with its 480x270 raymarch); the environment cube with its IBL products; and for
+each the CPU oracle on a bounded sample of the same work.
 """
 import argparse
 import json
