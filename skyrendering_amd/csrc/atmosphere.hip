@@ -57,19 +57,56 @@ __global__ __launch_bounds__(256) void sky_view_kernel(SkyhipAtmosphereBufferDat
 // ---------------------------------------------------------------------------
 // Aerial perspective (AtmosphereRenderer.glsl:191-243)
 // ---------------------------------------------------------------------------
+// kApLanes consecutive lanes share a froxel: each integrates a contiguous part of its float
+// counter's steps (lane p takes steps [p c, (p + 1) c), c = ceil(n / kApLanes), starting from the
+// counter value the shader's loop reaches there), and the first lane combines the parts in
+// order, L = L0 + Tr0 (L1 + Tr1 (L2 + Tr2 L3)), Tr = Tr0 Tr1 Tr2 Tr3.  A 32^3 LUT is 32,768
+// froxels: one lane each would leave half the SIMDs idle and the rest one wave deep.
+constexpr int kApLanes = 4;
+
 __global__ __launch_bounds__(256) void aerial_perspective_kernel(
     SkyhipAtmosphereBufferData b, SkyhipAtmosphereRenderBufferData rbd, const float4* __restrict__ T, int tw, int th,
     const float4* __restrict__ MS, int mw, int mh, const uint16_t* __restrict__ blue_noise, int bn_size,
     float4* __restrict__ out_l, float4* __restrict__ out_t, int w, int h, int d) {
-    int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= w * h * d) return;
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int n_texels = w * h * d;
+    const int part = int(g % kApLanes);
+    const bool valid = g / kApLanes < n_texels;
+    const int t = valid ? int(g / kApLanes) : 0;  // every lane of a group reaches the shuffles
     const AtmDev A = make_atm(b);
     const RenderDev R = make_render(rbd);
-    F3 L, Tr;
-    aerial_texel(A, R, rbd.inv_view_projection, T, tw, th, MS, mw, mh, blue_noise, bn_size, t % w, (t / w) % h,
-                 t / (w * h), w, h, d, L, Tr);
-    out_l[t] = make_float4(L.x, L.y, L.z, 0.0f);
-    out_t[t] = make_float4(Tr.x, Tr.y, Tr.z, 0.0f);
+    const AerialRay ray = aerial_ray(A, R, rbd.inv_view_projection, blue_noise, bn_size, t % w, (t / w) % h,
+                                     t / (w * h), w, h, d);
+    F3 L = f3(0.0f), Tr = f3(1.0f);
+    if (ray.marching_distance > 0.0f) {
+        const float steps = R.ap_steps;
+        int n = 0;  // iterations of the shader's loop `for (i = start_i; i < steps; i += 1)`
+        float i0 = ray.start_i;
+        for (float i = ray.start_i; i < steps; i += 1.0f) ++n;
+        const int c = (n + kApLanes - 1) / kApLanes, first = part * c;
+        for (int k = 0; k < first && k < n; ++k) i0 += 1.0f;
+        const int count = max(0, min(c, n - first));
+        const ScatterRay sc = scatter_ray<false>(A, ray.s, ray.marching_distance, steps);
+        scattered_luminance_part(A, T, tw, th, MS, mw, mh, ray.s, sc, i0, count, L, Tr);
+    }
+    // Combine from the last part down: acc = L_p + Tr_p acc.
+    F3 accL = L, accT = Tr;
+    for (int p = kApLanes - 1; p >= 1; --p) {
+        const F3 Lp = {__shfl_down(L.x, p, 64), __shfl_down(L.y, p, 64), __shfl_down(L.z, p, 64)};
+        const F3 Tp = {__shfl_down(Tr.x, p, 64), __shfl_down(Tr.y, p, 64), __shfl_down(Tr.z, p, 64)};
+        if (p == kApLanes - 1) {
+            accL = Lp;
+            accT = Tp;
+        } else {
+            accL = Lp + Tp * accL;
+            accT = Tp * accT;
+        }
+    }
+    if (part == 0 && valid) {
+        const F3 Lo = L + Tr * accL, To = Tr * accT;
+        out_l[t] = make_float4(Lo.x, Lo.y, Lo.z, 0.0f);
+        out_t[t] = make_float4(To.x, To.y, To.z, 0.0f);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -142,7 +179,8 @@ int launch_aerial_perspective(hipStream_t st, const SkyhipAtmosphereBufferData& 
                               const SkyhipAtmosphereRenderBufferData& r, const float* T, int tw, int th,
                               const float* MS, int mw, int mh, const uint16_t* bn, int bn_size, float* L, float* Tr,
                               int w, int h, int d) {
-    hipLaunchKernelGGL(aerial_perspective_kernel, dim3(div_up((long long)w * h * d, 256)), dim3(256), 0, st, b, r,
+    hipLaunchKernelGGL(aerial_perspective_kernel, dim3(div_up((long long)w * h * d * kApLanes, 256)), dim3(256), 0, st,
+                       b, r,
                        reinterpret_cast<const float4*>(T), tw, th, reinterpret_cast<const float4*>(MS), mw, mh, bn,
                        bn_size, reinterpret_cast<float4*>(L), reinterpret_cast<float4*>(Tr), w, h, d);
     SKYHIP_LAUNCH_CHECK("aerial_perspective_kernel");

@@ -41,6 +41,7 @@ struct AtmDev {
     float H2;         // rt^2 - rb^2
     float H;
     float inv_H;
+    float inv_thickness;
     float t_steps, ms_steps;
 };
 
@@ -65,10 +66,17 @@ __host__ __device__ inline AtmDev make_atm(const SkyhipAtmosphereBufferData& b) 
     a.H2 = a.thickness * (b.top_radius + b.bottom_radius);
     a.H = sqrtf(a.H2);
     a.inv_H = 1.0f / a.H;
+    a.inv_thickness = 1.0f / a.thickness;
     a.t_steps = b.transmittance_steps;
     a.ms_steps = b.multiscattering_steps;
     return a;
 }
+
+// Division and square root through the hardware's reciprocal and square-root instructions
+// (about 1 ulp) for the integrators' per-step arithmetic; the correctly rounded forms expand
+// to about ten instructions each, and a step has nine divisions and three square roots.
+__device__ __forceinline__ float fdiv_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Atmosphere.glsl:119-132 / :156-159 split into densities so the exps are shared.
 struct Densities {
@@ -115,25 +123,28 @@ __device__ inline F3 sun_visibility(const AtmDev& A, const float4* __restrict__ 
                                     float q, float altitude, float mu_s) {
     // GetTransmittanceTextureUvFromRMu (:90-103): rho = SafeSqrt(r^2 - R_b^2),
     // discriminant r^2 (mu^2 - 1) + R_t^2 = r^2 mu^2 + (H^2 - q).
-    float rho = sqrtf(fmaxf(q, 0.0f));
+    float rho = fsqrt_fast(fmaxf(q, 0.0f));
     float disc = r * r * mu_s * mu_s + (A.H2 - q);
-    float d = fmaxf(-r * mu_s + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+    float d = fmaxf(-r * mu_s + fsqrt_fast(fmaxf(disc, 0.0f)), 0.0f);
     float d_min = A.thickness - altitude;
     float d_max = rho + A.H;
-    float x_mu = (d - d_min) / (d_max - d_min);
+    float x_mu = fdiv_fast(d - d_min, d_max - d_min);
     float x_r = rho * A.inv_H;
     F3 t = lut_bilinear(T, tw, th, x_mu * float(tw - 1), x_r * float(th - 1));
-    float sin_theta_h = A.rb / r;
-    float cos_theta_h = -rho / r;
+    const float inv_r = __builtin_amdgcn_rcpf(r);
+    float sin_theta_h = A.rb * inv_r;
+    float cos_theta_h = -rho * inv_r;
     float e = sin_theta_h * A.sun_angular_radius;
-    return t * smoothstepf(-e, e, mu_s - cos_theta_h);
+    // smoothstep(-e, e, x)
+    float s = clamp01(fdiv_fast(mu_s - cos_theta_h + e, 2.0f * e));
+    return t * (s * s * (3.0f - 2.0f * s));
 }
 
 // GetMultiscatteringContribution (Atmosphere.glsl:169-178).
 __device__ inline F3 ms_contribution(const AtmDev& A, const float4* __restrict__ MS, int mw, int mh, float altitude,
                                      float mu_s) {
     float x_mu_s = mu_s * 0.5f + 0.5f;
-    float x_r = altitude / A.thickness;
+    float x_r = altitude * A.inv_thickness;
     return lut_bilinear(MS, mw, mh, x_mu_s * float(mw - 1), x_r * float(mh - 1));
 }
 
@@ -165,6 +176,44 @@ __device__ inline float distance_to_bottom(const AtmDev& A, float r, float alt0,
     return fmaxf(d, 0.0f);
 }
 
+// Per-ray constants of the integrator below.
+struct ScatterRay {
+    float dx, rp, mp, two_r_mu, rho0_2, rb2;
+};
+template <bool kMS>
+__device__ inline ScatterRay scatter_ray(const AtmDev& A, const RayStart& s, float marching_distance, float steps) {
+    const float iso = 1.0f / (4.0f * kPiF);
+    return {marching_distance / steps, kMS ? iso : rayleigh_phase(s.nu), kMS ? iso : mie_phase(A.g, s.nu),
+            2.0f * s.r * s.mu, s.alt0 * (s.r + A.rb), A.rb * A.rb};
+}
+
+// One step of the integrator at float counter value i.
+template <bool kMS>
+__device__ __forceinline__ void scatter_step(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                             const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
+                                             const ScatterRay& c, float i, F3& L, F3& Tr, F3& Lf) {
+    const float dx = c.dx;
+    float d = i * dx;
+    float q = fmaf(d, d + c.two_r_mu, c.rho0_2);  // r_d^2 - R_b^2, negative below ground
+    float r_d = fsqrt_fast(fmaxf(q + c.rb2, 0.0f));
+    float alt = fdiv_fast(q, r_d + A.rb);
+    float mu_s = fdiv_fast(s.r * s.mu_s0 + d * s.nu, r_d);
+    Densities dn = densities(A, alt);
+    F3 sr = A.rayleigh * dn.rayleigh;
+    F3 sm = A.mie_s * dn.mie;
+    F3 ext = sr + A.mie_ext * dn.mie + A.ozone * dn.ozone;
+    F3 one_minus_t = {-expm1f(-ext.x * dx), -expm1f(-ext.y * dx), -expm1f(-ext.z * dx)};
+    F3 Li = (sr * c.rp + sm * c.mp) * sun_visibility(A, T, tw, th, r_d, q, alt, mu_s);
+    F3 inv_ext = {__builtin_amdgcn_rcpf(ext.x), __builtin_amdgcn_rcpf(ext.y), __builtin_amdgcn_rcpf(ext.z)};
+    if (!kMS) {
+        Li += A.ms_mask * ms_contribution(A, MS, mw, mh, alt, mu_s) * (sr + sm);
+        Li *= A.solar;
+    }
+    L += Tr * Li * one_minus_t * inv_ext;
+    if (kMS) Lf += Tr * (sr + sm) * one_minus_t * inv_ext;
+    Tr *= F3{__expf(-ext.x * dx), __expf(-ext.y * dx), __expf(-ext.z * dx)};
+}
+
 // ComputeScatteredLuminance (Atmosphere.glsl:220-295).  kMS selects the
 // MULTISCATTERING_COMPUTE_PROGRAM variant.  The loop keeps the shader's float
 // counter (i = start_i; i < steps; ++i), so fractional step counts iterate
@@ -174,37 +223,25 @@ __device__ inline void scattered_luminance(const AtmDev& A, const float4* __rest
                                            const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
                                            float marching_distance, float steps, float start_i, F3& L, F3& Tr,
                                            F3& Lf) {
-    const float dx = marching_distance / steps;
-    const float iso = 1.0f / (4.0f * kPiF);
-    const float rp = kMS ? iso : rayleigh_phase(s.nu);
-    const float mp = kMS ? iso : mie_phase(A.g, s.nu);
-    const float two_r_mu = 2.0f * s.r * s.mu;
-    const float rho0_2 = s.alt0 * (s.r + A.rb);
-    const float rb2 = A.rb * A.rb;
+    const ScatterRay c = scatter_ray<kMS>(A, s, marching_distance, steps);
     L = f3(0.0f);
     Tr = f3(1.0f);
     Lf = f3(0.0f);
-    for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f) {
-        float d = i * dx;
-        float q = fmaf(d, d + two_r_mu, rho0_2);  // r_d^2 - R_b^2, negative below ground
-        float r_d = sqrtf(fmaxf(q + rb2, 0.0f));
-        float alt = q / (r_d + A.rb);
-        float mu_s = (s.r * s.mu_s0 + d * s.nu) / r_d;
-        Densities dn = densities(A, alt);
-        F3 sr = A.rayleigh * dn.rayleigh;
-        F3 sm = A.mie_s * dn.mie;
-        F3 ext = sr + A.mie_ext * dn.mie + A.ozone * dn.ozone;
-        F3 one_minus_t = {-expm1f(-ext.x * dx), -expm1f(-ext.y * dx), -expm1f(-ext.z * dx)};
-        F3 Li = (sr * rp + sm * mp) * sun_visibility(A, T, tw, th, r_d, q, alt, mu_s);
-        F3 inv_ext = {1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z};
-        if (!kMS) {
-            Li += A.ms_mask * ms_contribution(A, MS, mw, mh, alt, mu_s) * (sr + sm);
-            Li *= A.solar;
-        }
-        L += Tr * Li * one_minus_t * inv_ext;
-        if (kMS) Lf += Tr * (sr + sm) * one_minus_t * inv_ext;
-        Tr *= F3{__expf(-ext.x * dx), __expf(-ext.y * dx), __expf(-ext.z * dx)};
-    }
+    for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f)
+        scatter_step<kMS>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
+}
+
+// `count` consecutive steps of the same float counter, starting at counter value i0, with
+// (L, Tr) starting at (0, 1): a contiguous part of scattered_luminance's sum.  Parts combine
+// in order as L = L_a + Tr_a L_b, Tr = Tr_a Tr_b (exact in real arithmetic).
+__device__ inline void scattered_luminance_part(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                                const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
+                                                const ScatterRay& c, float i0, int count, F3& L, F3& Tr) {
+    L = f3(0.0f);
+    Tr = f3(1.0f);
+    F3 Lf = f3(0.0f);
+    float i = i0;
+    for (int k = 0; k < count; ++k, i += 1.0f) scatter_step<false>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
 }
 
 }  // namespace skyhip

@@ -5,7 +5,7 @@
 //   transmittance_texel   Atmosphere.glsl:71-88, :311-337 (FP32, or FP64 near the horizon)
 //   multiscattering_texel Atmosphere.glsl:344-438, one wave64 per texel
 //   sky_view_texel        AtmosphereRenderer.glsl:56-186
-//   aerial_texel          AtmosphereRenderer.glsl:191-243
+//   aerial_ray            AtmosphereRenderer.glsl:191-243 (set-up; the integration is split over lanes)
 #pragma once
 
 #include "atmosphere_device.h"
@@ -228,29 +228,27 @@ __device__ inline F3 unproject_direction(const float* m, float x, float y, float
     return {float(px / len), float(py / len), float(pz / len)};
 }
 
-// One aerial-perspective froxel (:191-243): luminance and transmittance.
-__device__ inline void aerial_texel(const AtmDev& A, const RenderDev& R, const float* inv_view_projection,
-                                    const float4* __restrict__ T, int tw, int th, const float4* __restrict__ MS,
-                                    int mw, int mh, const uint16_t* __restrict__ blue_noise, int bn_size, int ix,
-                                    int iy, int iz, int w, int h, int d, F3& L, F3& Tr) {
+// The set-up of one aerial-perspective froxel (:191-243): the ray start, the marching distance
+// (0: nothing to integrate) and the float counter's start value.
+struct AerialRay {
+    RayStart s;
+    float marching_distance, start_i;
+};
+__device__ inline AerialRay aerial_ray(const AtmDev& A, const RenderDev& R, const float* inv_view_projection,
+                                       const uint16_t* __restrict__ blue_noise, int bn_size, int ix, int iy, int iz,
+                                       int w, int h, int d) {
     const float u = float(ix) / float(w - 1), v = float(iy) / float(h - 1), q = float(iz) / float(d - 1);
     const F3 view = unproject_direction(inv_view_projection, u * 2.0f - 1.0f, v * 2.0f - 1.0f, 0.0f, R.cam, nullptr);
     float marching_distance = q * q * R.ap_max_distance;
     const float r = R.r;
     const float alt0 = r - A.rb;
-    RayStart s{r, alt0, dot(view, R.up), dot(R.sun, R.up), dot(view, R.sun)};
-    const bool hit = ray_hits_ground(A, r, alt0, s.mu);
-    float max_marching = hit ? marching_distance : distance_to_top(A, r, alt0, s.mu);
-    start_from_space(A, s, max_marching);
-    marching_distance = fminf(marching_distance, max_marching);
-    L = f3(0.0f);
-    Tr = f3(1.0f);
-    F3 Lf;
-    if (marching_distance > 0.0f) {
-        float start_i = 0.5f;
-        if (blue_noise) start_i = float(blue_noise[(iy & (bn_size - 1)) * bn_size + (ix & (bn_size - 1))]) / 65535.0f;
-        scattered_luminance<false>(A, T, tw, th, MS, mw, mh, s, marching_distance, R.ap_steps, start_i, L, Tr, Lf);
-    }
+    AerialRay a{RayStart{r, alt0, dot(view, R.up), dot(R.sun, R.up), dot(view, R.sun)}, 0.0f, 0.5f};
+    const bool hit = ray_hits_ground(A, r, alt0, a.s.mu);
+    float max_marching = hit ? marching_distance : distance_to_top(A, r, alt0, a.s.mu);
+    start_from_space(A, a.s, max_marching);
+    a.marching_distance = fminf(marching_distance, max_marching);
+    if (blue_noise) a.start_i = float(blue_noise[(iy & (bn_size - 1)) * bn_size + (ix & (bn_size - 1))]) / 65535.0f;
+    return a;
 }
 
 }  // namespace skyhip
