@@ -57,11 +57,8 @@ __global__ __launch_bounds__(256) void sky_view_kernel(SkyhipAtmosphereBufferDat
 // ---------------------------------------------------------------------------
 // Aerial perspective (AtmosphereRenderer.glsl:191-243)
 // ---------------------------------------------------------------------------
-// kApLanes consecutive lanes share a froxel: each integrates a contiguous part of its float
-// counter's steps (lane p takes steps [p c, (p + 1) c), c = ceil(n / kApLanes), starting from the
-// counter value the shader's loop reaches there), and the first lane combines the parts in
-// order, L = L0 + Tr0 (L1 + Tr1 (L2 + Tr2 L3)), Tr = Tr0 Tr1 Tr2 Tr3.  A 32^3 LUT is 32,768
-// froxels: one lane each would leave half the SIMDs idle and the rest one wave deep.
+// kApLanes consecutive lanes share a froxel (scattered_luminance_split): a 32^3 LUT is 32,768
+// froxels, and one lane each would leave half the SIMDs idle and the rest one wave deep.
 constexpr int kApLanes = 4;
 
 __global__ __launch_bounds__(256) void aerial_perspective_kernel(
@@ -77,35 +74,12 @@ __global__ __launch_bounds__(256) void aerial_perspective_kernel(
     const RenderDev R = make_render(rbd);
     const AerialRay ray = aerial_ray(A, R, rbd.inv_view_projection, blue_noise, bn_size, t % w, (t / w) % h,
                                      t / (w * h), w, h, d);
-    F3 L = f3(0.0f), Tr = f3(1.0f);
-    if (ray.marching_distance > 0.0f) {
-        const float steps = R.ap_steps;
-        int n = 0;  // iterations of the shader's loop `for (i = start_i; i < steps; i += 1)`
-        float i0 = ray.start_i;
-        for (float i = ray.start_i; i < steps; i += 1.0f) ++n;
-        const int c = (n + kApLanes - 1) / kApLanes, first = part * c;
-        for (int k = 0; k < first && k < n; ++k) i0 += 1.0f;
-        const int count = max(0, min(c, n - first));
-        const ScatterRay sc = scatter_ray<false>(A, ray.s, ray.marching_distance, steps);
-        scattered_luminance_part(A, T, tw, th, MS, mw, mh, ray.s, sc, i0, count, L, Tr);
-    }
-    // Combine from the last part down: acc = L_p + Tr_p acc.
-    F3 accL = L, accT = Tr;
-    for (int p = kApLanes - 1; p >= 1; --p) {
-        const F3 Lp = {__shfl_down(L.x, p, 64), __shfl_down(L.y, p, 64), __shfl_down(L.z, p, 64)};
-        const F3 Tp = {__shfl_down(Tr.x, p, 64), __shfl_down(Tr.y, p, 64), __shfl_down(Tr.z, p, 64)};
-        if (p == kApLanes - 1) {
-            accL = Lp;
-            accT = Tp;
-        } else {
-            accL = Lp + Tp * accL;
-            accT = Tp * accT;
-        }
-    }
+    F3 L, Tr;
+    scattered_luminance_split<kApLanes>(A, T, tw, th, MS, mw, mh, ray.s, ray.marching_distance, R.ap_steps,
+                                        ray.start_i, ray.marching_distance > 0.0f, part, L, Tr);
     if (part == 0 && valid) {
-        const F3 Lo = L + Tr * accL, To = Tr * accT;
-        out_l[t] = make_float4(Lo.x, Lo.y, Lo.z, 0.0f);
-        out_t[t] = make_float4(To.x, To.y, To.z, 0.0f);
+        out_l[t] = make_float4(L.x, L.y, L.z, 0.0f);
+        out_t[t] = make_float4(Tr.x, Tr.y, Tr.z, 0.0f);
     }
 }
 

@@ -78,6 +78,15 @@ __host__ __device__ inline AtmDev make_atm(const SkyhipAtmosphereBufferData& b) 
 __device__ __forceinline__ float fdiv_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fsqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// 1 - exp(-x) for x >= 0 given e = exp(-x): below x = 1/8 the series x - x^2/2 + ... + x^5/120
+// (truncation < 1e-8 relative), which keeps the relative precision that 1 - e loses to
+// cancellation for small x; above, 1 - e (cancellation costs at most 1/(1 - e^-1/8) ~ 8.5x
+// of e's rounding).
+__device__ __forceinline__ float one_minus_exp(float x, float e) {
+    const float series = x * (1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f - x * (1.0f / 120.0f)))));
+    return x < 0.125f ? series : 1.0f - e;
+}
+
 // Atmosphere.glsl:119-132 / :156-159 split into densities so the exps are shared.
 struct Densities {
     float rayleigh, mie, ozone;
@@ -202,7 +211,9 @@ __device__ __forceinline__ void scatter_step(const AtmDev& A, const float4* __re
     F3 sr = A.rayleigh * dn.rayleigh;
     F3 sm = A.mie_s * dn.mie;
     F3 ext = sr + A.mie_ext * dn.mie + A.ozone * dn.ozone;
-    F3 one_minus_t = {-expm1f(-ext.x * dx), -expm1f(-ext.y * dx), -expm1f(-ext.z * dx)};
+    const F3 tr = {__expf(-ext.x * dx), __expf(-ext.y * dx), __expf(-ext.z * dx)};
+    F3 one_minus_t = {one_minus_exp(ext.x * dx, tr.x), one_minus_exp(ext.y * dx, tr.y),
+                      one_minus_exp(ext.z * dx, tr.z)};
     F3 Li = (sr * c.rp + sm * c.mp) * sun_visibility(A, T, tw, th, r_d, q, alt, mu_s);
     F3 inv_ext = {__builtin_amdgcn_rcpf(ext.x), __builtin_amdgcn_rcpf(ext.y), __builtin_amdgcn_rcpf(ext.z)};
     if (!kMS) {
@@ -211,7 +222,7 @@ __device__ __forceinline__ void scatter_step(const AtmDev& A, const float4* __re
     }
     L += Tr * Li * one_minus_t * inv_ext;
     if (kMS) Lf += Tr * (sr + sm) * one_minus_t * inv_ext;
-    Tr *= F3{__expf(-ext.x * dx), __expf(-ext.y * dx), __expf(-ext.z * dx)};
+    Tr *= tr;
 }
 
 // ComputeScatteredLuminance (Atmosphere.glsl:220-295).  kMS selects the
@@ -242,6 +253,47 @@ __device__ inline void scattered_luminance_part(const AtmDev& A, const float4* _
     F3 Lf = f3(0.0f);
     float i = i0;
     for (int k = 0; k < count; ++k, i += 1.0f) scatter_step<false>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
+}
+
+// scattered_luminance<false> spread over kParts consecutive lanes of a wave (kParts a power of
+// two dividing 64; part = lane index within the group).  Part p integrates steps
+// [p c, (p + 1) c), c = ceil(n / kParts), of the shader's float counter (n its trip count,
+// i0 the counter value the loop reaches at step p c), and part 0 combines the parts in order:
+// L = L0 + Tr0 (L1 + Tr1 (...)), Tr = Tr0 Tr1 ...  All lanes of a group must call this
+// together (it shuffles); `active` (marching_distance > 0) must be the same across a group.
+// The result is valid on part 0.
+template <int kParts>
+__device__ inline void scattered_luminance_split(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                                 const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
+                                                 float marching_distance, float steps, float start_i, bool active,
+                                                 int part, F3& L, F3& Tr) {
+    L = f3(0.0f);
+    Tr = f3(1.0f);
+    if (active) {
+        int n = 0;
+        for (float i = start_i; i < steps; i += 1.0f) ++n;
+        const int c = (n + kParts - 1) / kParts, first = part * c;
+        float i0 = start_i;
+        for (int k = 0; k < first && k < n; ++k) i0 += 1.0f;
+        const ScatterRay sc = scatter_ray<false>(A, s, marching_distance, steps);
+        scattered_luminance_part(A, T, tw, th, MS, mw, mh, s, sc, i0, max(0, min(c, n - first)), L, Tr);
+    }
+    F3 accL = f3(0.0f), accT = f3(1.0f);  // the parts after this lane's, combined
+    for (int p = kParts - 1; p >= 1; --p) {
+        const F3 Lp = {__shfl_down(L.x, p, 64), __shfl_down(L.y, p, 64), __shfl_down(L.z, p, 64)};
+        const F3 Tp = {__shfl_down(Tr.x, p, 64), __shfl_down(Tr.y, p, 64), __shfl_down(Tr.z, p, 64)};
+        if (p == kParts - 1) {
+            accL = Lp;
+            accT = Tp;
+        } else {
+            accL = Lp + Tp * accL;
+            accT = Tp * accT;
+        }
+    }
+    if (kParts > 1) {
+        L = L + Tr * accL;
+        Tr = Tr * accT;
+    }
 }
 
 }  // namespace skyhip
