@@ -54,8 +54,8 @@ FLOP_PER_PIXEL = 260            # unprojection, shell intersection, composite wi
 FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
-    "cloud_raymarch_kernel<1,false>": (7355.3, 24303.0),
-    "path_trace_kernel<4,true,false>": (4.5604e6, 1.9302e4),
+    "cloud_raymarch_kernel<1,false>": (8141.5, 27958.8),
+    "path_trace_kernel<4,true,false>": (5.3842e6, 2.0778e5),
 }
 
 
