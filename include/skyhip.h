@@ -903,20 +903,24 @@ int skyhip_surrogate_parameter_count(void);
  * term, Atmosphere.glsl:220-295, along the query's ray): out_rgb device float[3 * count]. */
 int skyhip_sky_radiance(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyQuery* queries, int32_t count,
                         float* out_rgb);
-/* Surrogate inputs (8 floats per query, the 8th 0): 2 sqrt(altitude / thickness) - 1,
- * mu_view, mu_sun, 2 delta_phi / pi - 1, and the atmosphere's natural-log Rayleigh, Mie and
- * ozone scale factors (log_scales: device float[3 * atmospheres]) divided by ln 4. */
+/* Surrogate inputs (8 floats per query, the 8th 0): 2 sqrt(altitude / thickness) - 1; the
+ * view zenith as the sky-view LUT's latitude coordinate x_lat (square-root spacing from the
+ * horizon on both sides, GetSkyViewTextureUvFromCosLatLon), 2 x_lat - 1; mu_sun; the
+ * sun-view cosine mu_view mu_sun + sin_view sin_sun cos(delta_phi); and the atmosphere's
+ * natural-log Rayleigh, Mie and ozone scale factors (log_scales: device
+ * float[3 * atmospheres]) divided by ln 4. */
 int skyhip_surrogate_features(SkyhipContext* ctx, const SkyhipSkyQuery* queries, int32_t count,
                               const float* log_scales, int32_t atmosphere_count, float thickness,
                               float* out_features);
-/* Inference (BF16 MFMA, FP32 accumulate): out device float[3 * count] log radiance. */
+/* Inference (FP16 MFMA operands, FP32 accumulate, packed FP16 activations): out device
+ * float[3 * count] log radiance. */
 int skyhip_surrogate_forward(SkyhipContext* ctx, const float* params, const float* features, int32_t count,
                              float* out_log_radiance);
 /* Workspace of skyhip_surrogate_gradient for `count` samples. */
 size_t skyhip_surrogate_workspace_bytes(int32_t count);
 /* Gradient of loss = loss_scale * sum_samples w * sum_rgb (f(x) - target)^2 over `count`
  * samples (targets: device float[3 * count] log radiance; weights: device float[count], or
- * NULL for w = 1) by the BF16 MFMA forward and backward kernels; out_grad: device
+ * NULL for w = 1) by the FP16 MFMA forward and backward kernels; out_grad: device
  * float[13187]; out_loss: device float (may be NULL). */
 int skyhip_surrogate_gradient(SkyhipContext* ctx, const float* params, const float* features, const float* targets,
                               const float* weights, int32_t count, float loss_scale, float* out_grad,

@@ -1,26 +1,31 @@
 // Learned sky-view surrogate for gfx950: a 7 -> 64 -> 64 -> 64 -> 64 -> 3 MLP with
-// SiLU activations predicting RGB log-radiance of the sky along a view direction,
-// with BF16 MFMA operands and FP32 accumulation, trained on the GPU by the kernels
-// below and checked at run time against the plain integrator it replaces.
+// smooth-step gated activations z * smoothstep(-3, 3, z) predicting RGB log-radiance of
+// the sky along a view direction, with FP16 MFMA operands and FP32 accumulation, trained
+// on the GPU by the kernels below and checked at run time against the plain integrator
+// it replaces.
 //
 // Parameters (FP32 master copy, 13187 floats): for each layer l with fan-in K and
 // fan-out N, W_l as N x K row-major (out-major), then b_l[N]; layers in order.
-// Inputs (8 floats per sample, the 8th zero): 2 sqrt(h / H) - 1, mu_view, mu_sun,
-// 2 dphi / pi - 1, log(rayleigh scale) / log 4, log(mie scale) / log 4,
-// log(ozone scale) / log 4.  Output: log of the RGB radiance (solar illuminance units).
+// Inputs (8 floats per sample, the 8th zero): 2 sqrt(h / H) - 1; the view zenith as the
+// sky-view LUT's latitude coordinate, 2 x_lat - 1; mu_sun; the sun-view cosine
+// mu_view mu_sun + sin_view sin_sun cos(dphi); log(rayleigh scale) / log 4,
+// log(mie scale) / log 4, log(ozone scale) / log 4.  Output: log of the RGB radiance
+// (solar illuminance units).
 //
 // Kernels:
 //  - sky_radiance_kernel: the plain integrator per query (label generator and judge):
 //    ComputeScatteredLuminance with the multiple-scattering term along the view ray
 //    from altitude h, exactly the sky-view texel arithmetic for an explicit direction.
 //  - surrogate_forward_kernel: inference.  One wave holds every weight in registers as
-//    MFMA A operands and streams groups of 16 samples through v_mfma_f32_16x16x32_bf16;
+//    MFMA A operands and streams groups of 16 samples through v_mfma_f32_16x16x32_f16;
 //    a layer's accumulators (rows = neurons, columns = samples) become the next layer's
 //    B operand in place, the k order inside each 32-wide block being permuted to match
 //    the accumulator layout (the weights are loaded with the same permutation).
 //  - surrogate_train_kernel: forward and backward for 32 samples per workgroup with
-//    LDS-staged BF16 operands (any transpose is an LDS stride), writing the workgroup's
-//    gradient to its own slice (no atomics; the reduction order is fixed).
+//    LDS-staged FP16 operands (any transpose is an LDS stride), writing the workgroup's
+//    gradient to its own slice (no atomics; the reduction order is fixed).  The deltas
+//    are kept without the loss scale (2 w (y - t) and its back-propagation, well inside
+//    FP16's range) and the scale multiplies the gradient sums.
 //  - surrogate_reduce_kernel / surrogate_adam_kernel: gradient sum and Adam update.
 //  - sky render kernels: per-pixel panorama evaluation (surrogate), a per-tile residual
 //    check against the plain integrator at probe pixels, and the plain fallback for
@@ -32,20 +37,37 @@
 
 namespace skyhip {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kParams = 13187;
 constexpr int kOffW1 = 0, kOffB1 = 448, kOffW2 = 512, kOffB2 = 4608, kOffW3 = 4672, kOffB3 = 8768, kOffW4 = 8832,
               kOffB4 = 12928, kOffW5 = 12992, kOffB5 = 13184;
 
-// SiLU z * sigmoid(z) with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, each
-// ~1 ulp): the same function in training and inference, well inside BF16 operand rounding.
-__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
-__device__ __forceinline__ float silu_grad(float z) {
-    float s = sigmoid(z);
-    return s * (1.0f + z * (1.0f - s));
+// The activation a(z) = z s(t), t = clamp(z / 6 + 1/2, 0, 1), s = t^2 (3 - 2 t): a gate that
+// rises smoothly from 0 at z = -3 to 1 at z = 3 (a C1 function, like z sigmoid(z) in shape),
+// built from multiplies and adds only.  Evaluated on two values at once in packed FP16, in
+// this order, each operation rounding once to FP16:
+//   z = FP16(z) toward zero (v_cvt_pkrtz_f16_f32), t = clamp(fma(z, FP16(1/6), 1/2), 0, 1),
+//   v = fma(t, -2, 3), w = t v, s = t w, a = z s.
+// The same function feeds the next layer in training and in inference.
+__device__ __forceinline__ f16x2 activation2(float z0, float z1) {
+    const f16x2 z = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(z0, z1));
+    const f16x2 sixth = {(_Float16)(1.0f / 6.0f), (_Float16)(1.0f / 6.0f)}, half = {0.5f16, 0.5f16};
+    const f16x2 zero = {0.0f16, 0.0f16}, one = {1.0f16, 1.0f16}, m2 = {-2.0f16, -2.0f16}, three = {3.0f16, 3.0f16};
+    f16x2 t = __builtin_elementwise_fma(z, sixth, half);
+    t = __builtin_elementwise_min(__builtin_elementwise_max(t, zero), one);
+    const f16x2 v = __builtin_elementwise_fma(t, m2, three);
+    const f16x2 w = t * v;
+    const f16x2 sg = t * w;
+    return z * sg;
+}
+__device__ __forceinline__ _Float16 activation1(float z) { return activation2(z, 0.0f)[0]; }
+// da/dz = s + z t (1 - t) (ds/dz = 6 t (1 - t) / 6), in FP32 from the FP32 pre-activation.
+__device__ __forceinline__ float activation_grad(float z) {
+    const float t = fminf(fmaxf(z * (1.0f / 6.0f) + 0.5f, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t) + z * t * (1.0f - t);
 }
 
 // Sky radiance along (mu_view, dphi) from altitude h for sun zenith cosine mu_sun.
@@ -91,34 +113,69 @@ __global__ __launch_bounds__(256) void sky_radiance_kernel(SkyLuts L, const Skyh
     out_rgb[3 * i + 2] = v.z;
 }
 
+// The surrogate inputs in three parts, so that a panorama computes each part once per
+// view, per row and per pixel while a single query computes all three: the functions
+// below are the only definition of the inputs, which keeps the two paths bit-identical.
+//
+// The view zenith and azimuth enter in the sky-view LUT's coordinates
+// (GetSkyViewTextureUvFromCosLatLon, AtmosphereRenderer.glsl:113-132): latitude measured
+// from the horizon with square-root spacing on both sides; the relative azimuth enters as
+// the sun-view cosine it implies (monotonic in delta_phi for fixed zenith angles, the
+// argument of both phase functions).  The radiance is smooth in them.
+struct FeatureView {      // camera and atmosphere
+    float f0, f2, f4, f5, f6;  // altitude, mu_sun, log scales / ln 4
+    float horizon_up, horizon_down, sin_s;
+};
+struct FeatureRow {       // one view zenith of one view
+    float f1, mu_v, sin_v;
+};
+
+__device__ inline FeatureView feature_view(int atmosphere, float altitude, float mu_sun,
+                                           const float* __restrict__ log_scales, float thickness, int count) {
+    const float inv_log4 = 0.72134752044448170368f;  // 1 / ln 4
+    const int a = clamp_index(atmosphere, count);
+    FeatureView v;
+    v.f0 = 2.0f * sqrtf(fmaxf(altitude / thickness, 0.0f)) - 1.0f;
+    const float rb = 6360.0f;  // the preset's ground radius (the horizon angle's scale)
+    const float h = fmaxf(altitude, 0.0f), r = rb + h;
+    v.horizon_down = acosf(sqrtf(h * (r + rb)) / r);
+    v.horizon_up = kPiF - v.horizon_down;
+    v.f2 = mu_sun;
+    v.sin_s = sqrtf(fmaxf(1.0f - mu_sun * mu_sun, 0.0f));
+    v.f4 = log_scales[3 * a + 0] * inv_log4;
+    v.f5 = log_scales[3 * a + 1] * inv_log4;
+    v.f6 = log_scales[3 * a + 2] * inv_log4;
+    return v;
+}
+
+__device__ inline FeatureRow feature_row(const FeatureView& v, float mu_view) {
+    FeatureRow r;
+    const float lat = acosf(fminf(fmaxf(mu_view, -1.0f), 1.0f));
+    const float x_lat = lat < v.horizon_up ? 0.5f - 0.5f * sqrtf(fmaxf(1.0f - lat / v.horizon_up, 0.0f))
+                                           : 0.5f + 0.5f * sqrtf(fmaxf((lat - v.horizon_up) / v.horizon_down, 0.0f));
+    r.f1 = 2.0f * x_lat - 1.0f;
+    r.mu_v = mu_view;
+    r.sin_v = sqrtf(fmaxf(1.0f - mu_view * mu_view, 0.0f));
+    return r;
+}
+
+// The 7 inputs (the 8th is 0) from the view and row parts and cos(delta_phi).
+__device__ inline void features_from(const FeatureView& v, const FeatureRow& r, float cos_dphi, float f[8]) {
+    f[0] = v.f0;
+    f[1] = r.f1;
+    f[2] = v.f2;
+    f[3] = r.mu_v * v.f2 + r.sin_v * v.sin_s * cos_dphi;
+    f[4] = v.f4;
+    f[5] = v.f5;
+    f[6] = v.f6;
+    f[7] = 0.0f;
+}
+
 // The 7 surrogate inputs of a query (the 8th is 0).
 __device__ inline void features(const SkyhipSkyQuery& q, const float* __restrict__ log_scales, float thickness,
                                 int count, float f[8]) {
-    const float inv_log4 = 0.72134752044448170368f;  // 1 / ln 4
-    const int a = clamp_index(q.atmosphere, count);
-    f[0] = 2.0f * sqrtf(fmaxf(q.altitude / thickness, 0.0f)) - 1.0f;
-    // The view zenith and azimuth enter in the sky-view LUT's coordinates
-    // (GetSkyViewTextureUvFromCosLatLon, AtmosphereRenderer.glsl:113-132): latitude
-    // measured from the horizon with square-root spacing on both sides, longitude
-    // sqrt(0.5 - 0.5 cos dphi); the radiance is smooth in them.
-    const float rb = 6360.0f;  // the preset's ground radius (the horizon angle's scale)
-    const float h = fmaxf(q.altitude, 0.0f), r = rb + h;
-    const float horizon_down = acosf(sqrtf(h * (r + rb)) / r);
-    const float horizon_up = kPiF - horizon_down;
-    const float lat = acosf(fminf(fmaxf(q.mu_view, -1.0f), 1.0f));
-    const float x_lat = lat < horizon_up ? 0.5f - 0.5f * sqrtf(fmaxf(1.0f - lat / horizon_up, 0.0f))
-                                         : 0.5f + 0.5f * sqrtf(fmaxf((lat - horizon_up) / horizon_down, 0.0f));
-    f[1] = 2.0f * x_lat - 1.0f;
-    f[2] = q.mu_sun;
-    // The relative azimuth as the sun-view cosine it implies (monotonic in delta_phi for
-    // fixed zenith angles), the argument of both phase functions.
-    const float sv = sqrtf(fmaxf(1.0f - q.mu_view * q.mu_view, 0.0f)),
-                ss = sqrtf(fmaxf(1.0f - q.mu_sun * q.mu_sun, 0.0f));
-    f[3] = q.mu_view * q.mu_sun + sv * ss * cosf(q.delta_phi);
-    f[4] = log_scales[3 * a + 0] * inv_log4;
-    f[5] = log_scales[3 * a + 1] * inv_log4;
-    f[6] = log_scales[3 * a + 2] * inv_log4;
-    f[7] = 0.0f;
+    const FeatureView v = feature_view(q.atmosphere, q.altitude, q.mu_sun, log_scales, thickness, count);
+    features_from(v, feature_row(v, q.mu_view), cosf(q.delta_phi), f);
 }
 
 __global__ __launch_bounds__(256) void features_kernel(const SkyhipSkyQuery* __restrict__ q, int n,
@@ -141,9 +198,9 @@ __device__ __forceinline__ int permuted_neuron(int b, int q, int i) {
 }
 
 struct MlpRegs {
-    bf16x8 w1[4];
-    bf16x8 wh[3][4][2];
-    bf16x8 w5[2];
+    f16x8 w1[4];
+    f16x8 wh[3][4][2];
+    f16x8 w5[2];
 };
 
 __device__ inline void load_mlp(const float* __restrict__ p, MlpRegs& R, int lane) {
@@ -151,51 +208,57 @@ __device__ inline void load_mlp(const float* __restrict__ p, MlpRegs& R, int lan
     for (int j = 0; j < 4; ++j)
         for (int i = 0; i < 8; ++i) {
             int k = 8 * q + i;
-            R.w1[j][i] = (__bf16)(k < 7 ? p[kOffW1 + (16 * j + r) * 7 + k] : 0.0f);
+            R.w1[j][i] = (_Float16)(k < 7 ? p[kOffW1 + (16 * j + r) * 7 + k] : 0.0f);
         }
     const int offs[3] = {kOffW2, kOffW3, kOffW4};
     for (int L = 0; L < 3; ++L)
         for (int j = 0; j < 4; ++j)
             for (int b = 0; b < 2; ++b)
                 for (int i = 0; i < 8; ++i)
-                    R.wh[L][j][b][i] = (__bf16)p[offs[L] + (16 * j + r) * 64 + permuted_neuron(b, q, i)];
+                    R.wh[L][j][b][i] = (_Float16)p[offs[L] + (16 * j + r) * 64 + permuted_neuron(b, q, i)];
     for (int b = 0; b < 2; ++b)
-        for (int i = 0; i < 8; ++i) R.w5[b][i] = (__bf16)(r < 3 ? p[kOffW5 + r * 64 + permuted_neuron(b, q, i)] : 0.0f);
+        for (int i = 0; i < 8; ++i)
+            R.w5[b][i] = (_Float16)(r < 3 ? p[kOffW5 + r * 64 + permuted_neuron(b, q, i)] : 0.0f);
 }
 
-// B operands of the next layer from the four accumulator tiles (bias added, SiLU).
-__device__ __forceinline__ void activate(const f32x4 c[4], bf16x8 h[2]) {
+// B operands of the next layer from the four accumulator tiles (bias included): the
+// activation of accumulator items (2 i, 2 i + 1), two at a time.
+__device__ __forceinline__ void activate(const f32x4 c[4], f16x8 h[2]) {
     for (int b = 0; b < 2; ++b)
-        for (int i = 0; i < 4; ++i) {
-            h[b][i] = (__bf16)silu(c[2 * b][i]);
-            h[b][4 + i] = (__bf16)silu(c[2 * b + 1][i]);
+        for (int i = 0; i < 2; ++i) {
+            const f16x2 lo = activation2(c[2 * b][2 * i], c[2 * b][2 * i + 1]);
+            const f16x2 hi = activation2(c[2 * b + 1][2 * i], c[2 * b + 1][2 * i + 1]);
+            h[b][2 * i] = lo[0];
+            h[b][2 * i + 1] = lo[1];
+            h[b][4 + 2 * i] = hi[0];
+            h[b][4 + 2 * i + 1] = hi[1];
         }
 }
 
 // Forward pass of 16 samples (the lane's column); x: this lane's input operand.
 // Returns the output accumulator (lanes 0..15 hold rows 0..2 in items 0..2).
-__device__ __forceinline__ f32x4 mlp_forward16(const MlpRegs& R, const float* s_bias, bf16x8 x, int q) {
+__device__ __forceinline__ f32x4 mlp_forward16(const MlpRegs& R, const float* s_bias, f16x8 x, int q) {
     f32x4 c[4];
-    bf16x8 h[2];
+    f16x8 h[2];
     for (int j = 0; j < 4; ++j) {
         const float* b = s_bias + 16 * j + 4 * q;
         c[j] = f32x4{b[0], b[1], b[2], b[3]};
-        c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[j], x, c[j], 0, 0, 0);
+        c[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.w1[j], x, c[j], 0, 0, 0);
     }
     activate(c, h);
     for (int L = 0; L < 3; ++L) {
         for (int j = 0; j < 4; ++j) {
             const float* b = s_bias + 64 * (L + 1) + 16 * j + 4 * q;
             c[j] = f32x4{b[0], b[1], b[2], b[3]};
-            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.wh[L][j][0], h[0], c[j], 0, 0, 0);
-            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.wh[L][j][1], h[1], c[j], 0, 0, 0);
+            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.wh[L][j][0], h[0], c[j], 0, 0, 0);
+            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.wh[L][j][1], h[1], c[j], 0, 0, 0);
         }
         activate(c, h);
     }
     const float* b = s_bias + 256 + 4 * q;
     f32x4 y = {b[0], b[1], b[2], b[3]};
-    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w5[0], h[0], y, 0, 0, 0);
-    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w5[1], h[1], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.w5[0], h[0], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.w5[1], h[1], y, 0, 0, 0);
     return y;
 }
 
@@ -213,9 +276,9 @@ __device__ inline void load_biases(const float* __restrict__ p, float* s_bias) {
     __syncthreads();
 }
 
-__device__ __forceinline__ bf16x8 input_operand(const float* f, bool valid, int q) {
-    bf16x8 x;
-    for (int i = 0; i < 8; ++i) x[i] = (__bf16)((q == 0 && valid) ? f[i] : 0.0f);
+__device__ __forceinline__ f16x8 input_operand(const float* f, bool valid, int q) {
+    f16x8 x;
+    for (int i = 0; i < 8; ++i) x[i] = (_Float16)((q == 0 && valid) ? f[i] : 0.0f);
     return x;
 }
 
@@ -247,11 +310,11 @@ __global__ __launch_bounds__(256, 2) void surrogate_forward_kernel(const float* 
 // ---------------------------------------------------------------------------
 constexpr int kTS = 32;  // samples per workgroup
 
-// C[M x N] = A[M x K] B[K x N] with BF16 LDS operands at element strides
+// C[M x N] = A[M x K] B[K x N] with FP16 LDS operands at element strides
 // A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn]; 16 x 16 tiles spread over
 // the workgroup's waves, K a multiple of 32.  ep(m, n, value) consumes each output once.
 template <class Ep>
-__device__ inline void lds_gemm(const __bf16* A, int sam, int sak, const __bf16* B, int sbk, int sbn, int M, int N,
+__device__ inline void lds_gemm(const _Float16* A, int sam, int sak, const _Float16* B, int sbk, int sbn, int M, int N,
                                 int K, Ep ep) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     const int q = lane >> 4, r = lane & 15;
@@ -260,28 +323,28 @@ __device__ inline void lds_gemm(const __bf16* A, int sam, int sak, const __bf16*
         const int m0 = (t / tn_count) * 16, n0 = (t % tn_count) * 16;
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int k0 = 0; k0 < K; k0 += 32) {
-            bf16x8 a, b;
+            f16x8 a, b;
             for (int i = 0; i < 8; ++i) {
                 const int k = k0 + 8 * q + i;
                 a[i] = A[(m0 + r) * sam + k * sak];
                 b[i] = B[k * sbk + (n0 + r) * sbn];
             }
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
         }
         for (int i = 0; i < 4; ++i) ep(m0 + 4 * q + i, n0 + r, c[i]);
     }
 }
 
 struct TrainLds {
-    __bf16 w1[64 * 32];   // [neuron][input] (inputs 7..31 zero)
-    __bf16 wh[3][64 * 64];
-    __bf16 w5[32 * 64];   // [output][neuron] (outputs 3..31 zero)
-    __bf16 x[32 * kTS];   // [input][sample]
+    _Float16 w1[64 * 32];   // [neuron][input] (inputs 7..31 zero)
+    _Float16 wh[3][64 * 64];
+    _Float16 w5[32 * 64];   // [output][neuron] (outputs 3..31 zero)
+    _Float16 x[32 * kTS];   // [input][sample]
     float z[4][64 * kTS]; // pre-activations [neuron][sample]
-    __bf16 a[4][64 * kTS];
+    _Float16 a[4][64 * kTS];
     float y[16 * kTS];
-    __bf16 d5[32 * kTS];  // output deltas [output][sample] (rows 3..31 zero)
-    __bf16 d[2][64 * kTS];
+    _Float16 d5[32 * kTS];  // output deltas 2 w (y - t) [output][sample] (rows 3..31 zero)
+    _Float16 d[2][64 * kTS];  // hidden deltas (without the loss scale)
     float bias[272];
     float loss;
 };
@@ -297,19 +360,19 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
     const int tid = threadIdx.x, nt = blockDim.x;
     const int s0 = blockIdx.x * kTS;
     float* G = partial + (long long)blockIdx.x * kParams;
-    // Weights to BF16 (zero padding), inputs, biases.
+    // Weights to FP16 (zero padding), inputs, biases.
     for (int i = tid; i < 64 * 32; i += nt) {
         int m = i / 32, k = i % 32;
-        S.w1[i] = (__bf16)(k < 7 ? params[kOffW1 + m * 7 + k] : 0.0f);
+        S.w1[i] = (_Float16)(k < 7 ? params[kOffW1 + m * 7 + k] : 0.0f);
     }
     const int offs[3] = {kOffW2, kOffW3, kOffW4};
     for (int L = 0; L < 3; ++L)
-        for (int i = tid; i < 64 * 64; i += nt) S.wh[L][i] = (__bf16)params[offs[L] + i];
-    for (int i = tid; i < 32 * 64; i += nt) S.w5[i] = (__bf16)(i < 3 * 64 ? params[kOffW5 + i] : 0.0f);
+        for (int i = tid; i < 64 * 64; i += nt) S.wh[L][i] = (_Float16)params[offs[L] + i];
+    for (int i = tid; i < 32 * 64; i += nt) S.w5[i] = (_Float16)(i < 3 * 64 ? params[kOffW5 + i] : 0.0f);
     for (int i = tid; i < 32 * kTS; i += nt) {
         int k = i / kTS, s = i % kTS;
-        S.x[i] = (__bf16)((k < 8 && s0 + s < n) ? feats[8 * (s0 + s) + k] : 0.0f);
-        S.d5[i] = (__bf16)0.0f;
+        S.x[i] = (_Float16)((k < 8 && s0 + s < n) ? feats[8 * (s0 + s) + k] : 0.0f);
+        S.d5[i] = (_Float16)0.0f;
     }
     if (tid == 0) S.loss = 0.0f;
     load_biases(params, S.bias);  // synchronises
@@ -317,21 +380,22 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
     lds_gemm(S.w1, 32, 1, S.x, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
         float z = c + S.bias[m];
         S.z[0][m * kTS + s] = z;
-        S.a[0][m * kTS + s] = (__bf16)silu(z);
+        S.a[0][m * kTS + s] = activation1(z);
     });
     __syncthreads();
     for (int L = 0; L < 3; ++L) {
         lds_gemm(S.wh[L], 64, 1, S.a[L], kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
             float z = c + S.bias[64 * (L + 1) + m];
             S.z[L + 1][m * kTS + s] = z;
-            S.a[L + 1][m * kTS + s] = (__bf16)silu(z);
+            S.a[L + 1][m * kTS + s] = activation1(z);
         });
         __syncthreads();
     }
     lds_gemm(S.w5, 64, 1, S.a[3], kTS, 1, 16, kTS, 64,
              [&](int m, int s, float c) { S.y[m * kTS + s] = c + S.bias[256 + m]; });
     __syncthreads();
-    // Loss loss_scale * sum w (y - t)^2 and output deltas.
+    // Loss loss_scale * sum w (y - t)^2 and the output deltas 2 w (y - t); the gradient
+    // sums below multiply by loss_scale.
     for (int i = tid; i < 3 * kTS; i += nt) {
         int m = i / kTS, s = i % kTS;
         float d = 0.0f, w = 0.0f;
@@ -340,53 +404,53 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
             w = weights ? weights[s0 + s] : 1.0f;
         }
         atomicAdd(&S.loss, w * d * d * loss_scale);
-        S.d5[m * kTS + s] = (__bf16)(2.0f * w * d * loss_scale);
+        S.d5[m * kTS + s] = (_Float16)(2.0f * w * d);
     }
     __syncthreads();
     // Output layer gradient: dW5 = d5 a4^T, db5 = sum d5.
     lds_gemm(S.d5, kTS, 1, S.a[3], 1, kTS, 16, 64, kTS, [&](int m, int k, float c) {
-        if (m < 3) G[kOffW5 + m * 64 + k] = c;
+        if (m < 3) G[kOffW5 + m * 64 + k] = c * loss_scale;
     });
     for (int m = tid; m < 3; m += nt) {
         float sum = 0.0f;
         for (int s = 0; s < kTS; ++s) sum += float(S.d5[m * kTS + s]);
-        G[kOffB5 + m] = sum;
+        G[kOffB5 + m] = sum * loss_scale;
     }
-    // delta4 = (W5^T d5) * silu'(z4)
+    // delta4 = (W5^T d5) * a'(z4)
     lds_gemm(S.w5, 1, 64, S.d5, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
-        S.d[0][m * kTS + s] = (__bf16)(c * silu_grad(S.z[3][m * kTS + s]));
+        S.d[0][m * kTS + s] = (_Float16)(c * activation_grad(S.z[3][m * kTS + s]));
     });
     __syncthreads();
     // Hidden layers 4, 3, 2 (weights wh[2], wh[1], wh[0]).
     const int woffs[3] = {kOffW2, kOffW3, kOffW4}, boffs[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
     int cur = 0;
     for (int L = 2; L >= 0; --L) {
-        const __bf16* dl = S.d[cur];
+        const _Float16* dl = S.d[cur];
         // dW = delta a_prev^T ; a_prev = a[L] (input of wh[L])
         lds_gemm(dl, kTS, 1, S.a[L], 1, kTS, 64, 64, kTS,
-                 [&](int m, int k, float c) { G[woffs[L] + m * 64 + k] = c; });
+                 [&](int m, int k, float c) { G[woffs[L] + m * 64 + k] = c * loss_scale; });
         for (int m = tid; m < 64; m += nt) {
             float sum = 0.0f;
             for (int s = 0; s < kTS; ++s) sum += float(dl[m * kTS + s]);
-            G[boffs[L + 1] + m] = sum;
+            G[boffs[L + 1] + m] = sum * loss_scale;
         }
-        // delta_prev = (W^T delta) * silu'(z[L])
-        __bf16* dn = S.d[cur ^ 1];
+        // delta_prev = (W^T delta) * a'(z[L])
+        _Float16* dn = S.d[cur ^ 1];
         lds_gemm(S.wh[L], 1, 64, dl, kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
-            dn[m * kTS + s] = (__bf16)(c * silu_grad(S.z[L][m * kTS + s]));
+            dn[m * kTS + s] = (_Float16)(c * activation_grad(S.z[L][m * kTS + s]));
         });
         __syncthreads();
         cur ^= 1;
     }
     // First layer: dW1 = delta1 x^T, db1.
-    const __bf16* d1 = S.d[cur];
+    const _Float16* d1 = S.d[cur];
     lds_gemm(d1, kTS, 1, S.x, 1, kTS, 64, 32, kTS, [&](int m, int k, float c) {
-        if (k < 7) G[kOffW1 + m * 7 + k] = c;
+        if (k < 7) G[kOffW1 + m * 7 + k] = c * loss_scale;
     });
     for (int m = tid; m < 64; m += nt) {
         float sum = 0.0f;
         for (int s = 0; s < kTS; ++s) sum += float(d1[m * kTS + s]);
-        G[kOffB1 + m] = sum;
+        G[kOffB1 + m] = sum * loss_scale;
     }
     __syncthreads();
     if (tid == 0) partial_loss[blockIdx.x] = S.loss;
@@ -453,40 +517,94 @@ struct SkyRenderArgs {
 
 // Query of pixel (x, y) of an equirectangular panorama around the camera's zenith:
 // zenith angle pi (y + 0.5) / H, azimuth from the sun 2 pi (x + 0.5) / W - pi.
+__device__ __forceinline__ float panorama_mu_view(int y, int h) { return cosf(kPiF * (float(y) + 0.5f) / float(h)); }
+__device__ __forceinline__ float panorama_delta_phi(int x, int w) {
+    return fabsf(2.0f * kPiF * (float(x) + 0.5f) / float(w) - kPiF);
+}
 __device__ inline SkyhipSkyQuery panorama_query(const SkyhipSkyView& v, int x, int y, int w, int h) {
     SkyhipSkyQuery q;
     q.atmosphere = v.atmosphere;
     q.altitude = v.altitude;
     q.mu_sun = v.mu_sun;
-    q.mu_view = cosf(kPiF * (float(y) + 0.5f) / float(h));
-    q.delta_phi = fabsf(2.0f * kPiF * (float(x) + 0.5f) / float(w) - kPiF);
+    q.mu_view = panorama_mu_view(y, h);
+    q.delta_phi = panorama_delta_phi(x, w);
     q.pad[0] = q.pad[1] = q.pad[2] = 0.0f;
     return q;
 }
 
-// Surrogate pass: each wave renders 16-pixel runs of one row (lanes 0..15 own a pixel each).
+// Surrogate pass.  Each wave owns a contiguous range of 64-pixel runs (row-major over the
+// views' pixels) and takes them in order: every lane computes its pixel's network inputs
+// (the view and row parts only when its view or row changes, which for widths that are
+// multiples of 64 happens for all lanes together once per row) and stages them in LDS as
+// FP16; the four 16-sample column blocks then run through the network (lane group 0
+// supplies the input operand of each block); the log radiance goes back through LDS so
+// that every lane exponentiates and stores its own pixel.  The staging is private to the
+// wave, so no workgroup barrier is needed.
 __global__ __launch_bounds__(256, 2) void sky_render_surrogate_kernel(const float* __restrict__ params,
                                                                    SkyRenderArgs a, int n_views) {
     __shared__ float s_bias[272];
+    __shared__ f16x8 s_in[4][64];
+    __shared__ float4 s_y[4][64];
     load_biases(params, s_bias);
-    const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
     MlpRegs R;
     load_mlp(params, R, lane);
-    const long long per_view = (long long)a.width * a.height;
-    const long long groups = (per_view * n_views + 15) / 16;
-    const int waves = gridDim.x * (blockDim.x >> 6);
-    for (long long g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < groups; g += waves) {
-        const long long idx = g * 16 + r;
-        const bool valid = idx < per_view * n_views;
-        const int view = valid ? int(idx / per_view) : 0;
-        const int pix = valid ? int(idx % per_view) : 0;
+    const int w = a.width, h = a.height;
+    const long long per_view = (long long)w * h;
+    const long long total = per_view * n_views;
+    const long long runs = (total + 63) / 64;
+    const long long waves = (long long)gridDim.x * (blockDim.x >> 6);
+    const long long per_wave = (runs + waves - 1) / waves;
+    const long long run0 = (blockIdx.x * (long long)(blockDim.x >> 6) + wave) * per_wave;
+    const long long run1 = run0 + per_wave < runs ? run0 + per_wave : runs;
+    // This lane's pixel of the first run, then advanced by 64 pixels per run.
+    long long idx = run0 * 64 + lane;
+    int view = int(idx / per_view), x, y;
+    {
+        const long long pix = idx - (long long)view * per_view;
+        y = int(pix / w);
+        x = int(pix - (long long)y * w);
+    }
+    int row_view = -1, row_y = -1;  // the view and row that fv / fr were computed for
+    FeatureView fv = {};
+    FeatureRow fr = {};
+    const f16x8 zero8 = {};
+    for (long long run = run0; run < run1; ++run) {
+        const bool valid = view < n_views;
         float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (q == 0 && valid) {
-            SkyhipSkyQuery qq = panorama_query(a.views[view], pix % a.width, pix / a.width, a.width, a.height);
-            features(qq, a.log_scales, a.thickness, a.luts.count, f);
+        if (valid) {
+            if (view != row_view || y != row_y) {
+                const SkyhipSkyView& v = a.views[view];
+                fv = feature_view(v.atmosphere, v.altitude, v.mu_sun, a.log_scales, a.thickness, a.luts.count);
+                fr = feature_row(fv, panorama_mu_view(y, h));
+                row_view = view;
+                row_y = y;
+            }
+            features_from(fv, fr, cosf(panorama_delta_phi(x, w)), f);
         }
-        f32x4 y = mlp_forward16(R, s_bias, input_operand(f, valid, q), q);
-        if (q == 0 && valid) a.out[idx] = make_float4(__expf(y[0]), __expf(y[1]), __expf(y[2]), 0.0f);
+        s_in[wave][lane] = input_operand(f, valid, 0);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+        for (int k = 0; k < 4; ++k) {
+            const f16x8 xin = q == 0 ? s_in[wave][16 * k + r] : zero8;
+            const f32x4 yk = mlp_forward16(R, s_bias, xin, q);
+            if (q == 0) s_y[wave][16 * k + r] = make_float4(yk[0], yk[1], yk[2], 0.0f);
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            const float4 o = s_y[wave][lane];
+            a.out[idx] = make_float4(__expf(o.x), __expf(o.y), __expf(o.z), 0.0f);
+        }
+        __builtin_amdgcn_wave_barrier();
+        idx += 64;
+        x += 64;
+        while (x >= w) {
+            x -= w;
+            if (++y == h) {
+                y = 0;
+                ++view;
+            }
+        }
     }
 }
 
@@ -724,8 +842,10 @@ extern "C" int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* para
         return fail(SKYHIP_ERR_INVALID_ARGUMENT, "tolerance and radiance floor must be > 0");
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(256), 0, st, counters, 1);
-    const long long groups = ((long long)width * height * view_count + 15) / 16;
-    const int blocks = int(std::max(1LL, std::min((groups + 3) / 4, (long long)ctx->cu_count * 8)));
+    // One wave per 64-pixel run, grid-strided; two workgroups per CU is what the kernel's
+    // registers (the weights) leave room for, so each wave loads the weights once.
+    const long long runs = ((long long)width * height * view_count + 63) / 64;
+    const int blocks = int(std::max(1LL, std::min((runs + 3) / 4, (long long)ctx->cu_count * 2)));
     hipLaunchKernelGGL(sky_render_surrogate_kernel, dim3(blocks), dim3(256), 0, st, params, a, view_count);
     dim3 tiles(a.tiles_x * a.tiles_y, view_count);
     const long long probes = 4LL * a.tiles_x * a.tiles_y * view_count;
