@@ -905,7 +905,8 @@ int skyhip_sky_radiance(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const Sky
                         float* out_rgb);
 /* Surrogate inputs (8 floats per query, the 8th 0): 2 sqrt(altitude / thickness) - 1; the
  * view zenith as the sky-view LUT's latitude coordinate x_lat (square-root spacing from the
- * horizon on both sides, GetSkyViewTextureUvFromCosLatLon), 2 x_lat - 1; mu_sun; the
+ * horizon on both sides, GetSkyViewTextureUvFromCosLatLon), 2 x_lat - 1;
+ * sign(mu_sun) sqrt(|mu_sun|); the
  * sun-view cosine mu_view mu_sun + sin_view sin_sun cos(delta_phi); and the atmosphere's
  * natural-log Rayleigh, Mie and ozone scale factors (log_scales: device
  * float[3 * atmospheres]) divided by ln 4. */

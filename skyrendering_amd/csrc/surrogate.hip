@@ -7,7 +7,7 @@
 // Parameters (FP32 master copy, 13187 floats): for each layer l with fan-in K and
 // fan-out N, W_l as N x K row-major (out-major), then b_l[N]; layers in order.
 // Inputs (8 floats per sample, the 8th zero): 2 sqrt(h / H) - 1; the view zenith as the
-// sky-view LUT's latitude coordinate, 2 x_lat - 1; mu_sun; the sun-view cosine
+// sky-view LUT's latitude coordinate, 2 x_lat - 1; sign(mu_sun) sqrt(|mu_sun|); the sun-view cosine
 // mu_view mu_sun + sin_view sin_sun cos(dphi); log(rayleigh scale) / log 4,
 // log(mie scale) / log 4, log(ozone scale) / log 4.  Output: log of the RGB radiance
 // (solar illuminance units).
@@ -123,8 +123,8 @@ __global__ __launch_bounds__(256) void sky_radiance_kernel(SkyLuts L, const Skyh
 // the sun-view cosine it implies (monotonic in delta_phi for fixed zenith angles, the
 // argument of both phase functions).  The radiance is smooth in them.
 struct FeatureView {      // camera and atmosphere
-    float f0, f2, f4, f5, f6;  // altitude, mu_sun, log scales / ln 4
-    float horizon_up, horizon_down, sin_s;
+    float f0, f2, f4, f5, f6;  // altitude, sun zenith, log scales / ln 4
+    float mu_s, horizon_up, horizon_down, sin_s;
 };
 struct FeatureRow {       // one view zenith of one view
     float f1, mu_v, sin_v;
@@ -140,7 +140,10 @@ __device__ inline FeatureView feature_view(int atmosphere, float altitude, float
     const float h = fmaxf(altitude, 0.0f), r = rb + h;
     v.horizon_down = acosf(sqrtf(h * (r + rb)) / r);
     v.horizon_up = kPiF - v.horizon_down;
-    v.f2 = mu_sun;
+    // The sun zenith as sign(mu_s) sqrt(|mu_s|): the square root stretches the low-sun range,
+    // where the sunlight's path through the atmosphere, and so the sky's colour, changes fastest.
+    v.mu_s = mu_sun;
+    v.f2 = copysignf(sqrtf(fabsf(mu_sun)), mu_sun);
     v.sin_s = sqrtf(fmaxf(1.0f - mu_sun * mu_sun, 0.0f));
     v.f4 = log_scales[3 * a + 0] * inv_log4;
     v.f5 = log_scales[3 * a + 1] * inv_log4;
@@ -164,7 +167,7 @@ __device__ inline void features_from(const FeatureView& v, const FeatureRow& r, 
     f[0] = v.f0;
     f[1] = r.f1;
     f[2] = v.f2;
-    f[3] = r.mu_v * v.f2 + r.sin_v * v.sin_s * cos_dphi;
+    f[3] = r.mu_v * v.mu_s + r.sin_v * v.sin_s * cos_dphi;
     f[4] = v.f4;
     f[5] = v.f5;
     f[6] = v.f6;
