@@ -415,11 +415,13 @@ SURROGATE_MAX_ALTITUDE_KM = 10.0
 SURROGATE_MAX_SUN_ZENITH_DEG = 90.0
 
 
-def bench_surrogate(ctx, epochs=1200, width=1920, height=1080, n_views=64, cpu_seconds=4.0):
+def bench_surrogate(ctx, epochs=1200, width=1920, height=1080, n_views=64, cpu_seconds=4.0, compare_bf16=True):
     """Trains the sky-view surrogate on 2^22 plain-integrator samples (1024 atmospheres) drawn
     uniformly over the panoramas' domain, then renders n_views panoramas three ways: plain
     integrator per pixel, surrogate with the per-tile residual check and fallback (tolerance
-    1e-2), and the sky-view LUT sampled bilinearly."""
+    1e-2), and the sky-view LUT sampled bilinearly.  The default network is the FP16 one
+    with the gated activation; the BF16 / SiLU network is trained on the same data and timed
+    beside it (`bf16_silu`)."""
     from skyrendering_amd import surrogate as sg
     t0 = time.perf_counter()
     atm = sg.SkyAtmospheres(ctx, sg.draw_scales(1024, 41))
@@ -466,6 +468,19 @@ def bench_surrogate(ctx, epochs=1200, width=1920, height=1080, n_views=64, cpu_s
     out["roofline"] = {"bound": "mfma", "unit": "TFLOP/s", "kernel": "sky_render_surrogate_kernel (+ probe check)",
                        "flop_per_sample": SURROGATE_FLOP_PER_SAMPLE, "achieved": round(achieved, 2),
                        "peak": F16_PEAK_TFLOPS, "frac": round(achieved / F16_PEAK_TFLOPS, 4)}
+    alt = None
+    if compare_bf16:
+        alt = sg.SkySurrogate(ctx, seed=43, network="bf16_silu")
+        alt.set_output_bias(data.targets_host.mean(axis=0))
+        t0 = time.perf_counter()
+        alt_hist = alt.train(data, epochs=epochs, batch=8192, lr=2e-3, lr_final=2e-5)
+        ctx.synchronize()
+        b = {"network": "BF16 MFMA operands, SiLU", "train_s": round(time.perf_counter() - t0, 2),
+             "final_loss": alt_hist[-1], "surrogate_ms": round(timed(lambda: pano.render_surrogate(alt)), 3)}
+        b["fallback_tile_fraction"] = round(pano.fallback_tiles() / (pano.tiles * n_views), 4)
+        b["network_pass_ms"] = round(timed(lambda: pano.render_surrogate(alt, tolerance=1e30)), 3)
+        b["speedup_vs_plain"] = round(out["plain_ms"] / b["surrogate_ms"], 3)
+        out["bf16_silu"] = b
     del pano
     # Errors at a quarter resolution (the same views), against the plain pass and the FP64 oracle.
     import oracle
@@ -473,9 +488,13 @@ def bench_surrogate(ctx, epochs=1200, width=1920, height=1080, n_views=64, cpu_s
     small.render_plain()
     plain = small.image()[..., :3].astype(np.float64)
     errs = {}
-    for name, fn in (("surrogate", lambda: small.render_surrogate(model)),
-                     ("network_only", lambda: small.render_surrogate(model, tolerance=1e30)),
-                     ("lut", small.render_lut)):
+    variants = [("surrogate", lambda: small.render_surrogate(model)),
+                ("network_only", lambda: small.render_surrogate(model, tolerance=1e30)),
+                ("lut", small.render_lut)]
+    if alt is not None:
+        variants += [("bf16_silu_surrogate", lambda: small.render_surrogate(alt)),
+                     ("bf16_silu_network_only", lambda: small.render_surrogate(alt, tolerance=1e30))]
+    for name, fn in variants:
         fn()
         e = sg.relative_error(small.image()[..., :3].astype(np.float64), plain, 1e-6)
         errs[name] = {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)), "max": float(e.max()),

@@ -899,6 +899,16 @@ typedef struct SkyhipSkyLuts {
 
 /* 13187: W1 (64x7), b1, W2..W4 (64x64), b2..b4, W5 (3x64), b5, each W out-major. */
 int skyhip_surrogate_parameter_count(void);
+/* The network kind, an argument of the forward, gradient and render entry points: the same
+ * layers and parameters with
+ *  - SKYHIP_SURROGATE_F16_GATE: FP16 MFMA operands and the gate a(z) = z t^2 (3 - 2 t),
+ *    t = clamp(z / 6 + 1/2, 0, 1), evaluated in packed FP16 (the default), or
+ *  - SKYHIP_SURROGATE_BF16_SILU: BF16 MFMA operands and SiLU z sigmoid(z) in FP32.
+ * Both accumulate in FP32. */
+enum {
+    SKYHIP_SURROGATE_F16_GATE = 0,
+    SKYHIP_SURROGATE_BF16_SILU = 1
+};
 /* The plain integrator per query (ComputeScatteredLuminance with the multiple-scattering
  * term, Atmosphere.glsl:220-295, along the query's ray): out_rgb device float[3 * count]. */
 int skyhip_sky_radiance(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const SkyhipSkyQuery* queries, int32_t count,
@@ -913,19 +923,19 @@ int skyhip_sky_radiance(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const Sky
 int skyhip_surrogate_features(SkyhipContext* ctx, const SkyhipSkyQuery* queries, int32_t count,
                               const float* log_scales, int32_t atmosphere_count, float thickness,
                               float* out_features);
-/* Inference (FP16 MFMA operands, FP32 accumulate, packed FP16 activations): out device
+/* Inference (MFMA operands and activation per `network`, FP32 accumulate): out device
  * float[3 * count] log radiance. */
-int skyhip_surrogate_forward(SkyhipContext* ctx, const float* params, const float* features, int32_t count,
-                             float* out_log_radiance);
+int skyhip_surrogate_forward(SkyhipContext* ctx, int32_t network, const float* params, const float* features,
+                             int32_t count, float* out_log_radiance);
 /* Workspace of skyhip_surrogate_gradient for `count` samples. */
 size_t skyhip_surrogate_workspace_bytes(int32_t count);
 /* Gradient of loss = loss_scale * sum_samples w * sum_rgb (f(x) - target)^2 over `count`
  * samples (targets: device float[3 * count] log radiance; weights: device float[count], or
- * NULL for w = 1) by the FP16 MFMA forward and backward kernels; out_grad: device
+ * NULL for w = 1) by the MFMA forward and backward kernels of `network`; out_grad: device
  * float[13187]; out_loss: device float (may be NULL). */
-int skyhip_surrogate_gradient(SkyhipContext* ctx, const float* params, const float* features, const float* targets,
-                              const float* weights, int32_t count, float loss_scale, float* out_grad,
-                              float* out_loss, void* workspace, size_t workspace_bytes);
+int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, const float* params, const float* features,
+                              const float* targets, const float* weights, int32_t count, float loss_scale,
+                              float* out_grad, float* out_loss, void* workspace, size_t workspace_bytes);
 /* One Adam step (step counts from 1) on device params / moments. */
 int skyhip_surrogate_adam(SkyhipContext* ctx, float* params, const float* grad, float* m, float* v, float lr,
                           float beta1, float beta2, float eps, int32_t step);
@@ -936,8 +946,8 @@ int skyhip_surrogate_adam(SkyhipContext* ctx, float* params, const float* grad, 
  * if any probe's max_c |surrogate - plain| / max(plain, radiance_floor) exceeds
  * `tolerance` the whole tile is recomputed by the plain integrator.  tile_flags: device
  * uint32[view_count * ceil(width/16) * ceil(height/16)] (1 = fell back); counters: device
- * uint32[1], the number of tiles that fell back. */
-int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* params, const SkyhipSkyLuts* luts,
+ * uint32[1], the number of tiles that fell back.  network: as for skyhip_surrogate_forward. */
+int skyhip_sky_render_surrogate(SkyhipContext* ctx, int32_t network, const float* params, const SkyhipSkyLuts* luts,
                                 const SkyhipSkyView* views, int32_t view_count, const float* log_scales,
                                 float thickness, int32_t width, int32_t height, float tolerance,
                                 float radiance_floor, float* out_rgba, uint32_t* tile_flags, uint32_t* counters);

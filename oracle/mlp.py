@@ -1,15 +1,19 @@
 """The oracle (test infrastructure) for the sky-view surrogate MLP: an FP64 numpy
 restatement of its forward pass and of the gradient of loss = scale * sum w (y - t)^2.
 
-The hidden activation is a(z) = z s(t), t = clamp(z / 6 + 1/2, 0, 1), s = t^2 (3 - 2 t).
+Two networks share the layers and the parameter layout:
+- "f16_gate": hidden activation a(z) = z s(t), t = clamp(z / 6 + 1/2, 0, 1),
+  s = t^2 (3 - 2 t), FP16 MFMA operands;
+- "bf16_silu": hidden activation z sigmoid(z), BF16 MFMA operands.
 
-With half=True every operand the GPU kernels feed to FP16 MFMA is rounded to FP16 at the
-same points: the inputs and the weights (round to nearest even), each hidden activation
-(the kernels' packed FP16 sequence, emulated operation by operation: the FP32
+With rounded=True every operand the GPU kernels feed to MFMA is rounded at the same points:
+the inputs and the weights (to nearest even, FP16 or BF16), each hidden activation (for
+f16_gate the kernels' packed FP16 sequence, emulated operation by operation: the FP32
 pre-activation rounded toward zero, then fma, clamp, fma and three multiplies, each
-rounded to nearest even), and each back-propagated delta (kept without the loss scale,
-which multiplies the gradient sums).  Everything else, including the accumulations, runs
-in FP64.  Only tests/ and the benchmark's checks use this module.
+rounded to nearest even; for bf16_silu the FP64 SiLU rounded to BF16), and each
+back-propagated delta (kept without the loss scale, which multiplies the gradient sums).
+Everything else, including the accumulations, runs in FP64.  Only tests/ and the
+benchmark's checks use this module.
 """
 from __future__ import annotations
 
@@ -86,10 +90,35 @@ def pack(layers):
     return np.concatenate([np.concatenate([W.ravel(), b]) for W, b in layers])
 
 
-def forward(params, x, half=False, cache=False):
+NETWORKS = ("f16_gate", "bf16_silu")
+
+
+def sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def silu(z):
+    return z * sigmoid(z)
+
+
+def silu_grad(z):
+    s = sigmoid(z)
+    return s * (1.0 + z * (1.0 - s))
+
+
+def _network(network, rounded):
+    """(operand rounding, activation, activation derivative) of a network."""
+    if network not in NETWORKS:
+        raise ValueError(network)
+    exact = lambda v: np.asarray(v, dtype=np.float64)  # noqa: E731
+    if network == "f16_gate":
+        return (f16_round, activation_f16, activation_grad) if rounded else (exact, activation, activation_grad)
+    return (bf16_round, lambda z: bf16_round(silu(z)), silu_grad) if rounded else (exact, silu, silu_grad)
+
+
+def forward(params, x, network="f16_gate", rounded=False, cache=False):
     """x: [n, >= 7] inputs; returns y [n, 3] (and the pre-activations if cache)."""
-    r = f16_round if half else (lambda v: np.asarray(v, dtype=np.float64))
-    act = activation_f16 if half else activation
+    r, act, _ = _network(network, rounded)
     layers = unpack(params)
     a = r(np.asarray(x, dtype=np.float64)[:, :7]).T  # [7, n]
     zs, acts = [], [a]
@@ -102,11 +131,11 @@ def forward(params, x, half=False, cache=False):
         acts.append(a)
 
 
-def loss_and_grad(params, x, t, loss_scale, half=False, weights=None):
+def loss_and_grad(params, x, t, loss_scale, network="f16_gate", rounded=False, weights=None):
     """(loss, gradient as a flat vector) of loss = loss_scale * sum_n w_n sum_c (y - t)^2."""
-    r = f16_round if half else (lambda v: np.asarray(v, dtype=np.float64))
+    r, _, act_grad = _network(network, rounded)
     layers = unpack(params)
-    y, zs, acts = forward(params, x, half=half, cache=True)
+    y, zs, acts = forward(params, x, network=network, rounded=rounded, cache=True)
     d = y.T - np.asarray(t, dtype=np.float64).T  # [3, n]
     w = np.ones(d.shape[1]) if weights is None else np.asarray(weights, dtype=np.float64)
     loss = loss_scale * float(np.sum(w * d * d))
@@ -114,8 +143,8 @@ def loss_and_grad(params, x, t, loss_scale, half=False, weights=None):
     grads = [None] * len(layers)
     for li in range(len(layers) - 1, -1, -1):
         W, _ = layers[li]
-        a_prev = acts[li]  # already rounded in forward when half
+        a_prev = acts[li]  # already rounded in forward when rounded
         grads[li] = (loss_scale * (delta @ a_prev.T), loss_scale * delta.sum(axis=1))
         if li > 0:
-            delta = r((r(W).T @ delta) * activation_grad(zs[li - 1]))
+            delta = r((r(W).T @ delta) * act_grad(zs[li - 1]))
     return loss, pack(grads)

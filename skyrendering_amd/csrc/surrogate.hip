@@ -38,6 +38,7 @@
 namespace skyhip {
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -69,6 +70,47 @@ __device__ __forceinline__ float activation_grad(float z) {
     const float t = fminf(fmaxf(z * (1.0f / 6.0f) + 0.5f, 0.0f), 1.0f);
     return t * t * (3.0f - 2.0f * t) + z * t * (1.0f - t);
 }
+
+// SiLU z sigmoid(z) with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, each ~1 ulp),
+// in FP32, rounded to BF16 for the next layer.
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+__device__ __forceinline__ float silu_grad(float z) {
+    const float s = sigmoid(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// The two networks (same layers, same parameter layout, same inputs):
+//  - NetF16Gate (SKYHIP_SURROGATE_F16_GATE): FP16 MFMA operands, the gated activation above
+//    in packed FP16.  The default.
+//  - NetBF16SiLU (SKYHIP_SURROGATE_BF16_SILU): BF16 MFMA operands, SiLU in FP32.
+struct NetF16Gate {
+    typedef _Float16 T;
+    typedef f16x8 V8;
+    __device__ static __forceinline__ f32x4 mfma(V8 a, V8 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ void act_pair(float z0, float z1, T& a0, T& a1) {
+        const f16x2 r = activation2(z0, z1);
+        a0 = r[0];
+        a1 = r[1];
+    }
+    __device__ static __forceinline__ T act(float z) { return activation1(z); }
+    __device__ static __forceinline__ float act_grad(float z) { return activation_grad(z); }
+};
+struct NetBF16SiLU {
+    typedef __bf16 T;
+    typedef bf16x8 V8;
+    __device__ static __forceinline__ f32x4 mfma(V8 a, V8 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ void act_pair(float z0, float z1, T& a0, T& a1) {
+        a0 = (__bf16)silu(z0);
+        a1 = (__bf16)silu(z1);
+    }
+    __device__ static __forceinline__ T act(float z) { return (__bf16)silu(z); }
+    __device__ static __forceinline__ float act_grad(float z) { return silu_grad(z); }
+};
 
 // Sky radiance along (mu_view, dphi) from altitude h for sun zenith cosine mu_sun.
 __device__ inline F3 sky_radiance(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
@@ -200,68 +242,75 @@ __device__ __forceinline__ int permuted_neuron(int b, int q, int i) {
     return 32 * b + (i < 4 ? 4 * q + i : 16 + 4 * q + (i - 4));
 }
 
+template <class Net>
 struct MlpRegs {
-    f16x8 w1[4];
-    f16x8 wh[3][4][2];
-    f16x8 w5[2];
+    typename Net::V8 w1[4];
+    typename Net::V8 wh[3][4][2];
+    typename Net::V8 w5[2];
 };
 
-__device__ inline void load_mlp(const float* __restrict__ p, MlpRegs& R, int lane) {
+template <class Net>
+__device__ inline void load_mlp(const float* __restrict__ p, MlpRegs<Net>& R, int lane) {
+    typedef typename Net::T T;
     const int q = lane >> 4, r = lane & 15;
     for (int j = 0; j < 4; ++j)
         for (int i = 0; i < 8; ++i) {
             int k = 8 * q + i;
-            R.w1[j][i] = (_Float16)(k < 7 ? p[kOffW1 + (16 * j + r) * 7 + k] : 0.0f);
+            R.w1[j][i] = (T)(k < 7 ? p[kOffW1 + (16 * j + r) * 7 + k] : 0.0f);
         }
     const int offs[3] = {kOffW2, kOffW3, kOffW4};
     for (int L = 0; L < 3; ++L)
         for (int j = 0; j < 4; ++j)
             for (int b = 0; b < 2; ++b)
                 for (int i = 0; i < 8; ++i)
-                    R.wh[L][j][b][i] = (_Float16)p[offs[L] + (16 * j + r) * 64 + permuted_neuron(b, q, i)];
+                    R.wh[L][j][b][i] = (T)p[offs[L] + (16 * j + r) * 64 + permuted_neuron(b, q, i)];
     for (int b = 0; b < 2; ++b)
         for (int i = 0; i < 8; ++i)
-            R.w5[b][i] = (_Float16)(r < 3 ? p[kOffW5 + r * 64 + permuted_neuron(b, q, i)] : 0.0f);
+            R.w5[b][i] = (T)(r < 3 ? p[kOffW5 + r * 64 + permuted_neuron(b, q, i)] : 0.0f);
 }
 
 // B operands of the next layer from the four accumulator tiles (bias included): the
 // activation of accumulator items (2 i, 2 i + 1), two at a time.
-__device__ __forceinline__ void activate(const f32x4 c[4], f16x8 h[2]) {
+template <class Net>
+__device__ __forceinline__ void activate(const f32x4 c[4], typename Net::V8 h[2]) {
+    typedef typename Net::T T;
     for (int b = 0; b < 2; ++b)
         for (int i = 0; i < 2; ++i) {
-            const f16x2 lo = activation2(c[2 * b][2 * i], c[2 * b][2 * i + 1]);
-            const f16x2 hi = activation2(c[2 * b + 1][2 * i], c[2 * b + 1][2 * i + 1]);
-            h[b][2 * i] = lo[0];
-            h[b][2 * i + 1] = lo[1];
-            h[b][4 + 2 * i] = hi[0];
-            h[b][4 + 2 * i + 1] = hi[1];
+            T l0, l1, h0, h1;
+            Net::act_pair(c[2 * b][2 * i], c[2 * b][2 * i + 1], l0, l1);
+            Net::act_pair(c[2 * b + 1][2 * i], c[2 * b + 1][2 * i + 1], h0, h1);
+            h[b][2 * i] = l0;
+            h[b][2 * i + 1] = l1;
+            h[b][4 + 2 * i] = h0;
+            h[b][4 + 2 * i + 1] = h1;
         }
 }
 
 // Forward pass of 16 samples (the lane's column); x: this lane's input operand.
 // Returns the output accumulator (lanes 0..15 hold rows 0..2 in items 0..2).
-__device__ __forceinline__ f32x4 mlp_forward16(const MlpRegs& R, const float* s_bias, f16x8 x, int q) {
+template <class Net>
+__device__ __forceinline__ f32x4 mlp_forward16(const MlpRegs<Net>& R, const float* s_bias, typename Net::V8 x, int q) {
     f32x4 c[4];
-    f16x8 h[2];
+    typename Net::V8 h[2];
     for (int j = 0; j < 4; ++j) {
         const float* b = s_bias + 16 * j + 4 * q;
         c[j] = f32x4{b[0], b[1], b[2], b[3]};
-        c[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.w1[j], x, c[j], 0, 0, 0);
+        c[j] = Net::mfma(R.w1[j], x, c[j]);
     }
-    activate(c, h);
+    activate<Net>(c, h);
     for (int L = 0; L < 3; ++L) {
         for (int j = 0; j < 4; ++j) {
             const float* b = s_bias + 64 * (L + 1) + 16 * j + 4 * q;
             c[j] = f32x4{b[0], b[1], b[2], b[3]};
-            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.wh[L][j][0], h[0], c[j], 0, 0, 0);
-            c[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.wh[L][j][1], h[1], c[j], 0, 0, 0);
+            c[j] = Net::mfma(R.wh[L][j][0], h[0], c[j]);
+            c[j] = Net::mfma(R.wh[L][j][1], h[1], c[j]);
         }
-        activate(c, h);
+        activate<Net>(c, h);
     }
     const float* b = s_bias + 256 + 4 * q;
     f32x4 y = {b[0], b[1], b[2], b[3]};
-    y = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.w5[0], h[0], y, 0, 0, 0);
-    y = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.w5[1], h[1], y, 0, 0, 0);
+    y = Net::mfma(R.w5[0], h[0], y);
+    y = Net::mfma(R.w5[1], h[1], y);
     return y;
 }
 
@@ -279,27 +328,29 @@ __device__ inline void load_biases(const float* __restrict__ p, float* s_bias) {
     __syncthreads();
 }
 
-__device__ __forceinline__ f16x8 input_operand(const float* f, bool valid, int q) {
-    f16x8 x;
-    for (int i = 0; i < 8; ++i) x[i] = (_Float16)((q == 0 && valid) ? f[i] : 0.0f);
+template <class Net>
+__device__ __forceinline__ typename Net::V8 input_operand(const float* f, bool valid, int q) {
+    typename Net::V8 x;
+    for (int i = 0; i < 8; ++i) x[i] = (typename Net::T)((q == 0 && valid) ? f[i] : 0.0f);
     return x;
 }
 
+template <class Net>
 __global__ __launch_bounds__(256, 2) void surrogate_forward_kernel(const float* __restrict__ params,
                                                                 const float* __restrict__ feats, int n,
                                                                 float* __restrict__ out) {
     __shared__ float s_bias[272];
     load_biases(params, s_bias);
     const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
-    MlpRegs R;
-    load_mlp(params, R, lane);
+    MlpRegs<Net> R;
+    load_mlp<Net>(params, R, lane);
     const int waves = gridDim.x * (blockDim.x >> 6);
     for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g * 16 < n; g += waves) {
         const int s = g * 16 + r;
         float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (q == 0 && s < n)
             for (int k = 0; k < 8; ++k) f[k] = feats[8 * s + k];
-        f32x4 y = mlp_forward16(R, s_bias, input_operand(f, s < n, q), q);
+        f32x4 y = mlp_forward16<Net>(R, s_bias, input_operand<Net>(f, s < n, q), q);
         if (q == 0 && s < n) {
             out[3 * s + 0] = y[0];
             out[3 * s + 1] = y[1];
@@ -316,8 +367,9 @@ constexpr int kTS = 32;  // samples per workgroup
 // C[M x N] = A[M x K] B[K x N] with FP16 LDS operands at element strides
 // A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn]; 16 x 16 tiles spread over
 // the workgroup's waves, K a multiple of 32.  ep(m, n, value) consumes each output once.
-template <class Ep>
-__device__ inline void lds_gemm(const _Float16* A, int sam, int sak, const _Float16* B, int sbk, int sbn, int M, int N,
+template <class Net, class Ep>
+__device__ inline void lds_gemm(const typename Net::T* A, int sam, int sak, const typename Net::T* B, int sbk, int sbn,
+                                int M, int N,
                                 int K, Ep ep) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     const int q = lane >> 4, r = lane & 15;
@@ -326,32 +378,34 @@ __device__ inline void lds_gemm(const _Float16* A, int sam, int sak, const _Floa
         const int m0 = (t / tn_count) * 16, n0 = (t % tn_count) * 16;
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int k0 = 0; k0 < K; k0 += 32) {
-            f16x8 a, b;
+            typename Net::V8 a, b;
             for (int i = 0; i < 8; ++i) {
                 const int k = k0 + 8 * q + i;
                 a[i] = A[(m0 + r) * sam + k * sak];
                 b[i] = B[k * sbk + (n0 + r) * sbn];
             }
-            c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+            c = Net::mfma(a, b, c);
         }
         for (int i = 0; i < 4; ++i) ep(m0 + 4 * q + i, n0 + r, c[i]);
     }
 }
 
+template <class T>
 struct TrainLds {
-    _Float16 w1[64 * 32];   // [neuron][input] (inputs 7..31 zero)
-    _Float16 wh[3][64 * 64];
-    _Float16 w5[32 * 64];   // [output][neuron] (outputs 3..31 zero)
-    _Float16 x[32 * kTS];   // [input][sample]
+    T w1[64 * 32];   // [neuron][input] (inputs 7..31 zero)
+    T wh[3][64 * 64];
+    T w5[32 * 64];   // [output][neuron] (outputs 3..31 zero)
+    T x[32 * kTS];   // [input][sample]
     float z[4][64 * kTS]; // pre-activations [neuron][sample]
-    _Float16 a[4][64 * kTS];
+    T a[4][64 * kTS];
     float y[16 * kTS];
-    _Float16 d5[32 * kTS];  // output deltas 2 w (y - t) [output][sample] (rows 3..31 zero)
-    _Float16 d[2][64 * kTS];  // hidden deltas (without the loss scale)
+    T d5[32 * kTS];  // output deltas 2 w (y - t) [output][sample] (rows 3..31 zero)
+    T d[2][64 * kTS];  // hidden deltas (without the loss scale)
     float bias[272];
     float loss;
 };
 
+template <class Net>
 __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __restrict__ params,
                                                               const float* __restrict__ feats,
                                                               const float* __restrict__ targets,
@@ -359,42 +413,43 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
                                                               float loss_scale, float* __restrict__ partial,
                                                               float* __restrict__ partial_loss) {
     extern __shared__ __align__(16) unsigned char smem[];
-    TrainLds& S = *reinterpret_cast<TrainLds*>(smem);
+    typedef typename Net::T T;
+    TrainLds<T>& S = *reinterpret_cast<TrainLds<T>*>(smem);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int s0 = blockIdx.x * kTS;
     float* G = partial + (long long)blockIdx.x * kParams;
     // Weights to FP16 (zero padding), inputs, biases.
     for (int i = tid; i < 64 * 32; i += nt) {
         int m = i / 32, k = i % 32;
-        S.w1[i] = (_Float16)(k < 7 ? params[kOffW1 + m * 7 + k] : 0.0f);
+        S.w1[i] = (T)(k < 7 ? params[kOffW1 + m * 7 + k] : 0.0f);
     }
     const int offs[3] = {kOffW2, kOffW3, kOffW4};
     for (int L = 0; L < 3; ++L)
-        for (int i = tid; i < 64 * 64; i += nt) S.wh[L][i] = (_Float16)params[offs[L] + i];
-    for (int i = tid; i < 32 * 64; i += nt) S.w5[i] = (_Float16)(i < 3 * 64 ? params[kOffW5 + i] : 0.0f);
+        for (int i = tid; i < 64 * 64; i += nt) S.wh[L][i] = (T)params[offs[L] + i];
+    for (int i = tid; i < 32 * 64; i += nt) S.w5[i] = (T)(i < 3 * 64 ? params[kOffW5 + i] : 0.0f);
     for (int i = tid; i < 32 * kTS; i += nt) {
         int k = i / kTS, s = i % kTS;
-        S.x[i] = (_Float16)((k < 8 && s0 + s < n) ? feats[8 * (s0 + s) + k] : 0.0f);
-        S.d5[i] = (_Float16)0.0f;
+        S.x[i] = (T)((k < 8 && s0 + s < n) ? feats[8 * (s0 + s) + k] : 0.0f);
+        S.d5[i] = (T)0.0f;
     }
     if (tid == 0) S.loss = 0.0f;
     load_biases(params, S.bias);  // synchronises
     // Forward.
-    lds_gemm(S.w1, 32, 1, S.x, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
+    lds_gemm<Net>(S.w1, 32, 1, S.x, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
         float z = c + S.bias[m];
         S.z[0][m * kTS + s] = z;
-        S.a[0][m * kTS + s] = activation1(z);
+        S.a[0][m * kTS + s] = Net::act(z);
     });
     __syncthreads();
     for (int L = 0; L < 3; ++L) {
-        lds_gemm(S.wh[L], 64, 1, S.a[L], kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
+        lds_gemm<Net>(S.wh[L], 64, 1, S.a[L], kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
             float z = c + S.bias[64 * (L + 1) + m];
             S.z[L + 1][m * kTS + s] = z;
-            S.a[L + 1][m * kTS + s] = activation1(z);
+            S.a[L + 1][m * kTS + s] = Net::act(z);
         });
         __syncthreads();
     }
-    lds_gemm(S.w5, 64, 1, S.a[3], kTS, 1, 16, kTS, 64,
+    lds_gemm<Net>(S.w5, 64, 1, S.a[3], kTS, 1, 16, kTS, 64,
              [&](int m, int s, float c) { S.y[m * kTS + s] = c + S.bias[256 + m]; });
     __syncthreads();
     // Loss loss_scale * sum w (y - t)^2 and the output deltas 2 w (y - t); the gradient
@@ -407,11 +462,11 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
             w = weights ? weights[s0 + s] : 1.0f;
         }
         atomicAdd(&S.loss, w * d * d * loss_scale);
-        S.d5[m * kTS + s] = (_Float16)(2.0f * w * d);
+        S.d5[m * kTS + s] = (T)(2.0f * w * d);
     }
     __syncthreads();
     // Output layer gradient: dW5 = d5 a4^T, db5 = sum d5.
-    lds_gemm(S.d5, kTS, 1, S.a[3], 1, kTS, 16, 64, kTS, [&](int m, int k, float c) {
+    lds_gemm<Net>(S.d5, kTS, 1, S.a[3], 1, kTS, 16, 64, kTS, [&](int m, int k, float c) {
         if (m < 3) G[kOffW5 + m * 64 + k] = c * loss_scale;
     });
     for (int m = tid; m < 3; m += nt) {
@@ -420,17 +475,17 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
         G[kOffB5 + m] = sum * loss_scale;
     }
     // delta4 = (W5^T d5) * a'(z4)
-    lds_gemm(S.w5, 1, 64, S.d5, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
-        S.d[0][m * kTS + s] = (_Float16)(c * activation_grad(S.z[3][m * kTS + s]));
+    lds_gemm<Net>(S.w5, 1, 64, S.d5, kTS, 1, 64, kTS, 32, [&](int m, int s, float c) {
+        S.d[0][m * kTS + s] = (T)(c * Net::act_grad(S.z[3][m * kTS + s]));
     });
     __syncthreads();
     // Hidden layers 4, 3, 2 (weights wh[2], wh[1], wh[0]).
     const int woffs[3] = {kOffW2, kOffW3, kOffW4}, boffs[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
     int cur = 0;
     for (int L = 2; L >= 0; --L) {
-        const _Float16* dl = S.d[cur];
+        const T* dl = S.d[cur];
         // dW = delta a_prev^T ; a_prev = a[L] (input of wh[L])
-        lds_gemm(dl, kTS, 1, S.a[L], 1, kTS, 64, 64, kTS,
+        lds_gemm<Net>(dl, kTS, 1, S.a[L], 1, kTS, 64, 64, kTS,
                  [&](int m, int k, float c) { G[woffs[L] + m * 64 + k] = c * loss_scale; });
         for (int m = tid; m < 64; m += nt) {
             float sum = 0.0f;
@@ -438,16 +493,16 @@ __global__ __launch_bounds__(256) void surrogate_train_kernel(const float* __res
             G[boffs[L + 1] + m] = sum * loss_scale;
         }
         // delta_prev = (W^T delta) * a'(z[L])
-        _Float16* dn = S.d[cur ^ 1];
-        lds_gemm(S.wh[L], 1, 64, dl, kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
-            dn[m * kTS + s] = (_Float16)(c * activation_grad(S.z[L][m * kTS + s]));
+        T* dn = S.d[cur ^ 1];
+        lds_gemm<Net>(S.wh[L], 1, 64, dl, kTS, 1, 64, kTS, 64, [&](int m, int s, float c) {
+            dn[m * kTS + s] = (T)(c * Net::act_grad(S.z[L][m * kTS + s]));
         });
         __syncthreads();
         cur ^= 1;
     }
     // First layer: dW1 = delta1 x^T, db1.
-    const _Float16* d1 = S.d[cur];
-    lds_gemm(d1, kTS, 1, S.x, 1, kTS, 64, 32, kTS, [&](int m, int k, float c) {
+    const T* d1 = S.d[cur];
+    lds_gemm<Net>(d1, kTS, 1, S.x, 1, kTS, 64, 32, kTS, [&](int m, int k, float c) {
         if (k < 7) G[kOffW1 + m * 7 + k] = c * loss_scale;
     });
     for (int m = tid; m < 64; m += nt) {
@@ -543,15 +598,17 @@ __device__ inline SkyhipSkyQuery panorama_query(const SkyhipSkyView& v, int x, i
 // supplies the input operand of each block); the log radiance goes back through LDS so
 // that every lane exponentiates and stores its own pixel.  The staging is private to the
 // wave, so no workgroup barrier is needed.
+template <class Net>
 __global__ __launch_bounds__(256, 2) void sky_render_surrogate_kernel(const float* __restrict__ params,
                                                                    SkyRenderArgs a, int n_views) {
     __shared__ float s_bias[272];
-    __shared__ f16x8 s_in[4][64];
+    typedef typename Net::V8 V8;
+    __shared__ V8 s_in[4][64];
     __shared__ float4 s_y[4][64];
     load_biases(params, s_bias);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
-    MlpRegs R;
-    load_mlp(params, R, lane);
+    MlpRegs<Net> R;
+    load_mlp<Net>(params, R, lane);
     const int w = a.width, h = a.height;
     const long long per_view = (long long)w * h;
     const long long total = per_view * n_views;
@@ -571,7 +628,7 @@ __global__ __launch_bounds__(256, 2) void sky_render_surrogate_kernel(const floa
     int row_view = -1, row_y = -1;  // the view and row that fv / fr were computed for
     FeatureView fv = {};
     FeatureRow fr = {};
-    const f16x8 zero8 = {};
+    const V8 zero8 = {};
     for (long long run = run0; run < run1; ++run) {
         const bool valid = view < n_views;
         float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -585,12 +642,12 @@ __global__ __launch_bounds__(256, 2) void sky_render_surrogate_kernel(const floa
             }
             features_from(fv, fr, cosf(panorama_delta_phi(x, w)), f);
         }
-        s_in[wave][lane] = input_operand(f, valid, 0);
+        s_in[wave][lane] = input_operand<Net>(f, valid, 0);
         __builtin_amdgcn_wave_barrier();
 #pragma unroll 1
         for (int k = 0; k < 4; ++k) {
-            const f16x8 xin = q == 0 ? s_in[wave][16 * k + r] : zero8;
-            const f32x4 yk = mlp_forward16(R, s_bias, xin, q);
+            const V8 xin = q == 0 ? s_in[wave][16 * k + r] : zero8;
+            const f32x4 yk = mlp_forward16<Net>(R, s_bias, xin, q);
             if (q == 0) s_y[wave][16 * k + r] = make_float4(yk[0], yk[1], yk[2], 0.0f);
         }
         __builtin_amdgcn_wave_barrier();
@@ -743,16 +800,27 @@ extern "C" int skyhip_surrogate_features(SkyhipContext* ctx, const SkyhipSkyQuer
     return SKYHIP_OK;
 }
 
-extern "C" int skyhip_surrogate_forward(SkyhipContext* ctx, const float* params, const float* features,
-                                        int32_t count, float* out_log_radiance) {
+static int check_network(int32_t network) {
+    if (network != SKYHIP_SURROGATE_F16_GATE && network != SKYHIP_SURROGATE_BF16_SILU)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "unknown surrogate network kind");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_surrogate_forward(SkyhipContext* ctx, int32_t network, const float* params,
+                                        const float* features, int32_t count, float* out_log_radiance) {
     SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(check_network(network));
     if (!params || !features || !out_log_radiance) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
     if (count < 0) return fail(SKYHIP_ERR_INVALID_SIZE, "negative count");
     if (count == 0) return SKYHIP_OK;
     const int groups = int(div_up(count, 16));
     const int blocks = std::max(1, std::min(int(div_up(groups, 4)), ctx->cu_count * 8));
-    hipLaunchKernelGGL(surrogate_forward_kernel, dim3(blocks), dim3(256), 0, ctx->stream, params, features, count,
-                       out_log_radiance);
+    if (network == SKYHIP_SURROGATE_F16_GATE)
+        hipLaunchKernelGGL(surrogate_forward_kernel<NetF16Gate>, dim3(blocks), dim3(256), 0, ctx->stream, params,
+                           features, count, out_log_radiance);
+    else
+        hipLaunchKernelGGL(surrogate_forward_kernel<NetBF16SiLU>, dim3(blocks), dim3(256), 0, ctx->stream, params,
+                           features, count, out_log_radiance);
     SKYHIP_LAUNCH_CHECK("surrogate_forward_kernel");
     return SKYHIP_OK;
 }
@@ -763,28 +831,39 @@ extern "C" size_t skyhip_surrogate_workspace_bytes(int32_t count) {
     return groups * kParams * sizeof(float) + groups * sizeof(float);
 }
 
-extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, const float* params, const float* features,
-                                         const float* targets, const float* weights, int32_t count,
-                                         float loss_scale, float* out_grad, float* out_loss, void* workspace,
-                                         size_t workspace_bytes) {
+extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, const float* params,
+                                         const float* features, const float* targets, const float* weights,
+                                         int32_t count, float loss_scale, float* out_grad, float* out_loss,
+                                         void* workspace, size_t workspace_bytes) {
     SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(check_network(network));
     if (!params || !features || !targets || !out_grad || !workspace)
         return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
     if (count < 1 || count > (1 << 26)) return fail(SKYHIP_ERR_INVALID_SIZE, "count outside [1, 2^26]");
     if (workspace_bytes < skyhip_surrogate_workspace_bytes(count))
         return fail(SKYHIP_ERR_INVALID_SIZE, "workspace smaller than skyhip_surrogate_workspace_bytes(count)");
     if (!std::isfinite(loss_scale)) return fail(SKYHIP_ERR_NON_FINITE, "non-finite loss scale");
-    static bool lds_ready = false;  // the training kernel's LDS footprint exceeds the 64 KiB default
+    // The training kernels' LDS footprint exceeds the 64 KiB default (both element types are
+    // two bytes, so both layouts have the same size).
+    static_assert(sizeof(TrainLds<_Float16>) == sizeof(TrainLds<__bf16>), "LDS layouts differ");
+    const int lds_bytes = int(sizeof(TrainLds<_Float16>));
+    static bool lds_ready = false;
     if (!lds_ready) {
-        SKYHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&surrogate_train_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainLds))));
+        SKYHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&surrogate_train_kernel<NetF16Gate>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
+        SKYHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&surrogate_train_kernel<NetBF16SiLU>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
         lds_ready = true;
     }
     const int groups = div_up(count, kTS);
     float* partial = reinterpret_cast<float*>(workspace);
     float* partial_loss = partial + (size_t)groups * kParams;
-    hipLaunchKernelGGL(surrogate_train_kernel, dim3(groups), dim3(256), sizeof(TrainLds), ctx->stream, params,
-                       features, targets, weights, count, loss_scale, partial, partial_loss);
+    if (network == SKYHIP_SURROGATE_F16_GATE)
+        hipLaunchKernelGGL(surrogate_train_kernel<NetF16Gate>, dim3(groups), dim3(256), lds_bytes, ctx->stream,
+                           params, features, targets, weights, count, loss_scale, partial, partial_loss);
+    else
+        hipLaunchKernelGGL(surrogate_train_kernel<NetBF16SiLU>, dim3(groups), dim3(256), lds_bytes, ctx->stream,
+                           params, features, targets, weights, count, loss_scale, partial, partial_loss);
     SKYHIP_LAUNCH_CHECK("surrogate_train_kernel");
     hipLaunchKernelGGL(surrogate_reduce_kernel, dim3(div_up(kParams + 1, 64)), dim3(1024), 0, ctx->stream, partial,
                        partial_loss, groups, out_grad, out_loss);
@@ -832,11 +911,13 @@ static int render_args(SkyhipContext* ctx, const SkyhipSkyLuts* luts, const Skyh
     return SKYHIP_OK;
 }
 
-extern "C" int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* params, const SkyhipSkyLuts* luts,
-                                           const SkyhipSkyView* views, int32_t view_count, const float* log_scales,
-                                           float thickness, int32_t width, int32_t height, float tolerance,
-                                           float radiance_floor, float* out_rgba, unsigned* tile_flags,
-                                           unsigned* counters) {
+extern "C" int skyhip_sky_render_surrogate(SkyhipContext* ctx, int32_t network, const float* params,
+                                           const SkyhipSkyLuts* luts, const SkyhipSkyView* views, int32_t view_count,
+                                           const float* log_scales, float thickness, int32_t width, int32_t height,
+                                           float tolerance, float radiance_floor, float* out_rgba,
+                                           unsigned* tile_flags, unsigned* counters) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(check_network(network));
     SkyRenderArgs a;
     SKYHIP_CHECK(render_args(ctx, luts, views, view_count, log_scales, thickness, width, height, out_rgba, tile_flags,
                              counters, tolerance, radiance_floor, a));
@@ -849,7 +930,12 @@ extern "C" int skyhip_sky_render_surrogate(SkyhipContext* ctx, const float* para
     // registers (the weights) leave room for, so each wave loads the weights once.
     const long long runs = ((long long)width * height * view_count + 63) / 64;
     const int blocks = int(std::max(1LL, std::min((runs + 3) / 4, (long long)ctx->cu_count * 2)));
-    hipLaunchKernelGGL(sky_render_surrogate_kernel, dim3(blocks), dim3(256), 0, st, params, a, view_count);
+    if (network == SKYHIP_SURROGATE_F16_GATE)
+        hipLaunchKernelGGL(sky_render_surrogate_kernel<NetF16Gate>, dim3(blocks), dim3(256), 0, st, params, a,
+                           view_count);
+    else
+        hipLaunchKernelGGL(sky_render_surrogate_kernel<NetBF16SiLU>, dim3(blocks), dim3(256), 0, st, params, a,
+                           view_count);
     dim3 tiles(a.tiles_x * a.tiles_y, view_count);
     const long long probes = 4LL * a.tiles_x * a.tiles_y * view_count;
     hipLaunchKernelGGL(sky_render_check_kernel, dim3(unsigned((probes + 255) / 256)), dim3(256), 0, st, a, view_count);

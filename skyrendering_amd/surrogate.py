@@ -3,9 +3,11 @@ the sky, trained on the GPU (skyhip_surrogate_gradient / skyhip_surrogate_adam) 
 labelled by the plain integrator (skyhip_sky_radiance), and evaluated per pixel with a
 per-tile residual check and plain-integrator fallback (skyhip_sky_render_surrogate).
 
-Inputs: camera altitude, cos view zenith, cos sun zenith, relative azimuth and the log
-Rayleigh / Mie / ozone scale factors of the atmosphere (each factor in [0.25, 4] around
-the 100 km Earth preset).  Atmospheres are built by the batched LUT builder with uniform
+Inputs: camera altitude, the view zenith (in the sky-view LUT's latitude coordinate), the
+sun zenith, the sun-view cosine and the log Rayleigh / Mie / ozone scale factors of the
+atmosphere (each factor in [0.25, 4] around the 100 km Earth preset).  Two networks share
+the layers and parameter layout: "f16_gate" (FP16 MFMA operands, a smooth-step gated
+activation in packed FP16; the default) and "bf16_silu" (BF16 MFMA operands, SiLU).  Atmospheres are built by the batched LUT builder with uniform
 LUT sizes, so one SkyhipSkyLuts view covers them all.
 """
 from __future__ import annotations
@@ -54,16 +56,19 @@ _abi.register("skyhip_surrogate_parameter_count", C.c_int, [])
 _abi.register("skyhip_sky_radiance", C.c_int, [C.c_void_p, _P(SkyLutsC), C.c_void_p, C.c_int32, C.c_void_p])
 _abi.register("skyhip_surrogate_features", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                                      C.c_float, C.c_void_p])
-_abi.register("skyhip_surrogate_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p])
+_abi.register("skyhip_surrogate_forward", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                                    C.c_void_p])
 _abi.register("skyhip_surrogate_workspace_bytes", C.c_size_t, [C.c_int32])
-_abi.register("skyhip_surrogate_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                                     C.c_int32, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
-                                                     C.c_size_t])
+_abi.register("skyhip_surrogate_gradient", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                     C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p,
+                                                     C.c_void_p, C.c_size_t])
 _abi.register("skyhip_surrogate_adam", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32])
-_abi.register("skyhip_sky_render_surrogate", C.c_int, [C.c_void_p, C.c_void_p, _P(SkyLutsC), C.c_void_p, C.c_int32,
-                                                       C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_float,
-                                                       C.c_float, C.c_void_p, C.c_void_p, C.c_void_p])
+_abi.register("skyhip_sky_render_surrogate", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, _P(SkyLutsC), C.c_void_p,
+                                                       C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32,
+                                                       C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p])
+
+NETWORKS = {"f16_gate": 0, "bf16_silu": 1}  # SKYHIP_SURROGATE_F16_GATE, SKYHIP_SURROGATE_BF16_SILU
 _abi.register("skyhip_sky_render_plain", C.c_int, [C.c_void_p, _P(SkyLutsC), C.c_void_p, C.c_int32, C.c_int32,
                                                    C.c_int32, C.c_void_p])
 _abi.register("skyhip_sky_render_lut", C.c_int, [C.c_void_p, _P(SkyLutsC), C.c_void_p, C.c_int32, C.c_void_p,
@@ -193,9 +198,13 @@ class SkyTrainingSet:
 class SkySurrogate:
     """The network's parameters and Adam state on the device."""
 
-    def __init__(self, ctx: Context, params=None, seed=0):
+    def __init__(self, ctx: Context, params=None, seed=0, network="f16_gate"):
         assert _abi.lib().skyhip_surrogate_parameter_count() == PARAMS
+        if network not in NETWORKS:
+            raise ValueError(f"network must be one of {sorted(NETWORKS)}")
         self.ctx = ctx
+        self.network = network
+        self.network_id = NETWORKS[network]
         p = init_params(seed) if params is None else np.asarray(params, np.float32)
         self.params = ctx.upload(p)
         self.m = ctx.alloc(PARAMS * 4).zero()
@@ -215,7 +224,8 @@ class SkySurrogate:
         self.params.upload(p)
 
     def forward(self, features: DeviceBuffer, n, out: DeviceBuffer):
-        _abi.check(_abi.lib().skyhip_surrogate_forward(self.ctx.handle, self.params.ptr, features.ptr, n, out.ptr))
+        _abi.check(_abi.lib().skyhip_surrogate_forward(self.ctx.handle, self.network_id, self.params.ptr, features.ptr,
+                                                       n, out.ptr))
 
     def gradient(self, features_ptr, targets_ptr, n, loss_scale, weights_ptr=None):
         need = _abi.lib().skyhip_surrogate_workspace_bytes(n)
@@ -224,9 +234,9 @@ class SkySurrogate:
                 self._ws.free()
             self._ws = self.ctx.alloc(need)
             self._ws_n = need
-        _abi.check(_abi.lib().skyhip_surrogate_gradient(self.ctx.handle, self.params.ptr, features_ptr, targets_ptr,
-                                                        weights_ptr, n, loss_scale, self.grad.ptr, self.loss.ptr,
-                                                        self._ws.ptr, self._ws_n))
+        _abi.check(_abi.lib().skyhip_surrogate_gradient(self.ctx.handle, self.network_id, self.params.ptr, features_ptr,
+                                                        targets_ptr, weights_ptr, n, loss_scale, self.grad.ptr,
+                                                        self.loss.ptr, self._ws.ptr, self._ws_n))
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         self.step += 1
@@ -279,7 +289,7 @@ class SkyPanoramas:
 
     def render_surrogate(self, model: SkySurrogate, tolerance=1e-2, radiance_floor=1e-6):
         _abi.check(_abi.lib().skyhip_sky_render_surrogate(
-            self.ctx.handle, model.params.ptr, C.byref(self.atm.luts), self.views.ptr, self.nv,
+            self.ctx.handle, model.network_id, model.params.ptr, C.byref(self.atm.luts), self.views.ptr, self.nv,
             self.atm.log_scales.ptr, self.atm.thickness, self.width, self.height, tolerance, radiance_floor,
             self.out.ptr, self.flags.ptr, self.counter.ptr))
 

@@ -1,13 +1,15 @@
 """GPU checks of the learned sky-view surrogate against the oracle.
 
-- Inference (register-resident weights, FP16 MFMA, packed FP16 activations): against the
-  FP64 oracle forward with the same FP16 operand rounding and the same activation
-  sequence; an FP32 pre-activation that lands next to an FP16 rounding boundary can round
-  differently than the oracle's FP64 sum, which moves one activation by one FP16 ulp, so at
+- Inference (register-resident weights; both networks, FP16 MFMA with packed FP16 gate
+  activations and BF16 MFMA with SiLU): against the FP64 oracle forward with the same
+  operand rounding and activation; an FP32 pre-activation that lands next to a rounding
+  boundary can round differently than the oracle's FP64 sum (and SiLU's hardware exp and
+  reciprocal differ from FP64 by an ulp), which moves one activation by one ulp, so at
   least 99 % of outputs agree within 1e-3 and all within 5e-2.
-- Gradient (training kernel, FP16 MFMA forward + backward, fixed-order reduction): against
-  the FP64 oracle gradient with the same rounding, per layer relative L2 < 1e-2 and cosine
-  similarity > 0.9999; against the unrounded FP64 gradient the FP16 effect is reported.
+- Gradient (training kernel, MFMA forward + backward, fixed-order reduction; both
+  networks): against the FP64 oracle gradient with the same rounding, per layer relative
+  L2 < 1e-2 and cosine similarity > 0.9999; against the unrounded FP64 gradient the
+  rounding effect is reported.
 - Labels (plain integrator per query): against the FP64 oracle, 1e-3 relative.
 - Training reduces the loss; the render pipeline's fallback recomputes exactly the flagged
   tiles with the plain integrator, and a trained network serves most tiles within the bound.
@@ -42,38 +44,40 @@ def _features(n, seed=4):
     return f
 
 
-def test_forward_matches_oracle(ctx):
+@pytest.mark.parametrize("network", ["f16_gate", "bf16_silu"])
+def test_forward_matches_oracle(ctx, network):
     p = _params()
     f = _features(1000)
-    model = sg.SkySurrogate(ctx, p)
+    model = sg.SkySurrogate(ctx, p, network=network)
     out = ctx.alloc(1000 * 12)
     model.forward(ctx.upload(f), 1000, out)
     got = out.download(np.float32, (1000, 3)).astype(np.float64)
-    ref = mlp.forward(p.astype(np.float64), f.astype(np.float64), half=True)
+    ref = mlp.forward(p.astype(np.float64), f.astype(np.float64), network=network, rounded=True)
     err = np.abs(got - ref) / (1.0 + np.abs(ref))
-    ref_fp64 = mlp.forward(p.astype(np.float64), f.astype(np.float64), half=False)
+    ref_fp64 = mlp.forward(p.astype(np.float64), f.astype(np.float64), network=network)
     e64 = np.abs(got - ref_fp64) / (1.0 + np.abs(ref_fp64))
-    print(f"forward vs FP16-rounded oracle: median {np.median(err):.2e}, p99 {np.quantile(err, 0.99):.2e}, "
+    print(f"{network}: forward vs the rounded oracle: median {np.median(err):.2e}, p99 {np.quantile(err, 0.99):.2e}, "
           f"max {err.max():.2e}; vs unrounded FP64: median {np.median(e64):.2e}, max {e64.max():.2e}")
     assert np.mean(err <= 1e-3) >= 0.99 and err.max() < 5e-2
 
 
-def test_gradient_matches_oracle(ctx):
+@pytest.mark.parametrize("network", ["f16_gate", "bf16_silu"])
+def test_gradient_matches_oracle(ctx, network):
     p = _params(seed=5, scale=1.0)
     n = 96  # three training workgroups
     f = _features(n, seed=6)
     t = np.random.default_rng(7).normal(-2.0, 1.0, size=(n, 3)).astype(np.float32)
     w = np.random.default_rng(8).uniform(0.1, 1.0, n).astype(np.float32)
-    model = sg.SkySurrogate(ctx, p)
+    model = sg.SkySurrogate(ctx, p, network=network)
     fd, td, wd = ctx.upload(f), ctx.upload(t), ctx.upload(w)
     scale = 1.0 / (3 * n)
     model.gradient(fd.ptr, td.ptr, n, scale, wd.ptr)
     g = model.grad.download(np.float32, (sg.PARAMS,)).astype(np.float64)
     loss = float(model.loss.download(np.float32, (1,))[0])
     ref_loss, ref = mlp.loss_and_grad(p.astype(np.float64), f.astype(np.float64), t.astype(np.float64), scale,
-                                      half=True, weights=w)
+                                      network=network, rounded=True, weights=w)
     _, ref64 = mlp.loss_and_grad(p.astype(np.float64), f.astype(np.float64), t.astype(np.float64), scale,
-                                 weights=w)
+                                 network=network, weights=w)
     assert abs(loss - ref_loss) <= 1e-4 * ref_loss
     o = 0
     for li, (k, nn) in enumerate(mlp.LAYERS):
@@ -82,7 +86,7 @@ def test_gradient_matches_oracle(ctx):
             rel = np.linalg.norm(a - b) / np.linalg.norm(b)
             cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
             rel64 = np.linalg.norm(a - c) / np.linalg.norm(c)
-            print(f"layer {li + 1} {name}: rel L2 {rel:.2e} (vs unrounded FP64 {rel64:.2e}), cosine {cos:.8f}")
+            print(f"{network} layer {li + 1} {name}: rel L2 {rel:.2e} (vs unrounded FP64 {rel64:.2e}), cosine {cos:.8f}")
             assert rel < 1e-2 and cos > 0.9999
             o += size
 
@@ -106,10 +110,11 @@ def test_labels_match_oracle(ctx):
     assert err.max() < 1e-3
 
 
-def test_training_reduces_loss(ctx):
+@pytest.mark.parametrize("network", ["f16_gate", "bf16_silu"])
+def test_training_reduces_loss(ctx, network):
     atm = sg.SkyAtmospheres(ctx, sg.draw_scales(16, 11))
     data = sg.SkyTrainingSet(ctx, atm, 1 << 16, seed=12)
-    model = sg.SkySurrogate(ctx, seed=13)
+    model = sg.SkySurrogate(ctx, seed=13, network=network)
     model.set_output_bias(data.targets_host.mean(axis=0))
     hist = model.train(data, epochs=6, batch=4096, lr=3e-3, lr_final=3e-4)
     print("loss per epoch:", [f"{h:.3e}" for h in hist])
@@ -153,12 +158,13 @@ def test_render_pipeline_fallback(ctx):
     assert e.max() < 1e-3
 
 
-def test_render_pass_matches_forward_entry_points(ctx):
+@pytest.mark.parametrize("network", ["f16_gate", "bf16_silu"])
+def test_render_pass_matches_forward_entry_points(ctx, network):
     scales, views = sg.evaluation_views(2, seed=16)
     atm = sg.SkyAtmospheres(ctx, scales, views=views)
     w, h = 200, 90  # 18,000 pixels per view: the last 64-pixel run of the last view is partial
     pano = sg.SkyPanoramas(ctx, atm, views, w, h)
-    model = sg.SkySurrogate(ctx, _params(seed=17, scale=1.0))
+    model = sg.SkySurrogate(ctx, _params(seed=17, scale=1.0), network=network)
     pano.render_surrogate(model, tolerance=1e30)
     img = pano.image()[..., :3].astype(np.float64)
     ys, xs = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
@@ -186,7 +192,7 @@ def test_render_pass_matches_forward_entry_points(ctx):
         # from the device's cosf by an ulp and move an FP16 input by one ulp; neighbouring
         # pixels differ by far more than such a change.
         e = e[~probe]
-        print(f"view {v}: render vs feature + forward entry points, median rel {np.median(e):.2e}, "
+        print(f"{network} view {v}: render vs feature + forward entry points, median rel {np.median(e):.2e}, "
               f"99.9 % {np.quantile(e, 0.999):.2e}, max {e.max():.2e}")
         assert np.quantile(e, 0.999) < 1e-4 and e.max() < 1e-2
 

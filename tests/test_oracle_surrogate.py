@@ -1,26 +1,38 @@
 """Pins the surrogate's oracle (oracle/mlp.py) on the CPU.
 
-- FP16 rounding (to nearest even) against torch's float32 -> float16 conversion; rounding
+- FP16 and BF16 rounding (to nearest even) against torch's float32 conversions; rounding
   toward zero against its definition (the FP16 value of largest magnitude not above |x|,
   found by stepping through the FP16 grid) and the range limit.
 - The activation: its closed form at points where t is 0, 1/2 and 1, its derivative against
   central differences, and the FP16 operation sequence against the FP64 function within
   a few FP16 ulps.
-- The analytic gradient against central finite differences in FP64 (no FP16 rounding,
-  which would make the loss piecewise constant), for every layer's weights and biases.
-- A one-hidden-unit special case in closed form: y = w2 a(a(a(a(w1 x + b1)))) + b2.
+- The analytic gradient against central finite differences in FP64 (no operand rounding,
+  which would make the loss piecewise constant), for every layer's weights and biases, for
+  both networks.
+- A one-hidden-unit special case in closed form: y = w2 a(a(a(a(w1 x + b1)))) + b2, for the
+  gate and for SiLU (against math.exp).
 - The FP64 sky-radiance query against the sky-view texel path it restates: a query
   along a sky-view texel's direction equals the oracle's sky-view LUT texel.
 """
 import math
 
 import numpy as np
+import pytest
 import torch
 
 import oracle
 from oracle import mlp
 from skyrendering_amd import presets
 from skyrendering_amd.atmosphere import AtmosphereRenderParameters
+
+
+def test_bf16_rounding_matches_torch():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(20000) * 10.0 ** rng.uniform(-8, 8, 20000),
+                        np.array([0.0, 1.0, -1.0, 1.00390625, 1.005859375, 3.0e38])]).astype(np.float32)
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    got = mlp.bf16_round(x).astype(np.float32)
+    assert np.array_equal(got, want)
 
 
 def test_f16_rounding_matches_torch():
@@ -54,24 +66,27 @@ def test_activation_closed_form_and_derivative():
     assert np.all(err <= 4 * 2.0 ** -10 * np.maximum(np.abs(z), 2.0 ** -4))
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("network", mlp.NETWORKS)
+def test_gradient_matches_finite_differences(network):
     rng = np.random.default_rng(1)
     p = rng.standard_normal(mlp.PARAMS) * 0.3
     x = rng.uniform(-1, 1, size=(9, 8))
     t = rng.standard_normal((9, 3))
-    loss, g = mlp.loss_and_grad(p, x, t, 0.1)
-    assert abs(loss - 0.1 * np.sum((mlp.forward(p, x) - t) ** 2)) < 1e-12
+    loss, g = mlp.loss_and_grad(p, x, t, 0.1, network=network)
+    assert abs(loss - 0.1 * np.sum((mlp.forward(p, x, network=network) - t) ** 2)) < 1e-12
     idx = np.concatenate([rng.choice(np.arange(o, o + k * n + n), 12, replace=False)
                           for o, (k, n) in zip(np.cumsum([0] + [k * n + n for k, n in mlp.LAYERS[:-1]]), mlp.LAYERS)])
     h = 1e-6
     for i in idx:
         e = np.zeros_like(p)
         e[i] = h
-        fd = (mlp.loss_and_grad(p + e, x, t, 0.1)[0] - mlp.loss_and_grad(p - e, x, t, 0.1)[0]) / (2 * h)
+        fd = (mlp.loss_and_grad(p + e, x, t, 0.1, network=network)[0] -
+              mlp.loss_and_grad(p - e, x, t, 0.1, network=network)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
-def test_single_path_closed_form():
+@pytest.mark.parametrize("network", mlp.NETWORKS)
+def test_single_path_closed_form(network):
     """Only unit 0 of each hidden layer active with unit weights: the network reduces to
     nested activations of the first input, written out here."""
     layers = [(np.zeros((n, k)), np.zeros(n)) for k, n in mlp.LAYERS]
@@ -83,11 +98,13 @@ def test_single_path_closed_form():
     layers[4][1][:] = [0.5, 0.0, 0.0]
     p = mlp.pack(layers)
     def s(z):
+        if network == "bf16_silu":
+            return z / (1.0 + math.exp(-z))
         t = min(max(z / 6.0 + 0.5, 0.0), 1.0)
         return z * t * t * (3.0 - 2.0 * t)
     for x0 in (-2.0, 0.3, 1.7):
         h = s(s(s(s(1.5 * x0 + 0.25))))
-        y = mlp.forward(p, np.array([[x0, 0, 0, 0, 0, 0, 0, 0]]))[0]
+        y = mlp.forward(p, np.array([[x0, 0, 0, 0, 0, 0, 0, 0]]), network=network)[0]
         assert np.allclose(y, [h + 0.5, 2 * h, -h], rtol=1e-14, atol=1e-15)
 
 
