@@ -77,7 +77,8 @@ def parse():
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--surrogate-epochs", type=int, default=1200)
+    ap.add_argument("--surrogate-epochs", type=int, default=2400)
+    ap.add_argument("--surrogate-no-bf16", action="store_true", help="skip the BF16 / SiLU network comparison")
     return ap.parse_args()
 
 
@@ -415,13 +416,14 @@ SURROGATE_MAX_ALTITUDE_KM = 10.0
 SURROGATE_MAX_SUN_ZENITH_DEG = 90.0
 
 
-def bench_surrogate(ctx, epochs=1200, width=1920, height=1080, n_views=64, cpu_seconds=4.0, compare_bf16=True):
+def bench_surrogate(ctx, epochs=2400, width=1920, height=1080, n_views=64, cpu_seconds=4.0, compare_bf16=True,
+                    compare_epochs=1200):
     """Trains the sky-view surrogate on 2^22 plain-integrator samples (1024 atmospheres) drawn
     uniformly over the panoramas' domain, then renders n_views panoramas three ways: plain
     integrator per pixel, surrogate with the per-tile residual check and fallback (tolerance
     1e-2), and the sky-view LUT sampled bilinearly.  The default network is the FP16 one
-    with the gated activation; the BF16 / SiLU network is trained on the same data and timed
-    beside it (`bf16_silu`)."""
+    with the gated activation; the BF16 / SiLU network is trained on the same data for
+    `compare_epochs` epochs and timed beside it (`bf16_silu`)."""
     from skyrendering_amd import surrogate as sg
     t0 = time.perf_counter()
     atm = sg.SkyAtmospheres(ctx, sg.draw_scales(1024, 41))
@@ -473,9 +475,10 @@ def bench_surrogate(ctx, epochs=1200, width=1920, height=1080, n_views=64, cpu_s
         alt = sg.SkySurrogate(ctx, seed=43, network="bf16_silu")
         alt.set_output_bias(data.targets_host.mean(axis=0))
         t0 = time.perf_counter()
-        alt_hist = alt.train(data, epochs=epochs, batch=8192, lr=2e-3, lr_final=2e-5)
+        alt_hist = alt.train(data, epochs=compare_epochs, batch=8192, lr=2e-3, lr_final=2e-5)
         ctx.synchronize()
-        b = {"network": "BF16 MFMA operands, SiLU", "train_s": round(time.perf_counter() - t0, 2),
+        b = {"network": "BF16 MFMA operands, SiLU", "epochs": compare_epochs,
+             "train_s": round(time.perf_counter() - t0, 2),
              "final_loss": alt_hist[-1], "surrogate_ms": round(timed(lambda: pano.render_surrogate(alt)), 3)}
         b["fallback_tile_fraction"] = round(pano.fallback_tiles() / (pano.tiles * n_views), 4)
         b["network_pass_ms"] = round(timed(lambda: pano.render_surrogate(alt, tolerance=1e30)), 3)
@@ -650,7 +653,7 @@ def main():
         batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
         progress(f"LUT batch {out['lut_batch']['ms_per_launch']} ms per launch")
     if not args.no_surrogate:
-        out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs)
+        out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs, compare_bf16=not args.surrogate_no_bf16)
         progress(f"surrogate {out['surrogate']['surrogate_ms']} ms vs plain {out['surrogate']['plain_ms']} ms")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
