@@ -653,7 +653,8 @@ def main():
         batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
         progress(f"LUT batch {out['lut_batch']['ms_per_launch']} ms per launch")
     if not args.no_surrogate:
-        out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs, compare_bf16=not args.surrogate_no_bf16)
+        out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs, compare_bf16=not args.surrogate_no_bf16,
+                                           compare_epochs=min(1200, args.surrogate_epochs))
         progress(f"surrogate {out['surrogate']['surrogate_ms']} ms vs plain {out['surrogate']['plain_ms']} ms")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
