@@ -117,3 +117,15 @@ def test_cloud_assign_buffer_data_builds_local_frame():
     assert abs(common.uSunDirection[2] - math.sin(math.radians(20))) < 1e-6
     assert common.uTopAltitude == np.float32(1.5) + np.float32(2.5)
     assert common.uBaseShadingIndex == 1 and common.uFrameID == 5.0
+
+
+def test_surrogate_network_kinds_match_the_header():
+    import os
+    import re
+    from skyrendering_amd import surrogate as sg
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "include", "skyhip.h")).read()
+    kinds = {m.group(1): int(m.group(2)) for m in re.finditer(r"SKYHIP_SURROGATE_(\w+) = (\d+)", text)}
+    assert kinds == {"F16_GATE": sg.NETWORKS["f16_gate"], "BF16_SILU": sg.NETWORKS["bf16_silu"]}
+    with pytest.raises(ValueError):
+        sg.SkySurrogate(None, network="fp32")
