@@ -13,9 +13,10 @@ multiple scattering 32x32x64/30, sky view 192x108/30, aerial perspective 32^3/50
 in ms per set over 16 sun angles; the roofline of the raymarch kernel; the voxel-cloud
 path tracer (1024x1024, 64 frames of one sample, brick majorants, with the single-majorant
 tracker timed beside it); the batched LUT build (256 atmosphere sets x 8 suns, mixed sizes,
-one hipGraph launch, autotuned); the learned sky-view surrogate (trained here on 2^22
-samples, then 64 panoramas at 1920x1080 rendered by the plain integrator, by the surrogate
-with its per-tile residual check and fallback, and by sky-view LUT sampling); the cloud
+one hipGraph launch, autotuned); the learned sky-view surrogate (the FP16 network trained here
+for 2,400 epochs on 2^22 samples, the BF16 / SiLU network for 1,200 beside it, then 64
+panoramas at 1920x1080 rendered by the plain integrator, by each network with its per-tile
+residual check and fallback, and by sky-view LUT sampling); the cloud
 shadow passes (512^2 map, blur, 160x90x128 froxel); the quarter-resolution cloud pipeline
 (checkerboard depth, shading index, temporal reconstruction, upscale composite at 1080p, and
 the frame with its 480x270 raymarch); the environment cube with its IBL products; and for
