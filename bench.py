@@ -117,7 +117,7 @@ def bench_lut_set(ctx, rounds):
 
 def bench_pathtrace(ctx, frames, reference_frames=1):
     """The voxel-cloud path tracer at 1024x1024: `frames` frames of one sample each (the
-    workload's 64 spp) with 8^3-brick majorants, timed with HIP events; the reference's
+    workload's 64 spp) with 16^3-brick majorants, timed with HIP events; the reference's
     single-majorant tracking is timed over `reference_frames` frames beside it."""
     from skyrendering_amd import presets
     from skyrendering_amd.pathtrace import PathTraceScene
@@ -151,7 +151,7 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     return scene, {
         "workload": f"voxel-cloud path tracing {scene.width}x{scene.height}, {w['spp']} spp (frames of 1 sample), "
                     f"512^3 FP16 fBm-lognormal grid, <= {w['init'].max_bounces} bounces, albedo {w['init'].albedo}, "
-                    f"HG g={w['init'].forward_phase_g}, sun + sky-view environment, 8^3-brick majorants",
+                    f"HG g={w['init'].forward_phase_g}, sun + sky-view environment, 16^3-brick majorants",
         "ms_per_frame": round(ms, 4), "ms_per_64spp": round(ms * 64, 2),
         "msamples_per_s": round(n / (ms * 1e-3) / 1e6, 2),
         "density_lookups_per_frame": counts["lookups"], "majorant_segments_per_frame": counts["segments"],

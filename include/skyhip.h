@@ -713,7 +713,7 @@ typedef struct SkyhipPathTracingParameters {
  *                    level 0 of the reference's environment-luminance cube up to resampling
  *                    (AtmosphereRenderer.glsl:248-275);
  *   brick_majorants: NULL to track with the global sigma_t_max as the reference does, or the
- *                    float[ceil(dim/8)^3] output of skyhip_voxel_grid_majorants for a
+ *                    float[ceil(dim/16)^3] output of skyhip_voxel_grid_majorants for a
  *                    VOXEL_F16 material (scaled by the voxel block's uDensity on use). */
 typedef struct SkyhipPathTracingInputs {
     const float* transmittance;
@@ -753,10 +753,12 @@ typedef struct SkyhipVoxelGridCreateInfo {
 int skyhip_voxel_grid_generate(SkyhipContext* ctx, const SkyhipVoxelGridCreateInfo* info, uint16_t* out_grid,
                                int32_t dim_x, int32_t dim_y, int32_t dim_z);
 
-/* Majorant per 8^3 brick of an FP16 grid: the maximum over the brick's texels and a
+/* Edge of a majorant brick in texels (16). */
+int skyhip_voxel_grid_brick_size(void);
+/* Majorant per 16^3 brick of an FP16 grid: the maximum over the brick's texels and a
  * one-texel apron on every side (clamped to the grid), which bounds trilinear
  * CLAMP_TO_BORDER(0) samples anywhere in the brick.  out_majorants: device
- * float[ceil(dim_x/8) * ceil(dim_y/8) * ceil(dim_z/8)], x fastest. */
+ * float[ceil(dim_x/16) * ceil(dim_y/16) * ceil(dim_z/16)], x fastest. */
 int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
                                 int32_t dim_z, float* out_majorants);
 

@@ -362,10 +362,14 @@ def voxel_grid_density(ci, dims, xyz):
     return out
 
 
+BRICK_TEXELS = 16  # kBrickTexels of oracle_pathtrace.h
+
+
 def brick_majorants(grid):
     g = np.ascontiguousarray(grid, dtype=np.uint16)
     dz, dy, dx = g.shape
-    out = np.empty(((dz + 7) // 8, (dy + 7) // 8, (dx + 7) // 8), np.float32)
+    b = BRICK_TEXELS
+    out = np.empty(((dz + b - 1) // b, (dy + b - 1) // b, (dx + b - 1) // b), np.float32)
     lib().oracle_brick_majorants(_ptr(g), dx, dy, dz, _ptr(out))
     return out
 

@@ -470,7 +470,8 @@ void oracle_voxel_grid_density_f32(const SkyhipVoxelGridCreateInfo* ci, int dx, 
 }
 void oracle_brick_majorants(const uint16_t* grid, int dx, int dy, int dz, float* out) {
     const int dim[3] = {dx, dy, dz};
-    int bx = (dx + 7) / 8, by = (dy + 7) / 8, bz = (dz + 7) / 8;
+    const int B = kBrickTexels;
+    int bx = (dx + B - 1) / B, by = (dy + B - 1) / B, bz = (dz + B - 1) / B;
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < bz; ++z)
         for (int y = 0; y < by; ++y)

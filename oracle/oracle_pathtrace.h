@@ -7,7 +7,7 @@
 // (AtmosphereRenderer.glsl:253-273) evaluated directly from the sky-view LUT.
 // Tracking uses the reference's single majorant kSigmaTMax, so each sample
 // consumes the reference's random sequence, or (when brick majorants are
-// given) the library's 8^3-brick majorant grid walked in texel space: the same
+// given) the library's 16^3-brick majorant grid walked in texel space: the same
 // tracking estimators with a piecewise-constant majorant, drawing random
 // numbers in the same order as the library's brick tracker.  Only the tests
 // and the benchmark's CPU baseline use this code.
@@ -83,13 +83,17 @@ inline float VoxelGridDensity(int x, int y, int z, const int dim[3], const Skyhi
     return std::fmax(std::exp(ci.lognormal_sigma * ci.fbm_inv_std * fbm) * falloff - ci.threshold, 0.0f);
 }
 
-// Brick majorant: max over the 8^3 brick and a one-texel apron, clamped to the grid.
+// Texels per brick edge of the library's majorant grid.
+constexpr int kBrickTexels = 16;
+
+// Brick majorant: max over the kBrickTexels^3 brick and a one-texel apron, clamped to the grid.
 inline float BrickMajorant(const uint16_t* g, const int dim[3], int bx, int by, int bz) {
     float m = 0.0f;
-    int lo[3] = {bx * 8 - 1, by * 8 - 1, bz * 8 - 1};
-    for (int z = std::max(lo[2], 0); z <= std::min(lo[2] + 9, dim[2] - 1); ++z)
-        for (int y = std::max(lo[1], 0); y <= std::min(lo[1] + 9, dim[1] - 1); ++y)
-            for (int x = std::max(lo[0], 0); x <= std::min(lo[0] + 9, dim[0] - 1); ++x)
+    const int B = kBrickTexels;
+    int lo[3] = {bx * B - 1, by * B - 1, bz * B - 1};
+    for (int z = std::max(lo[2], 0); z <= std::min(lo[2] + B + 1, dim[2] - 1); ++z)
+        for (int y = std::max(lo[1], 0); y <= std::min(lo[1] + B + 1, dim[1] - 1); ++y)
+            for (int x = std::max(lo[0], 0); x <= std::min(lo[0] + B + 1, dim[0] - 1); ++x)
                 m = std::fmax(m, half_to_float(g[(size_t(z) * dim[1] + y) * dim[0] + x]));
     return m;
 }
@@ -150,7 +154,7 @@ struct PathTracer {
     int tw = 0, th = 0, sw = 0, sh = 0, aw = 0, ah = 0, ad = 0;
     V3<S> cam, sun, box_min, box_max;
     S R, hb, ht;
-    // Brick-majorant grid (VOXEL_F16 only): one majorant per 8^3 brick, in texel space
+    // Brick-majorant grid (VOXEL_F16 only): one majorant per kBrickTexels^3 brick, in texel space
     // X = pos * tex_scale + tex_bias per axis (z measured from the cloud bottom).
     const float* majorants = nullptr;
     int nb[3] = {0, 0, 0};
@@ -176,7 +180,7 @@ struct PathTracer {
         majorants = m;
         const SkyhipCloudVoxelBufferData& vb = mat.voxel;
         for (int k = 0; k < 3; ++k) {
-            nb[k] = (mat.voxel_grid_dim[k] + 7) / 8;
+            nb[k] = (mat.voxel_grid_dim[k] + kBrickTexels - 1) / kBrickTexels;
             grid_n[k] = S(mat.voxel_grid_dim[k]);
         }
         for (int k = 0; k < 2; ++k) {
@@ -196,8 +200,8 @@ struct PathTracer {
         int b[3];
     };
     S next_boundary(const Walk& w, int i) const {
-        if (w.k[i] > S(0)) return w.b[i] < nb[i] - 1 ? (S(8 * (w.b[i] + 1)) - w.org[i]) * w.inv_k[i] : S(3.0e38);
-        if (w.k[i] < S(0)) return w.b[i] > 0 ? (S(8 * w.b[i]) - w.org[i]) * w.inv_k[i] : S(3.0e38);
+        if (w.k[i] > S(0)) return w.b[i] < nb[i] - 1 ? (S(kBrickTexels * (w.b[i] + 1)) - w.org[i]) * w.inv_k[i] : S(3.0e38);
+        if (w.k[i] < S(0)) return w.b[i] > 0 ? (S(kBrickTexels * w.b[i]) - w.org[i]) * w.inv_k[i] : S(3.0e38);
         return S(3.0e38);
     }
     bool walk_begin(Walk& w, V3<S> o, V3<S> d, S& t, S t_stop) const {
@@ -223,7 +227,7 @@ struct PathTracer {
         w.t_end = hi;
         for (int i = 0; i < 3; ++i) {
             S x = t * w.k[i] + w.org[i];
-            w.b[i] = std::min(std::max(int(std::floor(x * S(0.125))), 0), nb[i] - 1);
+            w.b[i] = std::min(std::max(int(std::floor(x / S(kBrickTexels))), 0), nb[i] - 1);
             w.tn[i] = next_boundary(w, i);
         }
         return true;

@@ -9,7 +9,7 @@
 // sample to the pixel's accumulator in frame order as the reference's
 // per-frame imageLoad/imageStore does.  Tracking runs either with the
 // reference's single majorant kSigmaTMax, drawing the reference's random
-// sequence, or over the 8^3-brick majorant grid of an FP16 density volume,
+// sequence, or over the 16^3-brick majorant grid of an FP16 density volume,
 // walked with a 3D DDA in texel space: empty bricks cost one majorant fetch,
 // and inside occupied bricks the tentative-collision rate follows the local
 // maximum instead of the global one.
@@ -122,8 +122,14 @@ __device__ __forceinline__ float sigma_at(const PtArgs& a, F3 p) {
     return sample_sigma_t<kMat>(a.mat, p, clamp01((p.z - a.hb) * a.inv_thickness), a.cam);
 }
 
+// Texels per brick edge of the majorant grid.  Measured on the 1024^2 workload (512^3 grid):
+// 8 -> 11.25 ms per frame (82 M lookups, 240 M segments), 16 -> 10.70-10.86 ms (119 M, 123 M),
+// 32 -> 14.68 ms: past 16 the looser majorants cost more null collisions than the shorter
+// walk saves.
+constexpr int kBrick = 16;
+
 // Majorant-segment walker over the brick grid: the ray's texel-space position is
-// X(t) = org + t k per axis.  Bricks are 8 texels wide; the first and last brick of
+// X(t) = org + t k per axis.  Bricks are kBrick texels wide; the first and last brick of
 // the x and y axes extend one texel beyond the grid, where CLAMP_TO_BORDER samples
 // are still non-zero, so the walk covers [-1, N + 1] there and [0, N] in z.
 struct BrickWalk {
@@ -158,14 +164,14 @@ struct BrickWalk {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             float x = fmaf(t, k[i], org[i]);
-            b[i] = min(max(int(floorf(x * 0.125f)), 0), a.nb[i] - 1);
+            b[i] = min(max(int(floorf(x * (1.0f / kBrick))), 0), a.nb[i] - 1);
             tn[i] = next_boundary(a, i);
         }
         return true;
     }
     __device__ __forceinline__ float next_boundary(const PtArgs& a, int i) const {
-        if (k[i] > 0.0f) return b[i] < a.nb[i] - 1 ? (float(8 * (b[i] + 1)) - org[i]) * inv_k[i] : 3.0e38f;
-        if (k[i] < 0.0f) return b[i] > 0 ? (float(8 * b[i]) - org[i]) * inv_k[i] : 3.0e38f;
+        if (k[i] > 0.0f) return b[i] < a.nb[i] - 1 ? (float(kBrick * (b[i] + 1)) - org[i]) * inv_k[i] : 3.0e38f;
+        if (k[i] < 0.0f) return b[i] > 0 ? (float(kBrick * b[i]) - org[i]) * inv_k[i] : 3.0e38f;
         return 3.0e38f;
     }
     __device__ __forceinline__ float segment_end() const { return fminf(fminf(tn[0], tn[1]), fminf(tn[2], t_end)); }
@@ -472,15 +478,15 @@ __global__ __launch_bounds__(256) void path_trace_display_kernel(const float4* _
     for (int k = 0; k < 3; ++k) px[k] = __builtin_bit_cast(uint16_t, __float2half_rn(out[k]));
 }
 
-// Brick majorants of an FP16 grid: max over the brick plus a one-texel apron.
+// Brick majorants of an FP16 grid: max over the kBrick^3 brick plus a one-texel apron.
 __global__ __launch_bounds__(256) void majorant_kernel(const uint16_t* __restrict__ g, int nx, int ny, int nz,
                                                        int bx, int by, int bz, float* __restrict__ out) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= bx * by * bz) return;
     int x = i % bx, y = (i / bx) % by, z = i / (bx * by);
-    int x0 = max(8 * x - 1, 0), x1 = min(8 * x + 8, nx - 1);
-    int y0 = max(8 * y - 1, 0), y1 = min(8 * y + 8, ny - 1);
-    int z0 = max(8 * z - 1, 0), z1 = min(8 * z + 8, nz - 1);
+    int x0 = max(kBrick * x - 1, 0), x1 = min(kBrick * x + kBrick, nx - 1);
+    int y0 = max(kBrick * y - 1, 0), y1 = min(kBrick * y + kBrick, ny - 1);
+    int z0 = max(kBrick * z - 1, 0), z1 = min(kBrick * z + kBrick, nz - 1);
     float m = 0.0f;
     for (int zz = z0; zz <= z1; ++zz)
         for (int yy = y0; yy <= y1; ++yy) {
@@ -520,13 +526,15 @@ static void launch_path_trace(hipStream_t st, const PtArgs& a) {
 
 using namespace skyhip;
 
+extern "C" int skyhip_voxel_grid_brick_size(void) { return kBrick; }
+
 extern "C" int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
                                            int32_t dim_z, float* out_majorants) {
     SKYHIP_CHECK(check_ctx(ctx));
     if (!grid || !out_majorants) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
     if (dim_x < 2 || dim_y < 2 || dim_z < 2 || dim_x > 4096 || dim_y > 4096 || dim_z > 4096)
         return fail(SKYHIP_ERR_INVALID_SIZE, "grid dims outside [2, 4096]");
-    int bx = div_up(dim_x, 8), by = div_up(dim_y, 8), bz = div_up(dim_z, 8);
+    int bx = div_up(dim_x, kBrick), by = div_up(dim_y, kBrick), bz = div_up(dim_z, kBrick);
     hipLaunchKernelGGL(majorant_kernel, dim3(div_up(bx * by * bz, 256)), dim3(256), 0, ctx->stream, grid, dim_x,
                        dim_y, dim_z, bx, by, bz, out_majorants);
     SKYHIP_LAUNCH_CHECK("majorant_kernel");
@@ -633,7 +641,7 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
         const SkyhipCloudVoxelBufferData& vb = material->voxel;
         a.majorants = in->brick_majorants;
         for (int k = 0; k < 3; ++k) {
-            a.nb[k] = div_up(material->voxel_grid_dim[k], 8);
+            a.nb[k] = div_up(material->voxel_grid_dim[k], kBrick);
             a.grid_n[k] = float(material->voxel_grid_dim[k]);
         }
         // X = (pos * freq + bias) * N ; Z = (z - hb) / (ht - hb) * N (BrickWalk subtracts hb)

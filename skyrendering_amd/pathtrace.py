@@ -5,7 +5,7 @@ VolumetricCloud.cpp:495-581) over the HIP kernels of skyhip_cloud_path_trace.
 rendered mask at the viewport size, a frame counter that starts at 1, a tile schedule of
 sqrt_tile_count^2 regions (GetRenderRegion), and a display pass that composites the running
 mean over an RGBA16F image.  `VoxelGrid` builds the FP16 density volume of the path-tracing
-workload and its 8^3-brick majorants on the device.
+workload and its 16^3-brick majorants on the device.
 """
 from __future__ import annotations
 
@@ -22,6 +22,7 @@ from .device import Context, DeviceBuffer
 _P = C.POINTER
 _abi.register("skyhip_voxel_grid_generate", C.c_int, [C.c_void_p, _P(_abi.VoxelGridCreateInfoC), C.c_void_p,
                                                       C.c_int32, C.c_int32, C.c_int32])
+_abi.register("skyhip_voxel_grid_brick_size", C.c_int, [])
 _abi.register("skyhip_voxel_grid_majorants", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                                        C.c_void_p])
 _abi.register("skyhip_cloud_path_trace", C.c_int, [
@@ -107,7 +108,8 @@ class VoxelGrid:
         self.dims = tuple(int(d) for d in dims)
         dx, dy, dz = self.dims
         self.density = ctx.alloc(dx * dy * dz * 2)
-        self.brick_dims = tuple((d + 7) // 8 for d in self.dims)
+        b = _abi.lib().skyhip_voxel_grid_brick_size()
+        self.brick_dims = tuple((d + b - 1) // b for d in self.dims)
         bx, by, bz = self.brick_dims
         self.majorants = ctx.alloc(bx * by * bz * 4)
 

@@ -173,7 +173,7 @@ def uniform_blue_noise(size=128, seed=11) -> np.ndarray:
 # from the 192x108 sky-view LUT of the Earth preset, sun zenith 70 degrees.  Density:
 # a 512^3 FP16 grid (seeded 5-octave Perlin fBm, lognormal sigma 1.2, ellipsoidal
 # falloff, threshold 1) spanning 8 x 8 km over the 1.5-4 km cloud layer, centred 6 km
-# ahead of the camera, extinction = grid value x 8 / km.  Tracking: 8^3-brick majorants.
+# ahead of the camera, extinction = grid value x 8 / km.  Tracking: 16^3-brick majorants.
 # Region box half width 14 km (encloses the grid; the reference's default is 100 km).
 # ---------------------------------------------------------------------------
 def path_trace_workload():

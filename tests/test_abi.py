@@ -129,3 +129,8 @@ def test_surrogate_network_kinds_match_the_header():
     assert kinds == {"F16_GATE": sg.NETWORKS["f16_gate"], "BF16_SILU": sg.NETWORKS["bf16_silu"]}
     with pytest.raises(ValueError):
         sg.SkySurrogate(None, network="fp32")
+
+
+def test_brick_size_matches_the_oracle():
+    import oracle
+    assert _abi.lib().skyhip_voxel_grid_brick_size() == oracle.BRICK_TEXELS == 16

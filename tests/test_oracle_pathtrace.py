@@ -70,15 +70,17 @@ def test_brick_majorants_bound_trilinear_samples():
     g = oracle.voxel_grid_f16(ci.to_c(), dims)
     gf = g.view(np.float16).astype(np.float32)
     M = oracle.brick_majorants(g)
+    B = 16  # the library's brick edge in texels
+    assert oracle.BRICK_TEXELS == B
     nz, ny, nx = gf.shape
     ref = np.zeros_like(M)
     for bz in range(M.shape[0]):
         for by in range(M.shape[1]):
             for bx in range(M.shape[2]):
-                ref[bz, by, bx] = gf[max(8 * bz - 1, 0):8 * bz + 9, max(8 * by - 1, 0):8 * by + 9,
-                                     max(8 * bx - 1, 0):8 * bx + 9].max()
+                ref[bz, by, bx] = gf[max(B * bz - 1, 0):B * bz + B + 1, max(B * by - 1, 0):B * by + B + 1,
+                                     max(B * bx - 1, 0):B * bx + B + 1].max()
     assert np.array_equal(M, ref)
-    # Bound: a sample anywhere in a brick (texel-space X = u N in [8b, 8b + 8)), and in the
+    # Bound: a sample anywhere in a brick (texel-space X = u N in [B b, B b + B)), and in the
     # one-texel shell outside the grid, never exceeds the brick's majorant.
     mat = _abi.CloudMaterial()
     mat.kind = _abi.MATERIAL_VOXEL_F16
@@ -95,7 +97,7 @@ def test_brick_majorants_bound_trilinear_samples():
     X[:, 2] = np.clip(X[:, 2], 0, dims[2])
     for x in X:
         s = oracle.pt_sigma_t(common, mat, prm, (x[0] / dims[0], x[1] / dims[1], x[2] / dims[2]))
-        b = [min(max(int(math.floor(x[k] / 8)), 0), M.shape[2 - k] - 1) for k in range(3)]
+        b = [min(max(int(math.floor(x[k] / B)), 0), M.shape[2 - k] - 1) for k in range(3)]
         assert s <= M[b[2], b[1], b[0]] * (1 + 1e-12)
 
 
@@ -251,18 +253,19 @@ def _tracking_z_scores(common, mat, prm, rays, majorants, n=4000):
 
 
 def test_brick_tracking_is_unbiased():
-    """Delta and ratio tracking over the 8^3-brick majorant grid (and over the single
+    """Delta and ratio tracking over the 16^3-brick majorant grid (and over the single
     majorant) against exp(-optical depth) from quadrature of sigma along each ray.  The grid
     is constant per brick with a gradient across bricks and a three-brick empty gap, so the
     majorants vary and some bricks are empty; rays are oblique, cross the gap, and two graze
     the half-texel shell outside the grid where CLAMP_TO_BORDER samples are still non-zero.
     The same statistics reject majorants scaled by 0.7 and majorants mirrored along x (an
     index error), so they have the power to see a wrong walk."""
-    dims = (64, 48, 32)
-    bz, by, bx = np.meshgrid(np.arange(4), np.arange(6), np.arange(8), indexing="ij")
-    vals = (0.25 * (bx + 2 * by + bz)).astype(np.float16)
+    B = oracle.BRICK_TEXELS
+    dims = (9 * B, 4 * B, 2 * B)
+    bz, by, bx = np.meshgrid(np.arange(2), np.arange(4), np.arange(9), indexing="ij")
+    vals = (0.125 * (bx + 3 * by + 7 * bz)).astype(np.float16)
     vals[:, :, 2:5] = 0
-    g = np.ascontiguousarray(np.repeat(np.repeat(np.repeat(vals, 8, 0), 8, 1), 8, 2).view(np.uint16))
+    g = np.ascontiguousarray(np.repeat(np.repeat(np.repeat(vals, B, 0), B, 1), B, 2).view(np.uint16))
     M = oracle.brick_majorants(g)
     assert (M == 0).any() and len(np.unique(M)) > 10
     mat = _abi.CloudMaterial()
