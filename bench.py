@@ -56,7 +56,7 @@ FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
     "cloud_raymarch_kernel<1,false>": (8141.5, 27958.8),
-    "path_trace_kernel<4,true,false>": (5.3842e6, 2.0778e5),
+    "path_trace_kernel<4,true,false>": (6.9093e6, 2.0778e5),
 }
 
 
