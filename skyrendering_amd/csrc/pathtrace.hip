@@ -123,9 +123,9 @@ __device__ __forceinline__ float sigma_at(const PtArgs& a, F3 p) {
 }
 
 // Texels per brick edge of the majorant grid.  Measured on the 1024^2 workload (512^3 grid):
-// 8 -> 11.25 ms per frame (82 M lookups, 240 M segments), 16 -> 10.70-10.86 ms (119 M, 123 M),
-// 32 -> 14.68 ms: past 16 the looser majorants cost more null collisions than the shorter
-// walk saves.
+// 8 -> 11.25 ms per frame (82 M lookups, 240 M segments), 12 -> 10.76 ms (102 M, 164 M),
+// 16 -> 10.70-10.86 ms (119 M, 123 M), 20 -> 12.02 ms, 24 -> 12.84 ms, 32 -> 14.68 ms: past 16
+// the looser majorants cost more null collisions than the shorter walk saves.
 constexpr int kBrick = 16;
 
 // Majorant-segment walker over the brick grid: the ray's texel-space position is
