@@ -418,7 +418,7 @@ SURROGATE_MAX_SUN_ZENITH_DEG = 90.0
 
 
 def bench_surrogate(ctx, epochs=2400, width=1920, height=1080, n_views=64, cpu_seconds=4.0, compare_bf16=True,
-                    compare_epochs=1200):
+                    compare_epochs=1200, train_samples=1 << 22):
     """Trains the sky-view surrogate on 2^22 plain-integrator samples (1024 atmospheres) drawn
     uniformly over the panoramas' domain, then renders n_views panoramas three ways: plain
     integrator per pixel, surrogate with the per-tile residual check and fallback (tolerance
@@ -429,7 +429,7 @@ def bench_surrogate(ctx, epochs=2400, width=1920, height=1080, n_views=64, cpu_s
     t0 = time.perf_counter()
     atm = sg.SkyAtmospheres(ctx, sg.draw_scales(1024, 41))
     mu_sun_min = float(np.cos(np.radians(SURROGATE_MAX_SUN_ZENITH_DEG)))
-    data = sg.SkyTrainingSet(ctx, atm, 1 << 22, seed=42, altitude_range=(0.0, SURROGATE_MAX_ALTITUDE_KM),
+    data = sg.SkyTrainingSet(ctx, atm, train_samples, seed=42, altitude_range=(0.0, SURROGATE_MAX_ALTITUDE_KM),
                              mu_sun_range=(max(mu_sun_min, 0.0), 1.0))
     ctx.synchronize()
     data_s = time.perf_counter() - t0
