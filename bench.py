@@ -37,14 +37,14 @@ METRIC = "cloud raymarch Gsample-steps/s + LUT-set build ms; surrogate vs plain 
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic FP32 operation tally of the raymarch (VolumetricCloudRender.comp with
-# VolumetricCloudDefaultMaterial1.glsl), counting + - * / min max as 1, FMA as 2 and
-# each exp / log2 / sqrt / rcp as 1:
-FLOP_PER_DENSITY_EVAL = 46      # sample position, CalHeight01, distance, uv, LOD, cloud-map fetch, base density
-FLOP_PER_DETAIL_FETCH = 28      # detail uvw, LOD, fetch, erosion arithmetic (only where base density > 0)
-FLOP_PER_CLOUD_STEP = 72        # extinction, shadow-loop bookkeeping, phase (2 HG), env term, accumulation
-FLOP_PER_STEP = 8               # ray position and loop
-FLOP_PER_PIXEL = 260            # unprojection, shell intersection, composite with T-LUT and AP-LUT lookups
+# FP32 operations the kernels execute, measured with rocprofv3's SQ counters on this bench's
+# workloads (profiles/cloud_raymarch_1080p.md, profiles/earth_lut_set.md): per wave instruction
+# SQ_INSTS_VALU_{ADD,MUL,TRANS}_F32 count 64 lane operations and SQ_INSTS_VALU_FMA_F32 128, scaled
+# by the measured lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), then divided
+# by the executed sample-steps (raymarch) or integrator steps (LUT kernels) of the same launches.
+RAYMARCH_KERNEL = "cloud_raymarch_kernel<5,false>"
+RAYMARCH_FP32_FLOP_PER_STEP = 150.0
+FLOP_PER_INTEGRATOR_STEP = 150.0
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
@@ -55,7 +55,7 @@ FLOP_PER_PIXEL = 260            # unprojection, shell intersection, composite wi
 FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
-    "cloud_raymarch_kernel<1,false>": (8141.5, 27958.8),
+    "cloud_raymarch_kernel<5,false>": (8141.5, 27958.8),
     "path_trace_kernel<4,true,false>": (6.9093e6, 2.0778e5),
 }
 
@@ -80,21 +80,26 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-epochs", type=int, default=2400)
     ap.add_argument("--surrogate-no-bf16", action="store_true", help="skip the BF16 / SiLU network comparison")
+    ap.add_argument("--sections", default="all",
+                    help="comma-separated sections besides the headline raymarch: " + ", ".join(SECTIONS))
     return ap.parse_args()
 
 
-def bench_raymarch(ctx, args):
+def bench_raymarch(ctx, steps, warmup, workload=None):
+    """One raymarch pass per step over a 1920x1080 frame of `workload` (the cloud benchmark as
+    stated when None), timed with HIP events on the library stream."""
+    from skyrendering_amd import presets
     from skyrendering_amd.scenes import CloudScene
-    scene = CloudScene(ctx, 1920, 1080)
+    scene = CloudScene(ctx, 1920, 1080, workload=workload or presets.cloud_workload_config2())
     counts = scene.count_work()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         scene.run()
     ctx.synchronize()
     ctx.record(0)
-    for _ in range(args.steps):
+    for _ in range(steps):
         scene.run()
     ctx.record(1)
-    ms = ctx.elapsed_ms(0, 1) / args.steps
+    ms = ctx.elapsed_ms(0, 1) / steps
     return scene, counts, ms
 
 
@@ -113,6 +118,48 @@ def bench_lut_set(ctx, rounds):
     ctx.record(3)
     ms = ctx.elapsed_ms(2, 3) / (rounds * len(renders))
     return ms, presets.integrator_steps_per_lut_set(sizes)
+
+
+def lut_set_section(ctx, rounds, cpu_seconds):
+    """The Earth LUT set (transmittance 256x64 / 40, multiple scattering 32x32 x 64 / 30, sky view
+    192x108 / 30, aerial perspective 32^3 / 50.867) over 16 sun angles: ms per set on the GPU, its
+    VALU roofline from the measured FP32 operations per integrator step, and the oracle's FP32
+    path (OpenMP over texels) building whole sets on the host cores."""
+    from skyrendering_amd import presets
+    ms, steps = bench_lut_set(ctx, rounds)
+    flop = steps * FLOP_PER_INTEGRATOR_STEP
+    achieved = flop / (ms * 1e-3) / 1e12
+    out = {"workload": "Earth LUT set: transmittance 256x64 (40 steps), multiple scattering 32x32 x 64 directions "
+                       "(30), sky view 192x108 (30), aerial perspective 32x32x32 (50.867, 203 km); 16 sun "
+                       "zeniths 0-95 deg; FP32 (FP64 optical depth for the near-horizon transmittance columns)",
+           "ms_per_set": round(ms, 4), "integrator_steps": steps,
+           "gsteps_per_s": round(steps / (ms * 1e-3) / 1e9, 2),
+           "roofline": {"bound": "valu", "unit": "TFLOP/s", "flop_per_step": FLOP_PER_INTEGRATOR_STEP,
+                        "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                        "note": "latency-bound: the set is 4 dependent launches (T -> MS -> SV || AP) of "
+                                "16k-33k lanes; the FP32 operations per step are measured by SQ counters"}}
+    if cpu_seconds > 0:
+        import oracle
+        atm, renders, cam, sizes = presets.earth_lut_workload()
+        blk = atm.buffer_data()
+        (tw, th), (mw, mh), (sw, sh), (aw, ah, ad) = (sizes["transmittance"], sizes["multiscattering"],
+                                                      sizes["sky_view"], sizes["aerial_perspective"])
+        done, t_total = 0, 0.0
+        while t_total < cpu_seconds and done < 4 * len(renders):
+            r = renders[done % len(renders)].buffer_data(atm.bottom_radius)
+            t0 = time.perf_counter()
+            T = oracle.transmittance_lut(blk, tw, th, "f32").astype(np.float32)
+            M = oracle.multiscattering_lut(blk, T, mw, mh, "f32").astype(np.float32)
+            oracle.sky_view_lut(blk, r, T, M, sw, sh, precision="f32")
+            oracle.aerial_perspective_lut(blk, r, T, M, aw, ah, ad, precision="f32")
+            t_total += time.perf_counter() - t0
+            done += 1
+        out["cpu_baseline"] = {"value": round(t_total / done * 1e3, 3), "unit": "ms per LUT set",
+                               "cores": oracle.max_threads(), "kind": "port",
+                               "sample": f"{done} whole sets (sun angles of the workload in order), oracle FP32 "
+                                         f"OpenMP over texels, {t_total:.1f} s"}
+    return out
 
 
 def bench_pathtrace(ctx, frames, reference_frames=1):
@@ -138,6 +185,23 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     ref.tracer.render_frames(reference_frames)
     ctx.record(7)
     ref_ms = ctx.elapsed_ms(6, 7) / reference_frames
+    del ref
+    # The same frame with 8^3-brick majorants (the brick edge the workload states; 16 is the default).
+    from skyrendering_amd.pathtrace import VoxelGrid
+    w8 = presets.path_trace_workload()
+    w8["brick"] = 8
+    grid8 = VoxelGrid(ctx, w8["grid_dims"], brick=8).upload(scene.grid.download())
+    s8 = PathTraceScene(ctx, workload=w8, grid=grid8)
+    counts8 = s8.count_work(frames=1)
+    frames8 = max(1, frames // 4)
+    s8.tracer.render_frames(1)
+    ctx.synchronize()
+    ctx.record(6)
+    for _ in range(frames8):
+        s8.tracer.render_frames(1)
+    ctx.record(7)
+    ms8 = ctx.elapsed_ms(6, 7) / frames8
+    del s8, grid8
     n = scene.width * scene.height
     # Algorithmic bytes per frame: 8 FP16 texels per trilinear density lookup, one FP32 majorant
     # per brick segment, and the RGBA32F accumulator read and written once per pixel.
@@ -151,13 +215,16 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     return scene, {
         "workload": f"voxel-cloud path tracing {scene.width}x{scene.height}, {w['spp']} spp (frames of 1 sample), "
                     f"512^3 FP16 fBm-lognormal grid, <= {w['init'].max_bounces} bounces, albedo {w['init'].albedo}, "
-                    f"HG g={w['init'].forward_phase_g}, sun + sky-view environment, 16^3-brick majorants",
+                    f"HG g={w['init'].forward_phase_g}, sun + sky-view environment, 16^3-brick majorants "
+                    f"(8^3 beside it)",
         "ms_per_frame": round(ms, 4), "ms_per_64spp": round(ms * 64, 2),
         "msamples_per_s": round(n / (ms * 1e-3) / 1e6, 2),
         "density_lookups_per_frame": counts["lookups"], "majorant_segments_per_frame": counts["segments"],
         "path_vertices_per_frame": counts["vertices"], "shadow_rays_per_frame": counts["shadow_rays"],
         "glookups_per_s": round(counts["lookups"] / (ms * 1e-3) / 1e9, 3),
         "gvertices_per_s": round(counts["vertices"] / (ms * 1e-3) / 1e9, 4),
+        "brick8": {"ms_per_frame": round(ms8, 4), "frames_timed": frames8,
+                   "density_lookups_per_frame": counts8["lookups"], "majorant_segments_per_frame": counts8["segments"]},
         "single_majorant_ms_per_frame": round(ref_ms, 3),
         "single_majorant_density_lookups_per_frame": ref_counts["lookups"],
         "brick_speedup_vs_single_majorant": round(ref_ms / ms, 2),
@@ -214,8 +281,8 @@ def bench_environment(ctx, reps=20):
 
 
 def bench_cloud_shadow(ctx, scene, reps=20):
-    """VolumetricCloud::RenderShadow for the raymarch workload: the 512^2 shadow map (Material1,
-    6-12 steps per texel, temporal blend), the two blur passes and the shadow froxel
+    """VolumetricCloud::RenderShadow for the raymarch workload: the 512^2 shadow map (the workload's
+    material, 6-12 steps per texel, temporal blend), the two blur passes and the shadow froxel
     (1920/12 x 1080/12 x 128), per frame, timed with HIP events; the oracle's FP32 path beside it."""
     from skyrendering_amd.cloud import CloudShadow, cloud_shadow_matrices
     sun = np.array(scene.renderer.buffer.sun_direction[:], np.float32)
@@ -232,17 +299,21 @@ def bench_cloud_shadow(ctx, scene, reps=20):
     ms = ctx.elapsed_ms(14, 15) / reps
     mw, mh = sh.map_size
     fw, fh, fd = sh.froxel_size
-    out = {"workload": f"cloud shadow map {mw}x{mh} (Material1, temporal), 2 blur passes, shadow froxel "
+    out = {"workload": f"cloud shadow map {mw}x{mh} (Material1 + Worley volume, temporal), 2 blur passes, shadow froxel "
                        f"{fw}x{fh}x{fd}", "ms_per_frame": round(ms, 4),
            "bound": "latency (4 launches; the map's 262k threads march 6-12 material samples each)"}
     import oracle
     import ctypes as C
     from skyrendering_amd import _abi
     cm, det, disp = scene.textures.download()
+    wor = scene.textures.download_worley()
     keep = [np.ascontiguousarray(x) for x in (cm, det, disp)]
     m = _abi.CloudMaterial()
     C.memmove(C.byref(m), C.byref(scene.mat_block), C.sizeof(m))
     m.cloud_map, m.detail, m.displacement = (x.ctypes.data for x in keep)
+    if wor is not None:
+        keep.append(np.ascontiguousarray(wor))
+        m.worley_texture = keep[-1].ctypes.data
     t0 = time.perf_counter()
     raw = oracle.cloud_shadow_map(scene.common, m, scene.blue_noise_host, mw, mh, precision="f32").astype(np.float32)
     blurred = oracle.cloud_shadow_blur(oracle.cloud_shadow_blur(raw, 0), 1)
@@ -338,12 +409,10 @@ def bench_cloud_reconstruct(ctx, scene, reps=50):
     recon = oracle.cloud_reconstruct(c, cb, idx, render, dist, prev, w, h, "f32")
     oracle.cloud_upscale(c, cb, depth, recon, hdr_bits, "f32")
     cpu_s = time.perf_counter() - t0
-    out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms per frame (passes only)", "cores": 1,
-                           "kind": "port", "sample": "one frame of the four passes, oracle FP32, single thread"}
+    out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms per frame (passes only)",
+                           "cores": oracle.max_threads(), "kind": "port",
+                           "sample": "one frame of the four passes, oracle FP32 OpenMP over rows"}
     return out
-
-
-FLOP_PER_INTEGRATOR_STEP = 150  # sky-view / MS step: 2 LUT fetches, ~7 transcendentals, RGB accumulation
 
 
 def bench_lut_batch(ctx, launches=5):
@@ -374,7 +443,7 @@ def bench_lut_batch(ctx, launches=5):
         "ms_per_launch": round(ms, 3), "default_tuning_ms": round(default_ms, 3),
         "integrator_steps": steps, "gsteps_per_s": round(steps / (ms * 1e-3) / 1e9, 2),
         "tuning": tuned, "tuned_pass_sum_ms": round(tuned_pass_ms, 3),
-        "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak", "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "roof": "fp32 vector (VALU) peak", "unit": "TFLOP/s",
                      "flop_per_step": FLOP_PER_INTEGRATOR_STEP,
                      "achieved": round(steps * FLOP_PER_INTEGRATOR_STEP / (ms * 1e-3) / 1e12, 3),
                      "peak": FP32_PEAK_TFLOPS,
@@ -512,21 +581,22 @@ def bench_surrogate(ctx, epochs=2400, width=1920, height=1080, n_views=64, cpu_s
     ref = oracle.sky_radiance(blocks, T, MS, 30.0, q)
     e = sg.relative_error(plain[v, ys, xs], ref, 1e-6)
     out["plain_vs_fp64_max"] = float(e.max())
-    # CPU baseline: whole panorama rows (quarter resolution) through the oracle's plain
+    # CPU baseline: whole panorama rows (quarter resolution) through the oracle's FP32 plain
     # integrator until about cpu_seconds of work.
-    done, cpu_s, row = 0, 0.0, 0
-    while cpu_s < cpu_seconds and row < small.height * n_views:
-        vi, y = row % n_views, (row * 7) % small.height
-        qr = small.pixel_queries(vi, np.arange(small.width), np.full(small.width, y))
-        t0 = time.perf_counter()
-        oracle.sky_radiance(blocks, T, MS, 30.0, qr)
-        cpu_s += time.perf_counter() - t0
-        done += len(qr)
-        row += 1
-    out["cpu_baseline"] = {"value": round(done / cpu_s / 1e6, 5), "unit": "Msamples/s (plain integrator)",
-                           "cores": oracle.max_threads(), "kind": "port",
-                           "sample": f"{row} panorama rows of {small.width} pixels ({done} samples), oracle FP64 "
-                                     f"OpenMP, {cpu_s:.1f} s"}
+    if cpu_seconds > 0:
+        done, cpu_s, row = 0, 0.0, 0
+        while cpu_s < cpu_seconds and row < small.height * n_views:
+            vi, y = row % n_views, (row * 7) % small.height
+            qr = small.pixel_queries(vi, np.arange(small.width), np.full(small.width, y))
+            t0 = time.perf_counter()
+            oracle.sky_radiance(blocks, T, MS, 30.0, qr, precision="f32")
+            cpu_s += time.perf_counter() - t0
+            done += len(qr)
+            row += 1
+        out["cpu_baseline"] = {"value": round(done / cpu_s / 1e6, 5), "unit": "Msamples/s (plain integrator)",
+                               "cores": oracle.max_threads(), "kind": "port",
+                               "sample": f"{row} panorama rows of {small.width} pixels ({done} samples), oracle "
+                                         f"FP32 OpenMP, {cpu_s:.1f} s"}
     return out
 
 
@@ -566,16 +636,19 @@ def cpu_baseline(scene, target_seconds):
     apl, apt = r.aerial_perspective()
     cm, det, disp = scene.textures.download()
     inputs = oracle.HostCloudInputs(scene.mat_block, cloud_map=cm, detail=det, displacement=disp,
-                                    T=a.transmittance(), ap_l=apl, ap_t=apt, blue_noise=scene.blue_noise_host,
-                                    image_size=(scene.width, scene.height))
+                                    worley=scene.textures.download_worley(), T=a.transmittance(), ap_l=apl,
+                                    ap_t=apt, blue_noise=scene.blue_noise_host, image_size=(scene.width, scene.height))
     w, h = scene.width, scene.height
     threads = oracle.max_threads()
 
     def run(rows):
         px = (rows[:, None] * w + np.arange(w)[None, :]).reshape(-1)
+        oracle.bind_cloud_phase(scene.phase)
         t0 = time.perf_counter()
         _, _, steps = oracle.cloud_raymarch(a.buffer, scene.common, scene.cloud_block, inputs, w, px, "f32")
-        return time.perf_counter() - t0, int(steps.sum()), len(px)
+        dt = time.perf_counter() - t0
+        oracle.bind_cloud_phase(None)
+        return dt, int(steps.sum()), len(px)
 
     probe_rows = np.linspace(0, h - 1, 6).astype(np.int64)
     t, s, n = run(probe_rows)
@@ -598,6 +671,10 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+SECTIONS = ("lut_set", "material1", "cloud_shadow", "cloud_reconstruct", "environment", "path_trace", "lut_batch",
+            "surrogate", "cpu_baseline")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -605,59 +682,84 @@ def main():
               f"image rows and the path tracer by frames, but no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)
-    from skyrendering_amd import Context
+    want = set(SECTIONS) if args.sections == "all" else set(x for x in args.sections.split(",") if x)
+    unknown = want - set(SECTIONS)
+    if unknown:
+        print(f"bench.py: unknown sections {sorted(unknown)}; choose from {', '.join(SECTIONS)}", file=sys.stderr)
+        sys.exit(2)
+    for flag, name in ((args.no_pathtrace, "path_trace"), (args.no_batch, "lut_batch"),
+                       (args.no_surrogate, "surrogate"), (args.no_cpu_baseline, "cpu_baseline")):
+        if flag:
+            want.discard(name)
+    from skyrendering_amd import Context, presets
     ctx = Context(0)
     dev = ctx.device_info()
-    scene, counts, ms = bench_raymarch(ctx, args)
+    # The headline: the cloud benchmark as stated (Material1 + 32^3 Worley detail volume, HG 0.8 / -0.3,
+    # seeded 128^2 blue-noise tile).
+    scene, counts, ms = bench_raymarch(ctx, args.steps, args.warmup)
     steps = counts["steps"]
     value = steps / (ms * 1e-3) / 1e9
-    flops = (counts["evals"] * FLOP_PER_DENSITY_EVAL + counts["detail"] * FLOP_PER_DETAIL_FETCH +
-             counts["cloud_steps"] * FLOP_PER_CLOUD_STEP + steps * FLOP_PER_STEP +
-             scene.width * scene.height * FLOP_PER_PIXEL)
+    flops = steps * RAYMARCH_FP32_FLOP_PER_STEP
     achieved = flops / (ms * 1e-3) / 1e12
     progress(f"raymarch {ms:.4f} ms per frame")
-    lut_ms, lut_steps = bench_lut_set(ctx, args.lut_rounds)
-    progress(f"LUT set {lut_ms:.4f} ms")
+    traffic = profiled_traffic(RAYMARCH_KERNEL)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gsample-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded weather map, noise volumes, blue noise)",
-        "config": {"workload": "volumetric cloud raymarch 1920x1080, one ray per pixel, 64 primary + 6 light "
-                               "steps, layer 1.5-4 km, extinction 50/km, Material1 (512^2 RG8 uniform weather "
-                               "map, 128^3 R8 Perlin-Worley detail), camera 1 km, sun elevation 20 deg",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded weather map, noise volumes, blue-noise tile)",
+        "config": {"workload": "volumetric cloud raymarch 1920x1080, one ray per pixel, 64 primary + 6 light steps, "
+                               "layer 1.5-4 km, extinction 50/km, 512^2 RG8 weather map with uniform[0,1] "
+                               "coverage, 128^3 R8 Perlin-Worley base-shape volume, 32^3 R8 Worley detail volume, "
+                               "dual-lobe HG g=0.8 and -0.3, seeded 128^2 blue-noise tile, camera 1 km, sun "
+                               "elevation 20 deg",
                    "width": 1920, "height": 1080, "executed_steps_per_frame": steps,
                    "density_evals_per_frame": counts["evals"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak; the kernel is VALU/L2-latency bound",
+        "roofline": {"bound": "valu", "roof": "FP32 vector peak; the kernel is VALU-issue bound (textures are "
+                                              "L2-resident, HBM traffic is its inputs once and outputs once)",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "traffic": profiled_traffic("cloud_raymarch_kernel<1,false>"),
-                     "traffic_gbs": round(profiled_traffic("cloud_raymarch_kernel<1,false>") / (ms * 1e-3) / 1e9, 1),
-                     "kernel": "cloud_raymarch_kernel<1,false>",
-                     "algorithmic_flop_per_launch": flops},
-        "lut_set_ms": round(lut_ms, 4),
-        "lut_set_integrator_steps": lut_steps,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "traffic_gbs": round(traffic / (ms * 1e-3) / 1e9, 1), "kernel": RAYMARCH_KERNEL,
+                     "fp32_flop_per_step": RAYMARCH_FP32_FLOP_PER_STEP, "fp32_flop_per_launch": int(flops)},
         "device": dev,
     }
-    out["cloud_shadow"] = bench_cloud_shadow(ctx, scene)
-    progress(f"cloud shadow {out['cloud_shadow']['ms_per_frame']} ms per frame")
-    out["cloud_reconstruct"] = bench_cloud_reconstruct(ctx, scene)
-    progress(f"cloud checkerboard/index/reconstruct/upscale {out['cloud_reconstruct']['passes_ms']} ms, "
-             f"quarter-resolution frame {out['cloud_reconstruct']['frame_ms']} ms")
-    out["environment"] = bench_environment(ctx)
-    progress(f"environment cube + IBL {out['environment']['ms_per_update']} ms per update")
+    if "material1" in want:
+        m_scene, m_counts, m_ms = bench_raymarch(ctx, args.steps, args.warmup, workload=presets.cloud_workload())
+        out["material1_reference"] = {
+            "workload": "the same frame with the reference's Material1 (no Worley volume), its lobes "
+                        "(HG -0.15 x 2.16 / 0.85) and a uniform white-noise 128^2 jitter tile",
+            "ms_per_frame": round(m_ms, 4), "executed_steps_per_frame": m_counts["steps"],
+            "gsample_steps_per_s": round(m_counts["steps"] / (m_ms * 1e-3) / 1e9, 3)}
+        progress(f"raymarch (Material1) {m_ms:.4f} ms per frame")
+        del m_scene
+    if "lut_set" in want:
+        out["lut_set"] = lut_set_section(ctx, args.lut_rounds, args.cpu_seconds / 4 if "cpu_baseline" in want else 0)
+        out["lut_set_ms"] = out["lut_set"]["ms_per_set"]
+        progress(f"LUT set {out['lut_set_ms']:.4f} ms")
+    if "cloud_shadow" in want:
+        out["cloud_shadow"] = bench_cloud_shadow(ctx, scene)
+        progress(f"cloud shadow {out['cloud_shadow']['ms_per_frame']} ms per frame")
+    if "cloud_reconstruct" in want:
+        out["cloud_reconstruct"] = bench_cloud_reconstruct(ctx, scene)
+        progress(f"cloud checkerboard/index/reconstruct/upscale {out['cloud_reconstruct']['passes_ms']} ms, "
+                 f"quarter-resolution frame {out['cloud_reconstruct']['frame_ms']} ms")
+    if "environment" in want:
+        out["environment"] = bench_environment(ctx)
+        progress(f"environment cube + IBL {out['environment']['ms_per_update']} ms per update")
     pt_scene = None
-    if not args.no_pathtrace:
+    if "path_trace" in want:
         pt_scene, out["path_trace"] = bench_pathtrace(ctx, args.pt_frames)
         progress(f"path tracer {out['path_trace']['ms_per_frame']} ms per frame")
     batch = None
-    if not args.no_batch:
+    if "lut_batch" in want:
         batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
         progress(f"LUT batch {out['lut_batch']['ms_per_launch']} ms per launch")
-    if not args.no_surrogate:
+    if "surrogate" in want:
         out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs, compare_bf16=not args.surrogate_no_bf16,
-                                           compare_epochs=min(1200, args.surrogate_epochs))
+                                           compare_epochs=min(1200, args.surrogate_epochs),
+                                           cpu_seconds=args.cpu_seconds / 3 if "cpu_baseline" in want else 0)
         progress(f"surrogate {out['surrogate']['surrogate_ms']} ms vs plain {out['surrogate']['plain_ms']} ms")
-    if not args.no_cpu_baseline:
+    if "cpu_baseline" in want:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         progress("raymarch CPU baseline done")
         if pt_scene is not None:

@@ -275,8 +275,29 @@ typedef enum SkyhipCloudMaterialKind {
      * LINEAR, CLAMP_TO_BORDER(0)); the voxel block's uLodBias and uSampleLodK
      * are ignored.  No reference counterpart: it stands in for the OpenVDB-loaded
      * volume (VolumetricCloudVoxelMaterial.cpp:39-75). */
-    SKYHIP_MATERIAL_VOXEL_F16 = 4
+    SKYHIP_MATERIAL_VOXEL_F16 = 4,
+    /* Material1 with a second, high-frequency detail volume: a WorleyFBM R8 volume
+     * (skyhip_noise_worley, e.g. 32^3) that erodes Material1's density once more.  No
+     * reference counterpart (Material1 samples one 128^3 detail volume,
+     * VolumetricCloudDefaultMaterial1.glsl:14-28).  SampleSigmaT:
+     *   d1    = clamp(base - detail, 0, 1)            Material1's density before its
+     *                                                 uDensity * height01 factor
+     *   mask  = max(clamp(height01 - uHeightCut, 0, 1), clamp(uEdgeCur - coverage, 0, 1))
+     *                                                 Material1's erosion mask
+     *   w     = (worley(GetUVWLod(pos, uWorleySampleInfo)) + uWorleyBase) * uWorleyScale * mask
+     *   sigma = clamp(d1 - w, 0, 1) * uDensity * height01
+     * Material1 returns 0 where its base density is 0 before any detail fetch; so does this. */
+    SKYHIP_MATERIAL_DEFAULT1_WORLEY = 5
 } SkyhipCloudMaterialKind;
+
+/* The Worley detail term of SKYHIP_MATERIAL_DEFAULT1_WORLEY, 32 bytes (std140-compatible:
+ * the SampleInfo of VolumetricCloudDefaultMaterialCommon.glsl:5-10, then two floats). */
+typedef struct SkyhipCloudWorleyDetailBufferData {
+    SkyhipSampleInfo uWorleySampleInfo;
+    float uWorleyBase;
+    float uWorleyScale;
+    float padding[2];
+} SkyhipCloudWorleyDetailBufferData;
 
 typedef struct SkyhipCloudMaterial {
     int32_t kind; /* SkyhipCloudMaterialKind */
@@ -304,6 +325,13 @@ typedef struct SkyhipCloudMaterial {
     const uint16_t* voxel_grid;   /* FP16 3D, x fastest, voxel_grid_dim[0..2] (VOXEL_F16 only) */
     int32_t voxel_grid_dim[3];
     int32_t voxel_grid_padding;
+    /* DEFAULT1_WORLEY only: the Worley detail block and its R8 3D volume (worley_size^3,
+     * all levels packed, REPEAT + LINEAR mag + NEAREST_MIPMAP_NEAREST min like the detail
+     * volume).  Appended last, so the fields above keep their offsets. */
+    SkyhipCloudWorleyDetailBufferData worley;
+    const uint8_t* worley_texture;
+    int32_t worley_size;
+    int32_t worley_levels;
 } SkyhipCloudMaterial;
 
 /* NoiseCreateInfo, src/SkyRendering/VolumetricCloudDefaultMaterial.h:57-62 and
@@ -539,6 +567,21 @@ int skyhip_cloud_shadow_froxel(SkyhipContext* ctx, const SkyhipCloudCommonBuffer
 int skyhip_bind_shadow_froxel(SkyhipContext* ctx, const uint16_t* froxel_r16, int32_t width, int32_t height,
                               int32_t depth);
 
+/* The raymarch's dual-lobe phase, RayMarchStep (VolumetricCloudRender.comp:123-127):
+ *   phase = mix(HG(cos, back_g) * back_scale, HG(cos, forward_g), exp(-uSunMultiscatteringSigmaScale * sigma_t))
+ * The reference's lobes are the constants back_g = -0.15, back_scale = 2.16, forward_g = 0.85. */
+typedef struct SkyhipCloudPhase {
+    float back_g;
+    float back_scale;
+    float forward_g;
+    float padding;
+} SkyhipCloudPhase;
+
+/* Sets the lobes of later skyhip_cloud_raymarch calls on this context (the reference's are
+ * compile-time constants; this makes them a parameter).  NULL restores the reference's lobes
+ * (the default).  |g| must be below 1 and back_scale finite. */
+int skyhip_bind_cloud_phase(SkyhipContext* ctx, const SkyhipCloudPhase* phase);
+
 /* ------------------------------------------------------------------------- */
 /* Cloud checkerboard, shading index, temporal reconstruction and upscale     */
 /* ------------------------------------------------------------------------- */
@@ -629,6 +672,11 @@ int skyhip_noise_detail(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* perlin,
  * size^2 RGBA8 texture. */
 int skyhip_noise_displacement(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* perlin, uint8_t* out_rgba8,
                               int32_t size);
+/* The Worley detail volume of SKYHIP_MATERIAL_DEFAULT1_WORLEY: level 0 of a size^3 R8 volume,
+ * texel (x, y, z) = UNORM8(WorleyFBM(((x, y, z) + 0.5) / size)), the 3D WorleyFBM of NoiseGen.comp:36-50
+ * that DETAIL_MAP_GEN (:80-86) mixes into the detail volume, here stored on its own.  No
+ * reference counterpart. */
+int skyhip_noise_worley(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* worley, uint8_t* out_r8, int32_t size);
 /* glGenerateTextureMipmap (DynamicTexture::Generate, VolumetricCloudDefaultMaterial.h:39-49)
  * for a packed UNORM8 chain: fills levels 1..levels-1 from level 0 with a
  * 2x2 (dims = 2) or 2x2x2 (dims = 3) box filter, rounded half up. */
@@ -713,8 +761,10 @@ typedef struct SkyhipPathTracingParameters {
  *                    level 0 of the reference's environment-luminance cube up to resampling
  *                    (AtmosphereRenderer.glsl:248-275);
  *   brick_majorants: NULL to track with the global sigma_t_max as the reference does, or the
- *                    float[ceil(dim/16)^3] output of skyhip_voxel_grid_majorants for a
- *                    VOXEL_F16 material (scaled by the voxel block's uDensity on use). */
+ *                    float[ceil(dim/B)^3] output of skyhip_voxel_grid_majorants_sized for a
+ *                    VOXEL_F16 material (scaled by the voxel block's uDensity on use);
+ *   brick_size:      the brick edge B those majorants were built with: 0 for the default
+ *                    (skyhip_voxel_grid_brick_size, 16), else a power of two in [4, 64]. */
 typedef struct SkyhipPathTracingInputs {
     const float* transmittance;
     int32_t transmittance_width;
@@ -728,7 +778,7 @@ typedef struct SkyhipPathTracingInputs {
     int32_t ap_width;
     int32_t ap_height;
     int32_t ap_depth;
-    int32_t padding;
+    int32_t brick_size;
     const float* brick_majorants;
 } SkyhipPathTracingInputs;
 
@@ -753,12 +803,17 @@ typedef struct SkyhipVoxelGridCreateInfo {
 int skyhip_voxel_grid_generate(SkyhipContext* ctx, const SkyhipVoxelGridCreateInfo* info, uint16_t* out_grid,
                                int32_t dim_x, int32_t dim_y, int32_t dim_z);
 
-/* Edge of a majorant brick in texels (16). */
+/* Default edge of a majorant brick in texels (16: the fastest edge measured on the 512^3
+ * workload grid). */
 int skyhip_voxel_grid_brick_size(void);
-/* Majorant per 16^3 brick of an FP16 grid: the maximum over the brick's texels and a
- * one-texel apron on every side (clamped to the grid), which bounds trilinear
- * CLAMP_TO_BORDER(0) samples anywhere in the brick.  out_majorants: device
- * float[ceil(dim_x/16) * ceil(dim_y/16) * ceil(dim_z/16)], x fastest. */
+/* Majorant per B^3 brick of an FP16 grid (B a power of two in [4, 64]): the maximum over the
+ * brick's texels and a one-texel apron on every side (clamped to the grid), which bounds
+ * trilinear CLAMP_TO_BORDER(0) samples anywhere in the brick.  out_majorants: device
+ * float[ceil(dim_x/B) * ceil(dim_y/B) * ceil(dim_z/B)], x fastest.  The reference tracks with
+ * one global majorant (VolumetricCloud.h:159-167); local majorants are an addition. */
+int skyhip_voxel_grid_majorants_sized(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
+                                      int32_t dim_z, int32_t brick_size, float* out_majorants);
+/* skyhip_voxel_grid_majorants_sized with the default brick edge. */
 int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
                                 int32_t dim_z, float* out_majorants);
 

@@ -143,6 +143,7 @@ _PROTO.update({
     "oracle_cloud_map_texels": (None, [_NI, _NI, i32, vp, C.c_long, vp, i32]),
     "oracle_detail_texels": (None, [_NI, _NI, i32, vp, C.c_long, vp, i32]),
     "oracle_displacement_texels": (None, [_NI, i32, vp, C.c_long, vp, i32]),
+    "oracle_worley_texels": (None, [_NI, i32, vp, C.c_long, vp, i32]),
     "oracle_perlin_noise": (f64, [f64, f64, f64, C.c_uint, C.c_uint]),
     "oracle_worley_noise": (f64, [f64, f64, f64, C.c_uint, C.c_uint]),
     "oracle_wang_hash": (C.c_uint, [C.c_uint]),
@@ -155,6 +156,9 @@ _PROTO.update({
                                          P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32, vp,
                                          C.c_long, vp, vp, vp]),
     "oracle_cloud_raymarch_forced_f64": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
+                                                P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32,
+                                                vp, C.c_long, vp, vp, vp, vp, vp, vp]),
+    "oracle_cloud_raymarch_forced_f32": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
                                                 P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32,
                                                 vp, C.c_long, vp, vp, vp, vp, vp, vp]),
     "oracle_cloud_trace_f64": (i32, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
@@ -191,6 +195,13 @@ def displacement_texels(perlin, size, texels, precision="f64"):
     return out
 
 
+def worley_texels(worley, size, texels, precision="f64"):
+    t = np.ascontiguousarray(texels, dtype=np.int32)
+    out = np.empty(len(t), np.uint8)
+    lib().oracle_worley_texels(C.byref(worley), size, _ptr(t), len(t), _ptr(out), 1 if precision == "f64" else 0)
+    return out
+
+
 def generate_mipmaps(chain: np.ndarray, dims, channels, w, h, d, levels):
     chain = np.ascontiguousarray(chain, dtype=np.uint8)
     lib().oracle_generate_mipmaps(_ptr(chain), dims, channels, w, h, d, levels)
@@ -201,15 +212,16 @@ class HostCloudInputs:
     """Host copies of every raymarch input, kept alive for the oracle call."""
 
     def __init__(self, material, cloud_map=None, detail=None, displacement=None, voxel=None, T=None, ap_l=None,
-                 ap_t=None, blue_noise=None, depth=None, index=None, image_size=None):
-        self.keep = [x for x in (cloud_map, detail, displacement, voxel, T, ap_l, ap_t, blue_noise, depth, index)
-                     if x is not None]
+                 ap_t=None, blue_noise=None, depth=None, index=None, image_size=None, worley=None):
+        self.keep = [x for x in (cloud_map, detail, displacement, voxel, T, ap_l, ap_t, blue_noise, depth, index,
+                                 worley) if x is not None]
         m = _abi.CloudMaterial()
         C.memmove(C.byref(m), C.byref(material), C.sizeof(m))
         m.cloud_map = _ptr(cloud_map)
         m.detail = _ptr(detail)
         m.displacement = _ptr(displacement)
         m.voxel_texture = _ptr(voxel)
+        m.worley_texture = _ptr(worley)
         self.material = m
         im = CloudImagesC()
         im.transmittance, im.tw, im.th = _ptr(T), T.shape[1], T.shape[0]
@@ -233,20 +245,22 @@ def cloud_raymarch(atm, common, cloud, inputs: HostCloudInputs, width, pixels, p
     return rgba, dist, steps
 
 
-def cloud_raymarch_forced(atm, common, cloud, inputs: HostCloudInputs, width, pixels, forced_steps):
+def cloud_raymarch_forced(atm, common, cloud, inputs: HostCloudInputs, width, pixels, forced_steps, precision="f64"):
     """Checker mode: replays another implementation's per-pixel step decisions
     (uint32[n, 2]: executed, scheduled; each segment 0 | segment 1 << 16) and returns
-    (rgba, distance, steps, early-exit margin, step-count margin)."""
+    (rgba, distance, steps, early-exit margin, step-count margin).  precision "f32" runs the
+    oracle's FP32 instantiation under the same decisions."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     fs = np.ascontiguousarray(forced_steps, dtype=np.uint32).reshape(-1, 2)
-    rgba = np.empty((len(px), 4), np.float64)
-    dist = np.empty(len(px), np.float64)
+    dt = _dt(precision)
+    rgba = np.empty((len(px), 4), dt)
+    dist = np.empty(len(px), dt)
     steps = np.empty(len(px), np.int32)
     margin = np.empty(len(px), np.float64)
     step_margin = np.empty(len(px), np.float64)
-    lib().oracle_cloud_raymarch_forced_f64(C.byref(atm), C.byref(common), C.byref(cloud), C.byref(inputs.material),
-                                           C.byref(inputs.images), width, _ptr(px), len(px), _ptr(fs), _ptr(rgba),
-                                           _ptr(dist), _ptr(steps), _ptr(margin), _ptr(step_margin))
+    getattr(lib(), f"oracle_cloud_raymarch_forced_{precision}")(
+        C.byref(atm), C.byref(common), C.byref(cloud), C.byref(inputs.material), C.byref(inputs.images), width,
+        _ptr(px), len(px), _ptr(fs), _ptr(rgba), _ptr(dist), _ptr(steps), _ptr(margin), _ptr(step_margin))
     return rgba, dist, steps, margin, step_margin
 
 
@@ -263,12 +277,12 @@ register("oracle_pt_hg_sample_f64", None, [P(_abi.PathTracingParametersC), C.c_u
 register("oracle_pt_transmittance_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
                                               P(_abi.PathTracingParametersC), C.c_uint32, vp, vp])
 register("oracle_pt_track_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
-                                      P(_abi.PathTracingParametersC), vp, i32, C.c_uint32, vp, vp])
+                                      P(_abi.PathTracingParametersC), vp, i32, i32, C.c_uint32, vp, vp])
 register("oracle_pt_sigma_t_f64", f64, [P(_abi.CloudCommonBufferData), P(_abi.CloudMaterial),
                                         P(_abi.PathTracingParametersC), f64, f64, f64])
 register("oracle_voxel_grid_f16", None, [P(_abi.VoxelGridCreateInfoC), i32, i32, i32, vp])
 register("oracle_voxel_grid_density_f32", None, [P(_abi.VoxelGridCreateInfoC), i32, i32, i32, vp, C.c_long, vp])
-register("oracle_brick_majorants", None, [vp, i32, i32, i32, vp])
+register("oracle_brick_majorants", None, [vp, i32, i32, i32, i32, vp])
 register("oracle_float_to_half", C.c_uint16, [f32])
 register("oracle_half_to_float", f32, [C.c_uint16])
 
@@ -277,10 +291,10 @@ class HostPathTraceInputs:
     """Host copies of the path tracer's inputs, kept alive for the oracle call.
     grid: uint16 FP16 bits [z, y, x] (VOXEL_F16 material), T / sv_l / sv_t: RGBA32F 2D LUTs,
     ap_l / ap_t: RGBA32F [d, h, w, 4]; majorants: float32 [bz, by, bx] brick majorants (tracking
-    over the brick grid instead of the single majorant)."""
+    over the brick grid instead of the single majorant), built with brick edge `brick`."""
 
     def __init__(self, material, T, sv_l, sv_t, ap_l, ap_t, grid=None, cloud_map=None, detail=None,
-                 displacement=None, voxel=None, majorants=None):
+                 displacement=None, voxel=None, majorants=None, brick=None):
         if majorants is not None:
             majorants = np.asarray(majorants, np.float32)
         arrs = [np.ascontiguousarray(x) if x is not None else None
@@ -302,6 +316,7 @@ class HostPathTraceInputs:
         i.ap_luminance, i.ap_transmittance = _ptr(ap_l), _ptr(ap_t)
         i.ap_depth, i.ap_height, i.ap_width = ap_l.shape[0], ap_l.shape[1], ap_l.shape[2]
         i.brick_majorants = _ptr(majorants)
+        i.brick_size = 0 if brick is None else int(brick)
         self.inputs = i
 
 
@@ -333,14 +348,16 @@ def pt_transmittance(common, material, params, seed, o, d):
                                              _ptr(d))
 
 
-def pt_track(common, material, params, seed, o, d, mode="delta", majorants=None):
+def pt_track(common, material, params, seed, o, d, mode="delta", majorants=None, brick=None):
     """One tracking call: mode "delta" returns the collision distance (-1: none), "ratio" the
-    transmittance estimate; majorants (float32 [bz, by, bx]) selects brick-majorant tracking."""
+    transmittance estimate; majorants (float32 [bz, by, bx], brick edge `brick`, default
+    BRICK_TEXELS) selects brick-majorant tracking."""
     o = np.ascontiguousarray(o, dtype=np.float64)
     d = np.ascontiguousarray(d, dtype=np.float64)
     m = None if majorants is None else np.ascontiguousarray(majorants, np.float32)
     return lib().oracle_pt_track_f64(C.byref(common), C.byref(material), C.byref(params), _ptr(m),
-                                     {"delta": 0, "ratio": 1}[mode], seed, _ptr(o), _ptr(d))
+                                     BRICK_TEXELS if brick is None else int(brick), {"delta": 0, "ratio": 1}[mode],
+                                     seed, _ptr(o), _ptr(d))
 
 
 def pt_sigma_t(common, material, params, p):
@@ -362,15 +379,15 @@ def voxel_grid_density(ci, dims, xyz):
     return out
 
 
-BRICK_TEXELS = 16  # kBrickTexels of oracle_pathtrace.h
+BRICK_TEXELS = 16  # kBrickTexels of oracle_pathtrace.h: the default brick edge
 
 
-def brick_majorants(grid):
+def brick_majorants(grid, brick=None):
     g = np.ascontiguousarray(grid, dtype=np.uint16)
     dz, dy, dx = g.shape
-    b = BRICK_TEXELS
+    b = BRICK_TEXELS if brick is None else int(brick)
     out = np.empty(((dz + b - 1) // b, (dy + b - 1) // b, (dx + b - 1) // b), np.float32)
-    lib().oracle_brick_majorants(_ptr(g), dx, dy, dz, _ptr(out))
+    lib().oracle_brick_majorants(_ptr(g), dx, dy, dz, b, _ptr(out))
     return out
 
 
@@ -386,18 +403,20 @@ def pt_lambert_sample(seed, n):
 
 
 register("oracle_sky_radiance_f64", None, [vp, vp, i32, i32, vp, i32, i32, f32, vp, C.c_long, vp])
+register("oracle_sky_radiance_f32", None, [vp, vp, i32, i32, vp, i32, i32, f32, vp, C.c_long, vp])
 
 
-def sky_radiance(atmospheres, T, MS, steps, queries):
-    """FP64 sky radiance of queries [n, 5] (atmosphere, altitude, mu_view, mu_sun, delta_phi);
+def sky_radiance(atmospheres, T, MS, steps, queries, precision="f64"):
+    """Sky radiance of queries [n, 5] (atmosphere, altitude, mu_view, mu_sun, delta_phi);
     atmospheres: a ctypes array of AtmosphereBufferData; T [a, th, tw, 4] / MS [a, mh, mw, 4]
-    float32 LUT stacks.  Returns [n, 3]."""
+    float32 LUT stacks.  Returns [n, 3] (FP64, or FP32 for precision "f32")."""
     T = np.ascontiguousarray(T, dtype=np.float32)
     MS = np.ascontiguousarray(MS, dtype=np.float32)
     q = np.ascontiguousarray(queries, dtype=np.float64)
-    out = np.empty((len(q), 3), np.float64)
-    lib().oracle_sky_radiance_f64(atmospheres, _ptr(T), T.shape[2], T.shape[1], _ptr(MS), MS.shape[2], MS.shape[1],
-                                  float(steps), _ptr(q), len(q), _ptr(out))
+    out = np.empty((len(q), 3), _dt(precision))
+    getattr(lib(), f"oracle_sky_radiance_{precision}")(atmospheres, _ptr(T), T.shape[2], T.shape[1], _ptr(MS),
+                                                       MS.shape[2], MS.shape[1], float(steps), _ptr(q), len(q),
+                                                       _ptr(out))
     return out
 
 
@@ -567,6 +586,18 @@ def cloud_upscale(common, checkerboard, depth, reconstruct_bits, hdr_bits, preci
     getattr(lib(), f"oracle_cloud_upscale_{precision}")(C.byref(common), _ptr(cb), _ptr(d), _ptr(rb), _ptr(hb), w, h,
                                                         _ptr(out))
     return out
+
+
+# The raymarch's dual-lobe phase for oracle raymarch calls.
+register("oracle_bind_cloud_phase", None, [P(_abi.CloudPhase)])
+
+
+def bind_cloud_phase(phase):
+    """phase: (back_g, back_scale, forward_g), or None (the reference's lobes)."""
+    if phase is None:
+        lib().oracle_bind_cloud_phase(None)
+    else:
+        lib().oracle_bind_cloud_phase(C.byref(_abi.CloudPhase(*[float(x) for x in phase], 0.0)))
 
 
 # The ray-scatter-visibility froxel bound for oracle raymarch / path-trace calls.

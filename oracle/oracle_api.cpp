@@ -109,7 +109,8 @@ PathTracer<S> make_tracer(const Atmosphere<S>& A, const RenderBlock<S>& R, const
     P.sw = in->sky_view_width, P.sh = in->sky_view_height;
     P.ap_l = in->ap_luminance, P.ap_t = in->ap_transmittance;
     P.aw = in->ap_width, P.ah = in->ap_height, P.ad = in->ap_depth;
-    if (in->brick_majorants && material->kind == SKYHIP_MATERIAL_VOXEL_F16) P.use_bricks(in->brick_majorants);
+    if (in->brick_majorants && material->kind == SKYHIP_MATERIAL_VOXEL_F16)
+        P.use_bricks(in->brick_majorants, in->brick_size ? in->brick_size : kBrickTexels);
     return P;
 }
 
@@ -184,6 +185,27 @@ double oracle_extinction_f64(const SkyhipAtmosphereBufferData* b, double altitud
     return A.GetExtinction(altitude)[channel];
 }
 double oracle_mie_phase_f64(double g, double c) { return Atmosphere<double>::MiePhaseFunction(g, c); }
+// GetSunVisibility (Atmosphere.glsl:110-117) at (r, mu_s) with transmittance LUT T (RGBA32F tw x th).
+void oracle_sun_visibility_f64(const SkyhipAtmosphereBufferData* atm, const float* T, int tw, int th, double r,
+                               double mu_s, double* out) {
+    Atmosphere<double> A(*atm);
+    V3<double> v = A.GetSunVisibility(T, tw, th, r, mu_s);
+    out[0] = v.x, out[1] = v.y, out[2] = v.z;
+}
+// The samplers the oracle emulates, at normalised coordinates: LINEAR / CLAMP_TO_EDGE of RGBA32F
+// 2D and 3D textures (rgb), and textureLod of an UNORM8 2D mip chain with REPEAT.
+void oracle_tex2d_linear_f64(const float* rgba, int w, int h, double u, double v, double* out) {
+    V3<double> r = tex2d_linear_clamp<double>(rgba, w, h, u, v);
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+void oracle_tex3d_linear_f64(const float* rgba, int w, int h, int d, double u, double v, double q, double* out) {
+    V3<double> r = tex3d_linear_clamp<double>(rgba, w, h, d, u, v, q);
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+void oracle_texlod2d_repeat_f64(const uint8_t* tex, int size, int levels, int channels, double u, double v,
+                                double lambda, double* out) {
+    texlod2d_u8_repeat<double>(tex, size, levels, channels, u, v, lambda, channels, out);
+}
 double oracle_rayleigh_phase_f64(double c) { return Atmosphere<double>::RayleighPhaseFunction(c); }
 void oracle_transmittance_rmu_f64(const SkyhipAtmosphereBufferData* b, double r, double mu, double* out3) {
     Atmosphere<double> A(*b);
@@ -249,6 +271,15 @@ void oracle_detail_texels(const SkyhipNoiseCreateInfo* perlin, const SkyhipNoise
                      : DetailTexel<float>(x, y, z, size, *perlin, *worley);
     }
 }
+void oracle_worley_texels(const SkyhipNoiseCreateInfo* worley, int size, const int32_t* texels, long count,
+                          uint8_t* out, int f64) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < count; ++i) {
+        int t = texels[i];
+        int x = t % size, y = (t / size) % size, z = t / (size * size);
+        out[i] = f64 ? WorleyTexel<double>(x, y, z, size, *worley) : WorleyTexel<float>(x, y, z, size, *worley);
+    }
+}
 void oracle_displacement_texels(const SkyhipNoiseCreateInfo* perlin, int size, const int32_t* texels, long count,
                                 uint8_t* out_rgba, int f64) {
 #pragma omp parallel for schedule(dynamic, 64)
@@ -310,6 +341,17 @@ void oracle_cloud_raymarch_forced_f64(const SkyhipAtmosphereBufferData* atm, con
                                       int32_t* out_steps, double* out_margin, double* out_step_margin) {
     cloud_raymarch<double>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps,
                            forced_steps, out_margin, out_step_margin);
+}
+
+// The FP32 instantiation of the checker mode: an independent FP32 evaluation under the same
+// replayed decisions, which measures how far FP32 rounding alone moves a pixel.
+void oracle_cloud_raymarch_forced_f32(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudCommonBufferData* common,
+                                      const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
+                                      const CloudImages* images, int width, const int32_t* pixels, long count,
+                                      const uint32_t* forced_steps, float* out_rgba, float* out_dist,
+                                      int32_t* out_steps, double* out_margin, double* out_step_margin) {
+    cloud_raymarch<float>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps,
+                          forced_steps, out_margin, out_step_margin);
 }
 
 // Per-step trace of one pixel (8 doubles per step, see CloudRaymarchPixel); returns the step count.
@@ -398,14 +440,14 @@ double oracle_pt_transmittance_f64(const SkyhipCloudCommonBufferData* common, co
 // returns the collision distance from o or -1 when the ray leaves; mode 1: ratio
 // tracking, returns the transmittance estimate.
 double oracle_pt_track_f64(const SkyhipCloudCommonBufferData* common, const SkyhipCloudMaterial* material,
-                           const SkyhipPathTracingParameters* params, const float* majorants, int mode, uint32_t seed,
-                           const double* o, const double* d) {
+                           const SkyhipPathTracingParameters* params, const float* majorants, int brick, int mode,
+                           uint32_t seed, const double* o, const double* d) {
     static const SkyhipAtmosphereBufferData zero_atm = {};
     static const SkyhipAtmosphereRenderBufferData zero_render = {};
     Atmosphere<double> A(zero_atm);
     RenderBlock<double> R(zero_render);
     PathTracer<double> P(A, R, *common, *material, *params);
-    if (majorants) P.use_bricks(majorants);
+    if (majorants) P.use_bricks(majorants, brick);
     V3<double> oo = {o[0], o[1], o[2]}, dd = {d[0], d[1], d[2]};
     if (mode == 1) return P.transmittance_estimation(seed, oo, dd);
     double t0, t1, t;
@@ -428,30 +470,44 @@ double oracle_pt_sigma_t_f64(const SkyhipCloudCommonBufferData* common, const Sk
 // planet centre below the camera and the sun at azimuth 0.  queries: 5 doubles each
 // (atmosphere index, altitude, mu_view, mu_sun, delta_phi); LUTs per atmosphere at a
 // uniform stride; out: 3 doubles per query.
-void oracle_sky_radiance_f64(const SkyhipAtmosphereBufferData* atms, const float* T, int tw, int th, const float* MS,
-                             int mw, int mh, float steps, const double* queries, long count, double* out) {
+}  // extern "C"
+
+template <class S>
+void sky_radiance(const SkyhipAtmosphereBufferData* atms, const float* T, int tw, int th, const float* MS, int mw,
+                  int mh, float steps, const double* queries, long count, S* out) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (long i = 0; i < count; ++i) {
         const double* q = queries + 5 * i;
         const int a = int(q[0]);
-        Atmosphere<double> A(atms[a]);
+        Atmosphere<S> A(atms[a]);
         const float* Ta = T + size_t(a) * tw * th * 4;
         const float* Ma = MS + size_t(a) * mw * mh * 4;
-        double h = q[1], mu_v = q[2], mu_s = q[3], dphi = q[4];
-        V3<double> c = {0.0, 0.0, -A.bottom_radius};
-        V3<double> start = {0.0, 0.0, h};
-        double sv = std::sqrt(std::fmax(1.0 - mu_v * mu_v, 0.0)), ss = std::sqrt(std::fmax(1.0 - mu_s * mu_s, 0.0));
-        V3<double> dir = {sv * std::cos(dphi), sv * std::sin(dphi), mu_v};
-        V3<double> sun = {ss, 0.0, mu_s};
-        double r = A.bottom_radius + h;
-        double dist = A.RayIntersectsGround(r, mu_v) ? A.DistanceToBottomAtmosphereBoundary(r, mu_v)
-                                                     : A.DistanceToTopAtmosphereBoundary(r, mu_v);
-        V3<double> L = {0, 0, 0}, tr;
-        if (dist > 0.0)
+        S h = S(q[1]), mu_v = S(q[2]), mu_s = S(q[3]), dphi = S(q[4]);
+        V3<S> c = {S(0), S(0), -A.bottom_radius};
+        V3<S> start = {S(0), S(0), h};
+        S sv = std::sqrt(std::fmax(S(1) - mu_v * mu_v, S(0))), ss = std::sqrt(std::fmax(S(1) - mu_s * mu_s, S(0)));
+        V3<S> dir = {sv * std::cos(dphi), sv * std::sin(dphi), mu_v};
+        V3<S> sun = {ss, S(0), mu_s};
+        S r = A.bottom_radius + h;
+        S dist = A.RayIntersectsGround(r, mu_v) ? A.DistanceToBottomAtmosphereBoundary(r, mu_v)
+                                                : A.DistanceToTopAtmosphereBoundary(r, mu_v);
+        V3<S> L = {S(0), S(0), S(0)}, tr;
+        if (dist > S(0))
             L = A.ComputeScatteredLuminance(false, Ta, tw, th, Ma, mw, mh, 0.5f, c, start, dir, sun, dist, steps, tr,
                                             nullptr);
         out[3 * i] = L.x, out[3 * i + 1] = L.y, out[3 * i + 2] = L.z;
     }
+}
+extern "C" {
+
+void oracle_sky_radiance_f64(const SkyhipAtmosphereBufferData* atms, const float* T, int tw, int th, const float* MS,
+                             int mw, int mh, float steps, const double* queries, long count, double* out) {
+    sky_radiance<double>(atms, T, tw, th, MS, mw, mh, steps, queries, count, out);
+}
+// The FP32 instantiation (the CPU baseline of the plain integrator).
+void oracle_sky_radiance_f32(const SkyhipAtmosphereBufferData* atms, const float* T, int tw, int th, const float* MS,
+                             int mw, int mh, float steps, const double* queries, long count, float* out) {
+    sky_radiance<float>(atms, T, tw, th, MS, mw, mh, steps, queries, count, out);
 }
 
 // The FP16 density grid of the path-tracing workload, and its brick majorants.
@@ -468,14 +524,13 @@ void oracle_voxel_grid_density_f32(const SkyhipVoxelGridCreateInfo* ci, int dx, 
     const int dim[3] = {dx, dy, dz};
     for (long i = 0; i < count; ++i) out[i] = VoxelGridDensity(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], dim, *ci);
 }
-void oracle_brick_majorants(const uint16_t* grid, int dx, int dy, int dz, float* out) {
+void oracle_brick_majorants(const uint16_t* grid, int dx, int dy, int dz, int B, float* out) {
     const int dim[3] = {dx, dy, dz};
-    const int B = kBrickTexels;
     int bx = (dx + B - 1) / B, by = (dy + B - 1) / B, bz = (dz + B - 1) / B;
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < bz; ++z)
         for (int y = 0; y < by; ++y)
-            for (int x = 0; x < bx; ++x) out[(size_t(z) * by + y) * bx + x] = BrickMajorant(grid, dim, x, y, z);
+            for (int x = 0; x < bx; ++x) out[(size_t(z) * by + y) * bx + x] = BrickMajorant(grid, dim, x, y, z, B);
 }
 // Environment-luminance cube (all levels) and IBL precomputation; cubes are uint16 RGBA.
 void oracle_environment_cube_f64(const SkyhipAtmosphereBufferData* atm, const SkyhipAtmosphereRenderBufferData* render,
@@ -563,6 +618,9 @@ void oracle_cloud_upscale_f32(const SkyhipCloudCommonBufferData* c, const float*
 }
 void oracle_bind_shadow_froxel(const uint16_t* froxel, int w, int h, int d) {
     BoundFroxel() = FroxelBinding{froxel, froxel ? w : 0, froxel ? h : 0, froxel ? d : 0};
+}
+void oracle_bind_cloud_phase(const SkyhipCloudPhase* phase) {
+    BoundPhase() = phase ? *phase : ReferencePhase();
 }
 uint16_t oracle_float_to_half(float x) { return float_to_half_rne(x); }
 float oracle_half_to_float(uint16_t h) { return half_to_float(h); }

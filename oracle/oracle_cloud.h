@@ -15,6 +15,15 @@ struct CloudTextures {
     const SkyhipCloudMaterial* m;
 };
 
+// The raymarch's dual-lobe phase, set by oracle_bind_cloud_phase as the library's
+// skyhip_bind_cloud_phase sets a context's; the reference's constant lobes by default
+// (VolumetricCloudRender.comp:125-126).
+inline SkyhipCloudPhase ReferencePhase() { return SkyhipCloudPhase{-0.15f, 2.16f, 0.85f, 0.0f}; }
+inline SkyhipCloudPhase& BoundPhase() {
+    static SkyhipCloudPhase p = ReferencePhase();
+    return p;
+}
+
 template <class S>
 struct CloudCtx {
     const SkyhipCloudCommonBufferData* c;
@@ -53,10 +62,19 @@ struct CloudCtx {
         return texlod3d_u8(m->detail, dim, m->detail_levels, uvw[0], uvw[1], uvw[2], lod, false);
     }
 
+    // The Worley detail volume of SKYHIP_MATERIAL_DEFAULT1_WORLEY, sampled as the detail volume is.
+    S SampleWorley(V3<S> pos) const {
+        S uvw[3], lod;
+        GetUVWLod(pos, m->worley.uWorleySampleInfo, uvw, lod);
+        int dim[3] = {m->worley_size, m->worley_size, m->worley_size};
+        return texlod3d_u8(m->worley_texture, dim, m->worley_levels, uvw[0], uvw[1], uvw[2], lod, false);
+    }
+
     // Material SampleSigmaT(pos, height01)
     S SampleSigmaT(V3<S> pos, S height01) const {
         switch (m->kind) {
-            case SKYHIP_MATERIAL_DEFAULT1: {  // VolumetricCloudDefaultMaterial1.glsl:14-28
+            case SKYHIP_MATERIAL_DEFAULT1:  // VolumetricCloudDefaultMaterial1.glsl:14-28
+            case SKYHIP_MATERIAL_DEFAULT1_WORLEY: {
                 const SkyhipCloudMaterial1BufferData& p = m->material1;
                 S uvw[3], lod, ct[2];
                 GetUVWLod(pos, m->common.uCloudMapSampleInfo, uvw, lod);
@@ -65,10 +83,17 @@ struct CloudCtx {
                 density *= gclamp((S(1) - height01) * S(p.uBaseHeightHardness), S(0), S(1));
                 if (density == S(0)) return S(0);
                 S detail = SampleDetail(pos);
+                S mask = std::fmax(gclamp(height01 - S(p.uHeightCut), S(0), S(1)),
+                                   gclamp(S(p.uEdgeCur) - ct[0], S(0), S(1)));
                 detail = (detail + S(p.uDetailBase)) * S(p.uDetailScale);
-                detail *= std::fmax(gclamp(height01 - S(p.uHeightCut), S(0), S(1)),
-                                    gclamp(S(p.uEdgeCur) - ct[0], S(0), S(1)));
-                return gclamp(density - detail, S(0), S(1)) * S(m->common.uDensity) * height01;
+                detail *= mask;
+                density = gclamp(density - detail, S(0), S(1));
+                if (m->kind == SKYHIP_MATERIAL_DEFAULT1_WORLEY) {
+                    // The second erosion (SkyhipCloudMaterialKind, include/skyhip.h).
+                    S w = (SampleWorley(pos) + S(m->worley.uWorleyBase)) * S(m->worley.uWorleyScale);
+                    density = gclamp(density - w * mask, S(0), S(1));
+                }
+                return density * S(m->common.uDensity) * height01;
             }
             case SKYHIP_MATERIAL_DEFAULT0: {  // VolumetricCloudDefaultMaterial0.glsl:18-31
                 const SkyhipCloudMaterial0BufferData& p = m->material0;
@@ -146,8 +171,10 @@ void RayMarchStep(const CloudCtx<S>& C, MarchState<S>& ctx) {
     if (sigma_t < S(1e-5)) return;
     S tr = std::exp(-ctx.step_size * sigma_t);
     S transmittance_to_sun = C.SampleShadow(ctx.pos);
-    S phase = gmix(CloudCtx<S>::HenyeyGreenstein(ctx.cos_sun_view, S(-0.15)) * S(2.16),
-                   CloudCtx<S>::HenyeyGreenstein(ctx.cos_sun_view, S(0.85)),
+    // The lobes are the reference's constants unless oracle_bind_cloud_phase set others.
+    const SkyhipCloudPhase& ph = BoundPhase();
+    S phase = gmix(CloudCtx<S>::HenyeyGreenstein(ctx.cos_sun_view, S(ph.back_g)) * S(ph.back_scale),
+                   CloudCtx<S>::HenyeyGreenstein(ctx.cos_sun_view, S(ph.forward_g)),
                    std::exp(-S(C.b->uSunMultiscatteringSigmaScale) * sigma_t));
     S se[2];
     se[0] = transmittance_to_sun * phase;

@@ -60,6 +60,7 @@ V3<S> ProjectiveMul(const float* m, V3<S> v) {  // Common.glsl:7-10, in the shad
 // have equal parity, else min.
 inline void CheckerboardDepth(const float* depth, int w, int h, float* out) {
     const int hw = w / 2, hh = h / 2;
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < hh; ++y)
         for (int x = 0; x < hw; ++x) {
             const float a = depth[size_t(2 * y) * w + 2 * x], b = depth[size_t(2 * y) * w + 2 * x + 1];
@@ -79,6 +80,7 @@ void ShadingIndex(const SkyhipCloudCommonBufferData& c, const float* cb, int cw,
     const uint32_t base = c.uBaseShadingIndex;
     auto pos_to_index = [&](int x, int y) { return (base + uint32_t((x + y) & 1)) & 3u; };
     const int ox[8] = {0, 1, 1, 1, 0, -1, -1, -1}, oy[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+    #pragma omp parallel for schedule(static)
     for (int py = 0; py < qh; ++py)
         for (int px = 0; px < qw; ++px) {
             float tile[4];
@@ -119,6 +121,7 @@ void Reconstruct(const SkyhipCloudCommonBufferData& c, const float* cb, const fl
     const int ox[9] = {0, 0, 1, 1, 1, 0, -1, -1, -1}, oy[9] = {0, 1, 1, 0, -1, -1, -1, 0, 1};
     const V3<S> cam = cvt3<S>(c.uCameraPos);
     auto qidx = [&](int x, int y) { return size_t(Clampi(y, 0, qh - 1)) * qw + Clampi(x, 0, qw - 1); };
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < hh; ++y)
         for (int x = 0; x < hw; ++x) {
             const int bx = x >> 1, by = y >> 1;
@@ -190,6 +193,7 @@ void Upscale(const SkyhipCloudCommonBufferData& c, const float* cb, const float*
     using namespace recon;
     const int hw = w / 2, hh = h / 2;
     const int ox[4] = {-1, 1, 1, -1}, oy[4] = {1, 1, -1, -1};
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
             const S lin = LinearDepth<S>(c.uLinearDepthParam, S(depth[size_t(y) * w + x]));

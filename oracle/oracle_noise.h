@@ -144,6 +144,14 @@ uint8_t DetailTexel(int x, int y, int z, int size, const SkyhipNoiseCreateInfo& 
     return to_unorm8(RemapFrom01(p, w, S(1)));
 }
 
+// The Worley detail volume (no reference counterpart): WorleyFBM (NoiseGen.comp:36-50) alone at
+// the texel centre of a size^3 volume -> R8.
+template <class S>
+uint8_t WorleyTexel(int x, int y, int z, int size, const SkyhipNoiseCreateInfo& worley) {
+    V3<S> coord = {(S(x) + S(0.5)) / S(size), (S(y) + S(0.5)) / S(size), (S(z) + S(0.5)) / S(size)};
+    return to_unorm8(WorleyFBM(coord, worley));
+}
+
 // NoiseGen.comp DISPLACEMENT_GEN (:88-101): texel -> RGBA8, seed + channel.
 template <class S>
 void DisplacementTexel(int x, int y, int size, const SkyhipNoiseCreateInfo& perlin, uint8_t out[4]) {

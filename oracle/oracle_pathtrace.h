@@ -7,7 +7,7 @@
 // (AtmosphereRenderer.glsl:253-273) evaluated directly from the sky-view LUT.
 // Tracking uses the reference's single majorant kSigmaTMax, so each sample
 // consumes the reference's random sequence, or (when brick majorants are
-// given) the library's 16^3-brick majorant grid walked in texel space: the same
+// given) the library's B^3-brick majorant grid walked in texel space: the same
 // tracking estimators with a piecewise-constant majorant, drawing random
 // numbers in the same order as the library's brick tracker.  Only the tests
 // and the benchmark's CPU baseline use this code.
@@ -83,13 +83,12 @@ inline float VoxelGridDensity(int x, int y, int z, const int dim[3], const Skyhi
     return std::fmax(std::exp(ci.lognormal_sigma * ci.fbm_inv_std * fbm) * falloff - ci.threshold, 0.0f);
 }
 
-// Texels per brick edge of the library's majorant grid.
+// Default texels per brick edge of the library's majorant grid.
 constexpr int kBrickTexels = 16;
 
-// Brick majorant: max over the kBrickTexels^3 brick and a one-texel apron, clamped to the grid.
-inline float BrickMajorant(const uint16_t* g, const int dim[3], int bx, int by, int bz) {
+// Brick majorant: max over the B^3 brick and a one-texel apron, clamped to the grid.
+inline float BrickMajorant(const uint16_t* g, const int dim[3], int bx, int by, int bz, int B = kBrickTexels) {
     float m = 0.0f;
-    const int B = kBrickTexels;
     int lo[3] = {bx * B - 1, by * B - 1, bz * B - 1};
     for (int z = std::max(lo[2], 0); z <= std::min(lo[2] + B + 1, dim[2] - 1); ++z)
         for (int y = std::max(lo[1], 0); y <= std::min(lo[1] + B + 1, dim[1] - 1); ++y)
@@ -154,9 +153,10 @@ struct PathTracer {
     int tw = 0, th = 0, sw = 0, sh = 0, aw = 0, ah = 0, ad = 0;
     V3<S> cam, sun, box_min, box_max;
     S R, hb, ht;
-    // Brick-majorant grid (VOXEL_F16 only): one majorant per kBrickTexels^3 brick, in texel space
+    // Brick-majorant grid (VOXEL_F16 only): one majorant per brick^3 brick, in texel space
     // X = pos * tex_scale + tex_bias per axis (z measured from the cloud bottom).
     const float* majorants = nullptr;
+    int brick = kBrickTexels;
     int nb[3] = {0, 0, 0};
     S tex_scale[3] = {0, 0, 0}, tex_bias[3] = {0, 0, 0}, grid_n[3] = {0, 0, 0};
     S majorant_scale = 0;
@@ -176,11 +176,12 @@ struct PathTracer {
         box_max = {S(pp.region_box_half_width), S(pp.region_box_half_width), ht};
     }
 
-    void use_bricks(const float* m) {
+    void use_bricks(const float* m, int brick_size = kBrickTexels) {
         majorants = m;
+        brick = brick_size;
         const SkyhipCloudVoxelBufferData& vb = mat.voxel;
         for (int k = 0; k < 3; ++k) {
-            nb[k] = (mat.voxel_grid_dim[k] + kBrickTexels - 1) / kBrickTexels;
+            nb[k] = (mat.voxel_grid_dim[k] + brick - 1) / brick;
             grid_n[k] = S(mat.voxel_grid_dim[k]);
         }
         for (int k = 0; k < 2; ++k) {
@@ -200,8 +201,8 @@ struct PathTracer {
         int b[3];
     };
     S next_boundary(const Walk& w, int i) const {
-        if (w.k[i] > S(0)) return w.b[i] < nb[i] - 1 ? (S(kBrickTexels * (w.b[i] + 1)) - w.org[i]) * w.inv_k[i] : S(3.0e38);
-        if (w.k[i] < S(0)) return w.b[i] > 0 ? (S(kBrickTexels * w.b[i]) - w.org[i]) * w.inv_k[i] : S(3.0e38);
+        if (w.k[i] > S(0)) return w.b[i] < nb[i] - 1 ? (S(brick * (w.b[i] + 1)) - w.org[i]) * w.inv_k[i] : S(3.0e38);
+        if (w.k[i] < S(0)) return w.b[i] > 0 ? (S(brick * w.b[i]) - w.org[i]) * w.inv_k[i] : S(3.0e38);
         return S(3.0e38);
     }
     bool walk_begin(Walk& w, V3<S> o, V3<S> d, S& t, S t_stop) const {
@@ -227,7 +228,7 @@ struct PathTracer {
         w.t_end = hi;
         for (int i = 0; i < 3; ++i) {
             S x = t * w.k[i] + w.org[i];
-            w.b[i] = std::min(std::max(int(std::floor(x / S(kBrickTexels))), 0), nb[i] - 1);
+            w.b[i] = std::min(std::max(int(std::floor(x / S(brick))), 0), nb[i] - 1);
             w.tn[i] = next_boundary(w, i);
         }
         return true;
@@ -471,10 +472,13 @@ struct PathTracer {
                 if (!has_scattered) scattered_t = distance(cam, o);  // the segment start, as :232-233
                 has_scattered = true;
                 o = o + d * t;
+                // A real collision scatters with probability albedo (the reference's albedo is 1):
+                // both the sun estimate at this vertex and the continuation carry it.
+                throughput *= S(pp.albedo);
                 L += throughput * light(seed, o, v3(phase(dot(d, sun))));
                 S vop;
                 d = hg_sample(seed, d, vop);
-                throughput *= vop * S(pp.albedo);
+                throughput *= vop;
             }
         }
         return L;

@@ -167,6 +167,16 @@ class CloudVoxelBufferData(C.Structure):
 
 
 MATERIAL_DEFAULT0, MATERIAL_DEFAULT1, MATERIAL_MINIMAL, MATERIAL_VOXEL, MATERIAL_VOXEL_F16 = 0, 1, 2, 3, 4
+MATERIAL_DEFAULT1_WORLEY = 5
+
+
+class CloudWorleyDetailBufferData(C.Structure):
+    _fields_ = [("uWorleySampleInfo", SampleInfo), ("uWorleyBase", f32), ("uWorleyScale", f32),
+                ("padding", _arr(f32, 2))]
+
+
+class CloudPhase(C.Structure):
+    _fields_ = [("back_g", f32), ("back_scale", f32), ("forward_g", f32), ("padding", f32)]
 
 
 class CloudMaterial(C.Structure):
@@ -179,6 +189,8 @@ class CloudMaterial(C.Structure):
         ("displacement", vp), ("displacement_size", i32), ("displacement_levels", i32),
         ("voxel_texture", vp), ("voxel_dim", _arr(i32, 3)), ("voxel_levels", i32),
         ("voxel_grid", vp), ("voxel_grid_dim", _arr(i32, 3)), ("voxel_grid_padding", i32),
+        ("worley", CloudWorleyDetailBufferData), ("worley_texture", vp), ("worley_size", i32),
+        ("worley_levels", i32),
     ]
 
 
@@ -203,7 +215,7 @@ class PathTracingInputsC(C.Structure):
         ("transmittance", vp), ("transmittance_width", i32), ("transmittance_height", i32),
         ("sky_view_luminance", vp), ("sky_view_transmittance", vp), ("sky_view_width", i32),
         ("sky_view_height", i32), ("ap_luminance", vp), ("ap_transmittance", vp), ("ap_width", i32),
-        ("ap_height", i32), ("ap_depth", i32), ("padding", i32), ("brick_majorants", vp),
+        ("ap_height", i32), ("ap_depth", i32), ("brick_size", i32), ("brick_majorants", vp),
     ]
 
 

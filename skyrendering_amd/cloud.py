@@ -24,6 +24,7 @@ _abi.register("skyhip_noise_detail", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseC
                                                 C.POINTER(_abi.NoiseCreateInfo), C.c_void_p, C.c_int32])
 _abi.register("skyhip_noise_displacement", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseCreateInfo), C.c_void_p,
                                                       C.c_int32])
+_abi.register("skyhip_noise_worley", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseCreateInfo), C.c_void_p, C.c_int32])
 _abi.register("skyhip_generate_mipmaps", C.c_int, [C.c_void_p, C.c_void_p] + [C.c_int32] * 6)
 _abi.register("skyhip_cloud_raymarch", C.c_int, [
     C.c_void_p, C.POINTER(_abi.AtmosphereBufferData), C.POINTER(_abi.CloudCommonBufferData),
@@ -158,6 +159,38 @@ class VolumetricCloudDefaultMaterial1(DefaultMaterialCommon):
 
 
 @dataclass
+class VolumetricCloudDefaultMaterial1Worley(VolumetricCloudDefaultMaterial1):
+    """Material1 with a second, high-frequency Worley detail volume (SKYHIP_MATERIAL_DEFAULT1_WORLEY;
+    no reference counterpart).  The volume is WorleyFBM alone (worley_size^3 R8, mips), tiled
+    every worley_repeat_size km; it erodes Material1's density by
+    (worley + worley_base) * worley_scale * Material1's erosion mask."""
+    worley: NoiseCreateInfo = field(default_factory=lambda: NoiseCreateInfo(4, 2, 0.0, 1.0))
+    worley_size: int = 32
+    worley_repeat_size: float = 1.5
+    worley_base: float = 0.0
+    worley_scale: float = 0.35
+    kind = _abi.MATERIAL_DEFAULT1_WORLEY
+
+    @classmethod
+    def from_material1(cls, m1: VolumetricCloudDefaultMaterial1, **kw):
+        m = cls(**{f.name: getattr(m1, f.name) for f in fields(m1)})
+        for k, v in kw.items():
+            setattr(m, k, v)
+        return m
+
+    def worley_block(self, viewport, camera: Camera, offset=(0.0, 0.0)):
+        """The Worley block; its SampleInfo is formed as the default material's (CalKLod,
+        VolumetricCloudDefaultMaterial.cpp:24-28, 78-102) for the Worley volume's size and tiling."""
+        b = _abi.CloudWorleyDetailBufferData()
+        vw, vh = viewport
+        _abi.check(_abi.lib().skyhip_cloud_sample_info(self.worley_repeat_size, self.worley_size, camera.fovy, vw, vh,
+                                                       offset[0], offset[1], C.byref(b.uWorleySampleInfo)))
+        b.uWorleyBase = self.worley_base
+        b.uWorleyScale = self.worley_scale
+        return b
+
+
+@dataclass
 class VolumetricCloudDefaultMaterial0(DefaultMaterialCommon):
     """VolumetricCloudDefaultMaterial0, VolumetricCloudDefaultMaterial.h:142-169, .cpp:180-222."""
     detail_param: tuple = (0.4, 0.0)
@@ -216,9 +249,14 @@ class CloudTextures:
     """Device mip chains of the default material's textures (cloud map RG8 2D, detail R8 3D,
     displacement RGBA8 2D), generated by the noise kernels or uploaded from the host."""
 
-    def __init__(self, ctx: Context, cloud_map_size=512, detail_size=128, displacement_size=128):
+    def __init__(self, ctx: Context, cloud_map_size=512, detail_size=128, displacement_size=128, worley_size=None):
         self.ctx = ctx
         self.cloud_map_size, self.detail_size, self.displacement_size = cloud_map_size, detail_size, displacement_size
+        self.worley_size = worley_size
+        self.worley = None
+        if worley_size:
+            self.worley_levels = mip_levels(worley_size, worley_size, worley_size)
+            self.worley = ctx.alloc(mip_chain_bytes(worley_size, worley_size, worley_size, 1, self.worley_levels))
         self.cloud_map_levels = mip_levels(cloud_map_size, cloud_map_size)
         self.detail_levels = mip_levels(detail_size, detail_size, detail_size)
         self.displacement_levels = mip_levels(displacement_size, displacement_size)
@@ -237,6 +275,8 @@ class CloudTextures:
                                          self.detail.ptr, self.detail_size))
         _abi.check(L.skyhip_noise_displacement(h, C.byref(m.displacement_perlin.to_c()), self.displacement.ptr,
                                                self.displacement_size))
+        if self.worley is not None:
+            _abi.check(L.skyhip_noise_worley(h, C.byref(m.worley.to_c()), self.worley.ptr, self.worley_size))
         self.build_mips()
         return self
 
@@ -259,11 +299,17 @@ class CloudTextures:
         _abi.check(L.skyhip_generate_mipmaps(h, self.detail.ptr, 3, 1, n, n, n, self.detail_levels))
         n = self.displacement_size
         _abi.check(L.skyhip_generate_mipmaps(h, self.displacement.ptr, 2, 4, n, n, 1, self.displacement_levels))
+        if self.worley is not None:
+            n = self.worley_size
+            _abi.check(L.skyhip_generate_mipmaps(h, self.worley.ptr, 3, 1, n, n, n, self.worley_levels))
 
     def download(self):
         return (self.cloud_map.download(np.uint8, (self.cloud_map.nbytes,)),
                 self.detail.download(np.uint8, (self.detail.nbytes,)),
                 self.displacement.download(np.uint8, (self.displacement.nbytes,)))
+
+    def download_worley(self):
+        return None if self.worley is None else self.worley.download(np.uint8, (self.worley.nbytes,))
 
     def fill_material(self, mat: _abi.CloudMaterial):
         mat.cloud_map, mat.cloud_map_size, mat.cloud_map_levels = self.cloud_map.ptr, self.cloud_map_size, \
@@ -271,6 +317,9 @@ class CloudTextures:
         mat.detail, mat.detail_size, mat.detail_levels = self.detail.ptr, self.detail_size, self.detail_levels
         mat.displacement, mat.displacement_size, mat.displacement_levels = self.displacement.ptr, \
             self.displacement_size, self.displacement_levels
+        if self.worley is not None:
+            mat.worley_texture, mat.worley_size, mat.worley_levels = self.worley.ptr, self.worley_size, \
+                self.worley_levels
 
 
 @dataclass
@@ -344,6 +393,22 @@ def bind_shadow_froxel(ctx: Context, froxel: DeviceBuffer | None, size=(0, 0, 0)
     aerial-perspective term of later raymarch and path-trace calls; None unbinds."""
     w, h, d = size
     _abi.check(_abi.lib().skyhip_bind_shadow_froxel(ctx.handle, froxel.ptr if froxel is not None else None, w, h, d))
+
+
+_abi.register("skyhip_bind_cloud_phase", C.c_int, [C.c_void_p, C.POINTER(_abi.CloudPhase)])
+
+REFERENCE_PHASE = (-0.15, 2.16, 0.85)  # VolumetricCloudRender.comp:125-126: (back g, back scale, forward g)
+
+
+def bind_cloud_phase(ctx: Context, phase=None):
+    """skyhip_bind_cloud_phase: the lobes (back_g, back_scale, forward_g) of later raymarch calls
+    on ctx, phase = mix(HG(c, back_g) * back_scale, HG(c, forward_g), exp(-s sigma)); None
+    restores the reference's lobes."""
+    if phase is None:
+        _abi.check(_abi.lib().skyhip_bind_cloud_phase(ctx.handle, None))
+    else:
+        blk = _abi.CloudPhase(*[float(x) for x in phase], 0.0)
+        _abi.check(_abi.lib().skyhip_bind_cloud_phase(ctx.handle, C.byref(blk)))
 
 
 SHADOW_MAP_RESOLUTION = (512, 512)  # kShadowMapResolution, VolumetricCloud.cpp:52
@@ -423,6 +488,10 @@ def make_material(material, textures: CloudTextures | None, viewport, camera: Ca
         m.material0 = material.material0_block()
     else:
         m.material1 = material.material1_block()
+    if material.kind == _abi.MATERIAL_DEFAULT1_WORLEY:
+        if textures.worley is None:
+            raise ValueError("the Worley material needs CloudTextures with a Worley volume (worley_size)")
+        m.worley = material.worley_block(viewport, camera, offset)
     textures.fill_material(m)
     return m
 

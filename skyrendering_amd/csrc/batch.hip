@@ -274,12 +274,29 @@ int capture(SkyhipLutBatch* b) {
     return SKYHIP_OK;
 }
 
+// Rebuilds the tile lists and the graph for tuning t.  The graph is released before the tile
+// lists it points to are freed, so a failure part-way leaves no graph that reads freed memory:
+// the previous tuning is then restored, or, if that fails too, the batch is left without a
+// graph (skyhip_lut_batch_launch reports it) rather than with a dangling one.
 int apply_tuning(SkyhipLutBatch* b, const SkyhipLutBatchTuning& t) {
     SKYHIP_CHECK(check_tuning(t));
     SKYHIP_HIP(hipStreamSynchronize(b->ctx->stream));
+    const SkyhipLutBatchTuning old = b->tuning;
+    release_graph(b);
     b->tuning = t;
-    SKYHIP_CHECK(build_tiles(b));
-    return capture(b);
+    int rc = build_tiles(b);
+    if (rc == SKYHIP_OK) rc = capture(b);
+    if (rc == SKYHIP_OK) return SKYHIP_OK;
+    release_graph(b);
+    free_tiles(b);
+    if (std::memcmp(&old, &t, sizeof(t)) != 0 && check_tuning(old) == SKYHIP_OK) {
+        b->tuning = old;
+        if (build_tiles(b) != SKYHIP_OK || capture(b) != SKYHIP_OK) {
+            release_graph(b);
+            free_tiles(b);
+        }
+    }
+    return rc;
 }
 
 // Median time (ms) of `reps` runs of one pass (with its counter reset) on the context stream.
@@ -436,7 +453,15 @@ extern "C" int skyhip_lut_batch_autotune(SkyhipLutBatch* b, int32_t repetitions,
     // Valid T and MS pools first: the MS and SV candidates read them.
     SKYHIP_CHECK(skyhip_lut_batch_launch(b));
     SKYHIP_HIP(hipStreamSynchronize(b->ctx->stream));
+    const SkyhipLutBatchTuning initial = b->tuning;
     SkyhipLutBatchTuning best = b->tuning;
+    // The candidates rebuild the tile lists the graph reads: drop the graph first, and on any
+    // failure re-apply the tuning the batch had (apply_tuning leaves it launchable or graph-less).
+    release_graph(b);
+    auto restore = [&](int rc) {
+        (void)apply_tuning(b, initial);
+        return rc;
+    };
     struct Knob {
         int SkyhipLutBatchTuning::*field;
         std::vector<int> values;
@@ -463,18 +488,21 @@ extern "C" int skyhip_lut_batch_autotune(SkyhipLutBatch* b, int32_t repetitions,
                 cand.*(k.field) = v;
                 if (check_tuning(cand) != SKYHIP_OK) continue;
                 b->tuning = cand;
-                SKYHIP_CHECK(build_tiles(b));
+                int rc = build_tiles(b);
                 float ms = 0.0f;
-                if (k.pass == 0) SKYHIP_CHECK(time_pass(b, enqueue_t, repetitions, &ms));
-                if (k.pass == 1) SKYHIP_CHECK(time_pass(b, enqueue_ms, repetitions, &ms));
-                if (k.pass == 2) SKYHIP_CHECK(time_pass(b, enqueue_sv, repetitions, &ms));
+                if (rc == SKYHIP_OK && k.pass == 0) rc = time_pass(b, enqueue_t, repetitions, &ms);
+                if (rc == SKYHIP_OK && k.pass == 1) rc = time_pass(b, enqueue_ms, repetitions, &ms);
+                if (rc == SKYHIP_OK && k.pass == 2) rc = time_pass(b, enqueue_sv, repetitions, &ms);
+                if (rc != SKYHIP_OK) return restore(rc);
                 if (ms < best_ms) best_ms = ms, best_v = v;
             }
             best.*(k.field) = best_v;
             pass_ms[k.pass] = best_ms;
         }
     }
-    SKYHIP_CHECK(apply_tuning(b, best));
+    b->tuning = initial;  // apply_tuning's fallback if `best` cannot be applied
+    const int rc = apply_tuning(b, best);
+    if (rc != SKYHIP_OK) return rc;
     if (out_best) *out_best = best;
     if (out_ms) *out_ms = pass_ms[0] + pass_ms[1] + pass_ms[2];
     return SKYHIP_OK;

@@ -39,6 +39,7 @@ struct CloudArgs {
     // The context's bound ray-scatter-visibility froxel, or null (visibility 1).
     const uint16_t* froxel;
     int fw, fh, fd;
+    SkyhipCloudPhase phase;  // the context's dual-lobe phase
 };
 
 struct MarchAcc {
@@ -49,13 +50,14 @@ struct MarchAcc {
 constexpr int kMaxLightSteps = 16;
 
 // The part of RayMarchStep (VolumetricCloudRender.comp:116-137) after sigma_t and the
-// transmittance to the sun are known.
+// transmittance to the sun are known.  hg_back = HG(cos, back_g) * back_scale and
+// hg_forward = HG(cos, forward_g) depend on the ray only (cos = dot(sun, view)), so they are
+// evaluated once per ray.
 __device__ inline void march_accumulate(const CloudArgs& a, float sigma_t, float t_sun, float t, float h01,
-                                        float step_size, float cos_sun_view, MarchAcc& acc) {
+                                        float step_size, float hg_back, float hg_forward, MarchAcc& acc) {
     const float tr = __expf(-step_size * sigma_t);
     const float w = __expf(-a.cloud.uSunMultiscatteringSigmaScale * sigma_t);
-    const float phase = henyey_greenstein(cos_sun_view, -0.15f) * 2.16f * (1.0f - w) +
-                        henyey_greenstein(cos_sun_view, 0.85f) * w;
+    const float phase = hg_back * (1.0f - w) + hg_forward * w;  // mix(back, forward, w)
     float se_r = t_sun * phase;
     float se_g = a.cloud.uEnvBottomVisibility * (1.0f - h01) + h01;
     se_g = se_g - se_g * __expf(-a.cloud.uEnvMultiscatteringSigmaScale * sigma_t);
@@ -79,7 +81,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Per-lane state of one ray's march (two shell segments).
 struct RayState {
     F3 view;
-    float frag_dist, u, v, jitter, cos_sun_view;
+    float frag_dist, u, v, jitter, hg_back, hg_forward;
     float t1_0, t1_1, t2_1;
     float t_start, step_size;
     uint32_t k, n;
@@ -188,7 +190,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
         ray.t1_1 = t1[1];
         ray.t2_1 = t2[1];
         ray.jitter = jitter;
-        ray.cos_sun_view = cos_sun_view;
+        ray.hg_back = henyey_greenstein(cos_sun_view, a.phase.back_g) * a.phase.back_scale;
+        ray.hg_forward = henyey_greenstein(cos_sun_view, a.phase.forward_g);
         ray.view = view;
         ray.frag_dist = frag_dist;
         ray.t1_0 = t1[0];
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
         }
         if (lit) {
             if (kCount) wc.cloud_steps++;
-            march_accumulate(a, sigma, t_sun, t, h01, ray.step_size, ray.cos_sun_view, ray.acc);
+            march_accumulate(a, sigma, t_sun, t, h01, ray.step_size, ray.hg_back, ray.hg_forward, ray.acc);
         }
         if (!ray.done) {
             ++executed;
@@ -376,10 +379,22 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
     auto finite = [](const void* p, size_t bytes) { return finite_block(reinterpret_cast<const float*>(p), bytes / 4); };
     switch (m->kind) {
         case SKYHIP_MATERIAL_DEFAULT0:
-        case SKYHIP_MATERIAL_DEFAULT1: {
+        case SKYHIP_MATERIAL_DEFAULT1:
+        case SKYHIP_MATERIAL_DEFAULT1_WORLEY: {
             if (!finite(&m->common, sizeof(m->common)) || !finite(&m->material1, sizeof(m->material1)) ||
                 !finite(&m->material0, sizeof(m->material0)))
                 return fail(SKYHIP_ERR_NON_FINITE, "material block has a non-finite field");
+            if (m->kind == SKYHIP_MATERIAL_DEFAULT1_WORLEY) {
+                if (!finite(&m->worley, sizeof(m->worley)))
+                    return fail(SKYHIP_ERR_NON_FINITE, "Worley detail block has a non-finite field");
+                if (!m->worley_texture) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "Worley detail volume is NULL");
+                if (!pow2(m->worley_size) || m->worley_size > 1024 || m->worley_levels < 1 ||
+                    m->worley_levels > skyhip_mip_levels(m->worley_size, m->worley_size, m->worley_size))
+                    return fail(SKYHIP_ERR_INVALID_SIZE, "Worley detail volume: power-of-two size <= 1024, valid levels");
+                d.worley = m->worley;
+                int wdim[3] = {m->worley_size, m->worley_size, m->worley_size};
+                make_tex3d(d.worley_tex, m->worley_texture, wdim, m->worley_levels);
+            }
             if (!m->cloud_map || !m->detail) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "material texture is NULL");
             if (!pow2(m->cloud_map_size) || !pow2(m->detail_size) || m->cloud_map_size > 16384 || m->detail_size > 1024)
                 return fail(SKYHIP_ERR_INVALID_SIZE, "cloud map / detail sizes must be powers of two");
@@ -436,6 +451,10 @@ static int launch_raymarch(hipStream_t st, const CloudArgs& a) {
             break;
         case SKYHIP_MATERIAL_DEFAULT1:
             hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_DEFAULT1, kCount>), grid, dim3(256), lds, st, a);
+            break;
+        case SKYHIP_MATERIAL_DEFAULT1_WORLEY:
+            hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY, kCount>), grid, dim3(256), lds,
+                               st, a);
             break;
         case SKYHIP_MATERIAL_MINIMAL:
             hipLaunchKernelGGL((cloud_raymarch_kernel<SKYHIP_MATERIAL_MINIMAL, kCount>), grid, dim3(256), lds, st, a);
@@ -516,6 +535,7 @@ extern "C" int skyhip_cloud_raymarch(SkyhipContext* ctx, const SkyhipAtmosphereB
     a.pixel_steps = work_counters ? pixel_steps : nullptr;
     a.froxel = ctx->froxel;
     a.fw = ctx->froxel_w, a.fh = ctx->froxel_h, a.fd = ctx->froxel_d;
+    a.phase = ctx->phase;
     if (a.froxel && !(common->uInvShadowFroxelMaxDistance > 0.0f && std::isfinite(common->uInvShadowFroxelMaxDistance)))
         return fail(SKYHIP_ERR_NON_FINITE, "a shadow froxel is bound but uInvShadowFroxelMaxDistance is not > 0");
     // SampleShadow's FP32 counter `for (t = 1/n; t <= 1.0; t += 1/n)`, evaluated once here with
@@ -580,6 +600,10 @@ extern "C" int skyhip_cloud_sample_sigma_t(SkyhipContext* ctx, const SkyhipCloud
         case SKYHIP_MATERIAL_DEFAULT1:
             hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_DEFAULT1>, grid, dim3(256), 0, ctx->stream, m,
                                *common, positions_xyz, count, out_sigma_t);
+            break;
+        case SKYHIP_MATERIAL_DEFAULT1_WORLEY:
+            hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY>, grid, dim3(256), 0, ctx->stream,
+                               m, *common, positions_xyz, count, out_sigma_t);
             break;
         case SKYHIP_MATERIAL_MINIMAL:
             hipLaunchKernelGGL(sample_sigma_t_kernel<SKYHIP_MATERIAL_MINIMAL>, grid, dim3(256), 0, ctx->stream, m,

@@ -149,8 +149,9 @@ struct MaterialDev {
     SkyhipCloudMaterial0BufferData m0;
     float minimal_density;
     SkyhipCloudVoxelBufferData voxel;
+    SkyhipCloudWorleyDetailBufferData worley;
     Tex2D cloud_map, displacement;
-    Tex3D detail, voxel_tex;
+    Tex3D detail, voxel_tex, worley_tex;
     GridF16 grid;
 };
 
@@ -184,7 +185,8 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
     float ct[2];
     tex2d_repeat<2>(M.cloud_map, pos.x * ci.frequency + ci.bias[0], pos.y * ci.frequency + ci.bias[1],
                     __log2f(ci.k_lod * dist) + M.common.uLodBias, ct);
-    if (kMat == SKYHIP_MATERIAL_DEFAULT1) {  // VolumetricCloudDefaultMaterial1.glsl:14-28
+    if (kMat == SKYHIP_MATERIAL_DEFAULT1 || kMat == SKYHIP_MATERIAL_DEFAULT1_WORLEY) {
+        // VolumetricCloudDefaultMaterial1.glsl:14-28
         const SkyhipCloudMaterial1BufferData& p = M.m1;
         float density = clamp01((ct[0] - p.uBaseDensityThreshold) * p.uBaseEdgeHardness);
         density *= clamp01((1.0f - height01) * p.uBaseHeightHardness);
@@ -192,9 +194,25 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
         if (kCount) wc->detail++;
         float detail = tex3d_r8<false>(M.detail, pos.x * di.frequency + di.bias[0], pos.y * di.frequency + di.bias[1],
                                        pos.z * di.frequency, __log2f(di.k_lod * dist) + M.common.uLodBias);
+        const float mask = fmaxf(clamp01(height01 - p.uHeightCut), clamp01(p.uEdgeCur - ct[0]));
         detail = (detail + p.uDetailBase) * p.uDetailScale;
-        detail *= fmaxf(clamp01(height01 - p.uHeightCut), clamp01(p.uEdgeCur - ct[0]));
-        return clamp01(density - detail) * M.common.uDensity * height01;
+        detail *= mask;
+        density = clamp01(density - detail);
+        if (kMat == SKYHIP_MATERIAL_DEFAULT1_WORLEY) {
+            // The second erosion by the Worley volume (SkyhipCloudMaterialKind).  With a
+            // non-negative base and scale the term is >= 0, so a zero density stays zero
+            // without the fetch; otherwise the fetch is always made.
+            const SkyhipCloudWorleyDetailBufferData& q = M.worley;
+            const SkyhipSampleInfo& wi = q.uWorleySampleInfo;
+            if (density > 0.0f || !(q.uWorleyBase >= 0.0f && q.uWorleyScale >= 0.0f)) {
+                float w = tex3d_r8<false>(M.worley_tex, pos.x * wi.frequency + wi.bias[0],
+                                          pos.y * wi.frequency + wi.bias[1], pos.z * wi.frequency,
+                                          __log2f(wi.k_lod * dist) + M.common.uLodBias);
+                w = (w + q.uWorleyBase) * q.uWorleyScale;
+                density = clamp01(density - w * mask);
+            }
+        }
+        return density * M.common.uDensity * height01;
     }
     // SKYHIP_MATERIAL_DEFAULT0, VolumetricCloudDefaultMaterial0.glsl:18-31
     const SkyhipSampleInfo& si = M.common.uDisplacementSampleInfo;

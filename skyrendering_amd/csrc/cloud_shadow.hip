@@ -236,6 +236,9 @@ extern "C" int skyhip_cloud_shadow_map(SkyhipContext* ctx, const SkyhipCloudComm
     switch (a.mat.kind) {
         case SKYHIP_MATERIAL_DEFAULT0: launch_shadow_map<SKYHIP_MATERIAL_DEFAULT0>(ctx->stream, a); break;
         case SKYHIP_MATERIAL_DEFAULT1: launch_shadow_map<SKYHIP_MATERIAL_DEFAULT1>(ctx->stream, a); break;
+        case SKYHIP_MATERIAL_DEFAULT1_WORLEY:
+            launch_shadow_map<SKYHIP_MATERIAL_DEFAULT1_WORLEY>(ctx->stream, a);
+            break;
         case SKYHIP_MATERIAL_MINIMAL: launch_shadow_map<SKYHIP_MATERIAL_MINIMAL>(ctx->stream, a); break;
         case SKYHIP_MATERIAL_VOXEL: launch_shadow_map<SKYHIP_MATERIAL_VOXEL>(ctx->stream, a); break;
         default: launch_shadow_map<SKYHIP_MATERIAL_VOXEL_F16>(ctx->stream, a); break;

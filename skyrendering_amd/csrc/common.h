@@ -57,6 +57,8 @@ struct SkyhipContext {
     // The bound ray-scatter-visibility froxel (R16 UNORM [d][h][w]), or null: visibility 1.
     const uint16_t* froxel = nullptr;
     int froxel_w = 0, froxel_h = 0, froxel_d = 0;
+    // The raymarch's dual-lobe phase (the reference's constant lobes by default).
+    SkyhipCloudPhase phase = {-0.15f, 2.16f, 0.85f, 0.0f};
 };
 
 namespace skyhip {

@@ -249,6 +249,19 @@ extern "C" int skyhip_bind_shadow_froxel(SkyhipContext* ctx, const uint16_t* fro
     return SKYHIP_OK;
 }
 
+extern "C" int skyhip_bind_cloud_phase(SkyhipContext* ctx, const SkyhipCloudPhase* phase) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (phase == nullptr) {
+        ctx->phase = SkyhipCloudPhase{-0.15f, 2.16f, 0.85f, 0.0f};  // VolumetricCloudRender.comp:125-126
+        return SKYHIP_OK;
+    }
+    if (!(std::fabs(phase->back_g) < 1.0f && std::fabs(phase->forward_g) < 1.0f && std::isfinite(phase->back_scale)))
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "phase lobes need |g| < 1 and a finite back_scale");
+    ctx->phase = *phase;
+    ctx->phase.padding = 0.0f;
+    return SKYHIP_OK;
+}
+
 extern "C" int skyhip_synchronize(SkyhipContext* ctx) {
     SKYHIP_CHECK(check_ctx(ctx));
     SKYHIP_HIP(hipStreamSynchronize(ctx->stream));
@@ -673,3 +686,5 @@ static_assert(sizeof(SkyhipCloudMaterial1BufferData) == 32, "material1 block is 
 static_assert(sizeof(SkyhipCloudMaterial0BufferData) == 16, "material0 block is 16 bytes");
 static_assert(sizeof(SkyhipCloudMinimalBufferData) == 16, "minimal block is 16 bytes");
 static_assert(sizeof(SkyhipCloudVoxelBufferData) == 32, "voxel block is 32 bytes");
+static_assert(sizeof(SkyhipCloudWorleyDetailBufferData) == 32, "Worley detail block is 32 bytes");
+static_assert(sizeof(SkyhipCloudPhase) == 16, "phase block is 16 bytes");

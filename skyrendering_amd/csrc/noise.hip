@@ -1,6 +1,7 @@
 // Cloud-texture generators for gfx950 (NoiseGen.comp + shaders/Base/Noise.glsl):
 // the 2D cloud map (RG8: Perlin-FBM coverage, Worley-FBM type), the 3D detail
-// volume (R8 Perlin-Worley) and the 2D displacement map (RGBA8 Perlin-FBM),
+// volume (R8 Perlin-Worley), the 3D Worley detail volume (R8 Worley-FBM) and the 2D
+// displacement map (RGBA8 Perlin-FBM),
 // plus the box-filter mip chain builder.  One lane per texel.  Hashes are
 // exact uint32 arithmetic; FBM sums run in FP32 as in the shader.
 #include <hip/hip_fp16.h>
@@ -144,6 +145,17 @@ __global__ __launch_bounds__(256) void detail_kernel(SkyhipNoiseCreateInfo perli
     out[t] = to_unorm8(fminf(fmaxf(w + p * (1.0f - w), 0.0f), 1.0f));  // RemapFrom01(p, w, 1)
 }
 
+// The Worley detail volume: WorleyFBM alone at texel centres, UNORM8.
+__global__ __launch_bounds__(256) void worley_kernel(SkyhipNoiseCreateInfo worley, uint8_t* __restrict__ out,
+                                                     int size) {
+    long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)size * size * size) return;
+    int ix = int(t % size), iy = int((t / size) % size), iz = int(t / ((long long)size * size));
+    float x = (float(ix) + 0.5f) / float(size), y = (float(iy) + 0.5f) / float(size),
+          z = (float(iz) + 0.5f) / float(size);
+    out[t] = to_unorm8(worley_fbm(x, y, z, worley));
+}
+
 __global__ __launch_bounds__(256) void displacement_kernel(SkyhipNoiseCreateInfo perlin, uint8_t* __restrict__ out,
                                                            int size) {
     int t = blockIdx.x * 256 + threadIdx.x;
@@ -275,6 +287,18 @@ extern "C" int skyhip_noise_displacement(SkyhipContext* ctx, const SkyhipNoiseCr
     hipLaunchKernelGGL(displacement_kernel, dim3(div_up((long long)size * size, 256)), dim3(256), 0, ctx->stream,
                        *perlin, out_rgba8, size);
     SKYHIP_LAUNCH_CHECK("displacement_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_noise_worley(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* worley, uint8_t* out_r8,
+                                   int32_t size) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(validate_noise(worley));
+    if (out_r8 == nullptr) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "output is NULL");
+    if (size < 1 || size > 1024) return fail(SKYHIP_ERR_INVALID_SIZE, "Worley volume size %d outside [1, 1024]", size);
+    hipLaunchKernelGGL(worley_kernel, dim3(div_up((long long)size * size * size, 256)), dim3(256), 0, ctx->stream,
+                       *worley, out_r8, size);
+    SKYHIP_LAUNCH_CHECK("worley_kernel");
     return SKYHIP_OK;
 }
 

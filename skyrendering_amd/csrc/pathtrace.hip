@@ -9,7 +9,8 @@
 // sample to the pixel's accumulator in frame order as the reference's
 // per-frame imageLoad/imageStore does.  Tracking runs either with the
 // reference's single majorant kSigmaTMax, drawing the reference's random
-// sequence, or over the 16^3-brick majorant grid of an FP16 density volume,
+// sequence, or over the B^3-brick majorant grid of an FP16 density volume (B = 16
+// by default),
 // walked with a 3D DDA in texel space: empty bricks cost one majorant fetch,
 // and inside occupied bricks the tentative-collision rate follows the local
 // maximum instead of the global one.
@@ -26,8 +27,9 @@ int validate_render(const SkyhipAtmosphereRenderBufferData* b);
 // Safety bound on tentative collisions per tracking call (never reached by a
 // finite majorant over a bounded region; keeps every wave's loop finite).
 constexpr int kMaxTrackingSteps = 1 << 20;
-// Safety bound on majorant segments per walk (a walk crosses at most the sum of the
-// brick-grid dimensions, 384 at the largest grid).
+// Safety bound on majorant segments per walk.  A walk crosses at most the sum of the
+// brick-grid dimensions plus two: grids of up to 4096 texels a side in bricks of at least 4
+// texels give at most 3 * 1024 + 2 segments.
 constexpr int kMaxSegments = 4096;
 
 struct PtArgs {
@@ -56,6 +58,8 @@ struct PtArgs {
     // Brick majorants: texel-space X = pos * tex_scale + tex_bias per axis (z on the flat height)
     const float* majorants;
     int nb[3];
+    int brick;        // brick edge B in texels (a power of two)
+    float inv_brick;  // 1 / B, exact
     float tex_scale[3], tex_bias[3], grid_n[3];
     float majorant_scale;
     // The context's bound ray-scatter-visibility froxel, or null (visibility 1)
@@ -122,14 +126,14 @@ __device__ __forceinline__ float sigma_at(const PtArgs& a, F3 p) {
     return sample_sigma_t<kMat>(a.mat, p, clamp01((p.z - a.hb) * a.inv_thickness), a.cam);
 }
 
-// Texels per brick edge of the majorant grid.  Measured on the 1024^2 workload (512^3 grid):
-// 8 -> 11.25 ms per frame (82 M lookups, 240 M segments), 12 -> 10.76 ms (102 M, 164 M),
+// Default texels per brick edge of the majorant grid.  Measured on the 1024^2 workload (512^3
+// grid): 8 -> 11.25 ms per frame (82 M lookups, 240 M segments), 12 -> 10.76 ms (102 M, 164 M),
 // 16 -> 10.70-10.86 ms (119 M, 123 M), 20 -> 12.02 ms, 24 -> 12.84 ms, 32 -> 14.68 ms: past 16
 // the looser majorants cost more null collisions than the shorter walk saves.
-constexpr int kBrick = 16;
+constexpr int kDefaultBrick = 16;
 
 // Majorant-segment walker over the brick grid: the ray's texel-space position is
-// X(t) = org + t k per axis.  Bricks are kBrick texels wide; the first and last brick of
+// X(t) = org + t k per axis.  Bricks are a.brick texels wide; the first and last brick of
 // the x and y axes extend one texel beyond the grid, where CLAMP_TO_BORDER samples
 // are still non-zero, so the walk covers [-1, N + 1] there and [0, N] in z.
 struct BrickWalk {
@@ -164,14 +168,14 @@ struct BrickWalk {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             float x = fmaf(t, k[i], org[i]);
-            b[i] = min(max(int(floorf(x * (1.0f / kBrick))), 0), a.nb[i] - 1);
+            b[i] = min(max(int(floorf(x * a.inv_brick)), 0), a.nb[i] - 1);
             tn[i] = next_boundary(a, i);
         }
         return true;
     }
     __device__ __forceinline__ float next_boundary(const PtArgs& a, int i) const {
-        if (k[i] > 0.0f) return b[i] < a.nb[i] - 1 ? (float(kBrick * (b[i] + 1)) - org[i]) * inv_k[i] : 3.0e38f;
-        if (k[i] < 0.0f) return b[i] > 0 ? (float(kBrick * b[i]) - org[i]) * inv_k[i] : 3.0e38f;
+        if (k[i] > 0.0f) return b[i] < a.nb[i] - 1 ? (float(a.brick * (b[i] + 1)) - org[i]) * inv_k[i] : 3.0e38f;
+        if (k[i] < 0.0f) return b[i] > 0 ? (float(a.brick * b[i]) - org[i]) * inv_k[i] : 3.0e38f;
         return 3.0e38f;
     }
     __device__ __forceinline__ float segment_end() const { return fminf(fminf(tn[0], tn[1]), fminf(tn[2], t_end)); }
@@ -392,11 +396,14 @@ __device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, flo
             has_scattered = true;
             o = o + d * t;
             if (kCount) wc.vertices++;
+            // A real collision scatters with probability albedo (the reference's is 1): the sun
+            // estimate at this vertex and the continuation both carry it.
+            thr = thr * a.albedo;
             float ph = pt_phase(a, dot(d, a.sun));
             L += thr * (ratio_track<kMat, kBricks, kCount>(a, rng, o, a.sun, wc) * ph * sun_illuminance(a, o));
             float vop;
             d = hg_sample(a, rng, d, vop);
-            thr = thr * (vop * a.albedo);
+            thr = thr * vop;
         }
     }
     return L;
@@ -478,15 +485,15 @@ __global__ __launch_bounds__(256) void path_trace_display_kernel(const float4* _
     for (int k = 0; k < 3; ++k) px[k] = __builtin_bit_cast(uint16_t, __float2half_rn(out[k]));
 }
 
-// Brick majorants of an FP16 grid: max over the kBrick^3 brick plus a one-texel apron.
+// Brick majorants of an FP16 grid: max over the B^3 brick plus a one-texel apron.
 __global__ __launch_bounds__(256) void majorant_kernel(const uint16_t* __restrict__ g, int nx, int ny, int nz,
-                                                       int bx, int by, int bz, float* __restrict__ out) {
+                                                       int bx, int by, int bz, int B, float* __restrict__ out) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= bx * by * bz) return;
     int x = i % bx, y = (i / bx) % by, z = i / (bx * by);
-    int x0 = max(kBrick * x - 1, 0), x1 = min(kBrick * x + kBrick, nx - 1);
-    int y0 = max(kBrick * y - 1, 0), y1 = min(kBrick * y + kBrick, ny - 1);
-    int z0 = max(kBrick * z - 1, 0), z1 = min(kBrick * z + kBrick, nz - 1);
+    int x0 = max(B * x - 1, 0), x1 = min(B * x + B, nx - 1);
+    int y0 = max(B * y - 1, 0), y1 = min(B * y + B, ny - 1);
+    int z0 = max(B * z - 1, 0), z1 = min(B * z + B, nz - 1);
     float m = 0.0f;
     for (int zz = z0; zz <= z1; ++zz)
         for (int yy = y0; yy <= y1; ++yy) {
@@ -508,6 +515,10 @@ static void launch_path_trace(hipStream_t st, const PtArgs& a) {
             hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1, false, kCount>), grid, dim3(256), 0, st,
                                a);
             break;
+        case SKYHIP_MATERIAL_DEFAULT1_WORLEY:
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY, false, kCount>), grid, dim3(256), 0,
+                               st, a);
+            break;
         case SKYHIP_MATERIAL_MINIMAL:
             hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_MINIMAL, false, kCount>), grid, dim3(256), 0, st,
                                a);
@@ -526,19 +537,29 @@ static void launch_path_trace(hipStream_t st, const PtArgs& a) {
 
 using namespace skyhip;
 
-extern "C" int skyhip_voxel_grid_brick_size(void) { return kBrick; }
+extern "C" int skyhip_voxel_grid_brick_size(void) { return kDefaultBrick; }
 
-extern "C" int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
-                                           int32_t dim_z, float* out_majorants) {
+static bool valid_brick(int b) { return b >= 4 && b <= 64 && (b & (b - 1)) == 0; }
+
+extern "C" int skyhip_voxel_grid_majorants_sized(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x,
+                                                 int32_t dim_y, int32_t dim_z, int32_t brick_size,
+                                                 float* out_majorants) {
     SKYHIP_CHECK(check_ctx(ctx));
     if (!grid || !out_majorants) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
     if (dim_x < 2 || dim_y < 2 || dim_z < 2 || dim_x > 4096 || dim_y > 4096 || dim_z > 4096)
         return fail(SKYHIP_ERR_INVALID_SIZE, "grid dims outside [2, 4096]");
-    int bx = div_up(dim_x, kBrick), by = div_up(dim_y, kBrick), bz = div_up(dim_z, kBrick);
+    if (!valid_brick(brick_size)) return fail(SKYHIP_ERR_INVALID_SIZE, "brick size must be a power of two in [4, 64]");
+    const int B = brick_size;
+    int bx = div_up(dim_x, B), by = div_up(dim_y, B), bz = div_up(dim_z, B);
     hipLaunchKernelGGL(majorant_kernel, dim3(div_up(bx * by * bz, 256)), dim3(256), 0, ctx->stream, grid, dim_x,
-                       dim_y, dim_z, bx, by, bz, out_majorants);
+                       dim_y, dim_z, bx, by, bz, B, out_majorants);
     SKYHIP_LAUNCH_CHECK("majorant_kernel");
     return SKYHIP_OK;
+}
+
+extern "C" int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
+                                           int32_t dim_z, float* out_majorants) {
+    return skyhip_voxel_grid_majorants_sized(ctx, grid, dim_x, dim_y, dim_z, kDefaultBrick, out_majorants);
 }
 
 extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
@@ -640,8 +661,11 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
     if (bricks) {
         const SkyhipCloudVoxelBufferData& vb = material->voxel;
         a.majorants = in->brick_majorants;
+        a.brick = in->brick_size == 0 ? kDefaultBrick : in->brick_size;
+        if (!valid_brick(a.brick)) return fail(SKYHIP_ERR_INVALID_SIZE, "brick size must be 0 or a power of two in [4, 64]");
+        a.inv_brick = 1.0f / float(a.brick);
         for (int k = 0; k < 3; ++k) {
-            a.nb[k] = div_up(material->voxel_grid_dim[k], kBrick);
+            a.nb[k] = div_up(material->voxel_grid_dim[k], a.brick);
             a.grid_n[k] = float(material->voxel_grid_dim[k]);
         }
         // X = (pos * freq + bias) * N ; Z = (z - hb) / (ht - hb) * N (BrickWalk subtracts hb)

@@ -23,6 +23,8 @@ _P = C.POINTER
 _abi.register("skyhip_voxel_grid_generate", C.c_int, [C.c_void_p, _P(_abi.VoxelGridCreateInfoC), C.c_void_p,
                                                       C.c_int32, C.c_int32, C.c_int32])
 _abi.register("skyhip_voxel_grid_brick_size", C.c_int, [])
+_abi.register("skyhip_voxel_grid_majorants_sized", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                                            C.c_int32, C.c_void_p])
 _abi.register("skyhip_voxel_grid_majorants", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                                        C.c_void_p])
 _abi.register("skyhip_cloud_path_trace", C.c_int, [
@@ -101,14 +103,16 @@ class VoxelGridCreateInfo:
 
 
 class VoxelGrid:
-    """FP16 density grid (x fastest) and its brick majorants, resident on the device."""
+    """FP16 density grid (x fastest) and its brick majorants (brick edge `brick` texels, the
+    library's default when None), resident on the device."""
 
-    def __init__(self, ctx: Context, dims=(512, 512, 512)):
+    def __init__(self, ctx: Context, dims=(512, 512, 512), brick=None):
         self.ctx = ctx
         self.dims = tuple(int(d) for d in dims)
         dx, dy, dz = self.dims
         self.density = ctx.alloc(dx * dy * dz * 2)
-        b = _abi.lib().skyhip_voxel_grid_brick_size()
+        b = int(brick) if brick else _abi.lib().skyhip_voxel_grid_brick_size()
+        self.brick = b
         self.brick_dims = tuple((d + b - 1) // b for d in self.dims)
         bx, by, bz = self.brick_dims
         self.majorants = ctx.alloc(bx * by * bz * 4)
@@ -129,8 +133,8 @@ class VoxelGrid:
         return self
 
     def build_majorants(self):
-        _abi.check(_abi.lib().skyhip_voxel_grid_majorants(self.ctx.handle, self.density.ptr, *self.dims,
-                                                          self.majorants.ptr))
+        _abi.check(_abi.lib().skyhip_voxel_grid_majorants_sized(self.ctx.handle, self.density.ptr, *self.dims,
+                                                                self.brick, self.majorants.ptr))
 
     def download(self):
         dx, dy, dz = self.dims
@@ -266,7 +270,7 @@ class PathTraceScene:
             cam, self.earth.bottom_radius, self.renderer.sun_direction, 203.0, 0)
         # Grid placed ahead of the camera (local frame), footprint w["grid_extent_km"].
         if grid is None:
-            grid = VoxelGrid(ctx, w["grid_dims"]).generate(w["grid"])
+            grid = VoxelGrid(ctx, w["grid_dims"], brick=w.get("brick")).generate(w["grid"])
         self.grid = grid
         center = presets.grid_center_ahead(self.model, cam, w["grid_distance_km"])
         ext = w["grid_extent_km"]
@@ -286,6 +290,7 @@ class PathTraceScene:
         i.ap_width, i.ap_height = r.init.aerial_perspective_lut_size
         i.ap_depth = r.init.aerial_perspective_lut_depth
         i.brick_majorants = grid.majorants.ptr if bricks else None
+        i.brick_size = grid.brick if bricks else 0
         self.inputs = i
         self.init = init or w["init"]
         self.tracer = PathTracing(ctx, (self.width, self.height), self.init, a.buffer, r.buffer, self.common,

@@ -156,6 +156,48 @@ def cloud_workload():
             "blue_noise_size": 128, "atmosphere": earth_100km()}
 
 
+# The raymarch workload as the cloud benchmark states it: on top of cloud_workload(), a 32^3 R8
+# Worley detail volume (WorleyFBM, seed 4, base frequency 2, no remap, tiled every 1.5 km,
+# eroding by (w + 0) * 0.35 * Material1's mask: SKYHIP_MATERIAL_DEFAULT1_WORLEY), dual-lobe
+# Henyey-Greenstein phase with g = 0.8 forward and g = -0.3 back (the reference's mixing rule
+# and back-lobe scale 2.16 kept), and a seeded 128^2 blue-noise tile (seed 11).
+CONFIG2_PHASE = (-0.3, 2.16, 0.8)
+
+
+def cloud_workload_config2():
+    from .cloud import NoiseCreateInfo, VolumetricCloudDefaultMaterial1Worley
+    w = cloud_workload()
+    w["material"] = VolumetricCloudDefaultMaterial1Worley.from_material1(
+        w["material"], worley=NoiseCreateInfo(4, 2, 0.0, 1.0), worley_size=32, worley_repeat_size=1.5,
+        worley_base=0.0, worley_scale=0.35)
+    w["phase"] = CONFIG2_PHASE
+    w["blue_noise_kind"] = "blue"
+    return w
+
+
+def blue_noise_tile(size=128, seed=11, iterations=8, sigma=None) -> np.ndarray:
+    """A seeded, tileable blue-noise tile of uint16 values (UNORM16): white noise whose low
+    frequencies are removed by repeated spectral shaping.  Each iteration high-passes the
+    current tile on the torus (FFT, gain 1 - exp(-|f|^2 / (2 sigma^2)), sigma = size / 8
+    cycles per tile by default) and replaces every value by its rank, so the result is a
+    permutation of the size^2 evenly spaced levels (a uniform histogram) with little power
+    at low frequencies."""
+    n = int(size)
+    sigma = n / 8.0 if sigma is None else sigma
+    x = np.random.default_rng(seed).random((n, n))
+    f = np.fft.fftfreq(n) * n
+    f2 = f[:, None] ** 2 + f[None, :] ** 2
+    gain = 1.0 - np.exp(-f2 / (2.0 * sigma * sigma))
+    levels = (np.arange(n * n) + 0.5) / (n * n)
+    for _ in range(iterations):
+        y = np.real(np.fft.ifft2(np.fft.fft2(x) * gain))
+        order = np.argsort(y, axis=None, kind="stable")
+        x = np.empty(n * n)
+        x[order] = levels
+        x = x.reshape(n, n)
+    return np.round(x * 65535.0).astype(np.uint16)
+
+
 def uniform_weather_map(size=512, seeds=(2, 3)) -> np.ndarray:
     cov = np.random.default_rng(seeds[0]).integers(0, 256, size=(size, size), dtype=np.uint8)
     typ = np.random.default_rng(seeds[1]).integers(0, 256, size=(size, size), dtype=np.uint8)

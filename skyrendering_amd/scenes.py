@@ -14,7 +14,7 @@ import numpy as np
 from . import _abi, presets
 from .atmosphere import (Atmosphere, AtmosphereRenderer, AtmosphereRenderInitParameters,
                          AtmosphereRenderParameters)
-from .cloud import CloudTextures, VolumetricCloudMinimalMaterial, cloud_raymarch, make_material
+from .cloud import CloudTextures, VolumetricCloudMinimalMaterial, bind_cloud_phase, cloud_raymarch, make_material
 from .device import Context
 
 
@@ -45,14 +45,20 @@ class CloudScene:
         self.textures = None
         if not isinstance(self.material, VolumetricCloudMinimalMaterial):
             m = self.material
-            self.textures = CloudTextures(ctx, m.cloud_map_size, m.detail_size, m.displacement_size)
+            self.textures = CloudTextures(ctx, m.cloud_map_size, m.detail_size, m.displacement_size,
+                                          worley_size=getattr(m, "worley_size", None))
             self.textures.generate(m)
             if weather == "uniform":
                 self.textures.upload_level0(cloud_map=presets.uniform_weather_map(m.cloud_map_size,
                                                                                   w["weather_seeds"]))
         # Blue noise.
         if blue_noise is None:
-            blue_noise = presets.uniform_blue_noise(w["blue_noise_size"], w["blue_noise_seed"])
+            if w.get("blue_noise_kind") == "blue":
+                blue_noise = presets.blue_noise_tile(w["blue_noise_size"], w["blue_noise_seed"])
+            else:
+                blue_noise = presets.uniform_blue_noise(w["blue_noise_size"], w["blue_noise_seed"])
+        # The raymarch's phase lobes (None: the reference's), bound on the context for each run.
+        self.phase = w.get("phase")
         self.blue_noise_host = np.ascontiguousarray(blue_noise, dtype=np.uint16)
         self.blue_noise = ctx.upload(self.blue_noise_host)
         # Blocks (VolumetricCloud::Update).
@@ -71,6 +77,7 @@ class CloudScene:
         a = self.atmosphere
         r = self.renderer
         aw, ah = r.init.aerial_perspective_lut_size
+        bind_cloud_phase(self.ctx, self.phase)
         cloud_raymarch(self.ctx, a.buffer, self.common, self.cloud_block, self.mat_block, a.transmittance_texture,
                        a.transmittance_size, r.aerial_perspective_luminance_texture,
                        r.aerial_perspective_transmittance_texture, (aw, ah, r.init.aerial_perspective_lut_depth),

@@ -134,3 +134,25 @@ def test_surrogate_network_kinds_match_the_header():
 def test_brick_size_matches_the_oracle():
     import oracle
     assert _abi.lib().skyhip_voxel_grid_brick_size() == oracle.BRICK_TEXELS == 16
+
+
+def test_material_kinds_and_new_blocks_match_the_header():
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "include", "skyhip.h")).read()
+    kinds = {m.group(1): int(m.group(2)) for m in re.finditer(r"SKYHIP_MATERIAL_(\w+) = (\d+)", text)}
+    assert kinds == {"DEFAULT0": _abi.MATERIAL_DEFAULT0, "DEFAULT1": _abi.MATERIAL_DEFAULT1,
+                     "MINIMAL": _abi.MATERIAL_MINIMAL, "VOXEL": _abi.MATERIAL_VOXEL,
+                     "VOXEL_F16": _abi.MATERIAL_VOXEL_F16, "DEFAULT1_WORLEY": _abi.MATERIAL_DEFAULT1_WORLEY}
+    assert C.sizeof(_abi.CloudWorleyDetailBufferData) == 32 and C.sizeof(_abi.CloudPhase) == 16
+    # The Worley fields are appended: every earlier field keeps its offset.
+    assert _abi.CloudMaterial.worley.offset > _abi.CloudMaterial.voxel_grid_padding.offset
+    assert _abi.PathTracingInputsC.brick_size.offset == _abi.PathTracingInputsC.ap_depth.offset + 4
+
+
+def test_phase_binding_and_brick_size_reject_bad_input_before_device_work():
+    L = _abi.lib()
+    assert L.skyhip_bind_cloud_phase(None, None) == _abi.ERR_INVALID_ARGUMENT
+    assert L.skyhip_voxel_grid_majorants_sized(None, None, 8, 8, 8, 8, None) == _abi.ERR_INVALID_ARGUMENT
+    assert L.skyhip_noise_worley(None, None, None, 32) == _abi.ERR_INVALID_ARGUMENT

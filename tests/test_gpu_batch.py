@@ -125,5 +125,12 @@ def test_batch_rejects_bad_input(ctx):
     with pytest.raises(_abi.SkyhipError):
         LutBatch(ctx, bad_sets, [])
     b = LutBatch(ctx, sets, views)
+    b.launch()
+    ctx.synchronize()
+    ref = [b.view_outputs(v)[0] for v in range(len(views))]
     with pytest.raises(_abi.SkyhipError):
         b.set_tuning(sky_view_block=96)
+    # A rejected tuning leaves the batch launchable with its previous tuning and results.
+    b.launch()
+    ctx.synchronize()
+    assert all(np.array_equal(x, b.view_outputs(v)[0]) for v, x in enumerate(ref))

@@ -35,7 +35,8 @@ pytestmark = pytest.mark.gpu
 def _oracle_inputs(scene, bricks=False):
     T, sv_l, sv_t, ap_l, ap_t, grid = scene.host_inputs()
     majorants = scene.grid.download_majorants() if bricks else None
-    return oracle.HostPathTraceInputs(scene.material, T, sv_l, sv_t, ap_l, ap_t, grid=grid, majorants=majorants)
+    return oracle.HostPathTraceInputs(scene.material, T, sv_l, sv_t, ap_l, ap_t, grid=grid, majorants=majorants,
+                                      brick=scene.grid.brick if bricks else None)
 
 
 def _gpu_samples(scene, frames, first=1):
@@ -84,6 +85,9 @@ def test_voxel_grid_generator_matches_oracle(ctx):
           f"nonzero {np.mean(ref > 0):.3f}")
     assert np.all(ok) and frac <= 1e-3
     assert np.array_equal(g.download_majorants(), oracle.brick_majorants(got))
+    for brick in (8, 32):
+        gb = VoxelGrid(ctx, dims, brick=brick).upload(got)
+        assert np.array_equal(gb.download_majorants(), oracle.brick_majorants(got, brick)), brick
 
 
 def test_single_majorant_samples_match_oracle(ctx):
@@ -107,9 +111,12 @@ def test_single_majorant_samples_match_oracle(ctx):
     assert frac >= 0.9 and abs(z) < 3.0
 
 
-def test_brick_majorant_samples_match_oracle(ctx):
+@pytest.mark.parametrize("brick", [16, 8])
+def test_brick_majorant_samples_match_oracle(ctx, brick):
     w = cases.small_grid_workload()
+    w["brick"] = brick
     scene = PathTraceScene(ctx, 32, 32, workload=w, bricks=True)
+    assert scene.grid.brick == brick
     frames = 4
     gpu = _gpu_samples(scene, frames)
     ref = oracle.path_trace(scene.atmosphere.buffer, scene.renderer.buffer, scene.common, scene.tracer.params,
@@ -122,7 +129,7 @@ def test_brick_majorant_samples_match_oracle(ctx):
     lum_g, lum_o = gpu[..., :3].sum(-1).ravel(), ref[..., :3].sum(-1).ravel()
     se = math.sqrt(lum_g.var() / lum_g.size + lum_o.var() / lum_o.size)
     z = (lum_g.mean() - lum_o.mean()) / se
-    print(f"brick majorants: {frac:.4f} of samples match ({scattered.mean():.3f} scattered, "
+    print(f"{brick}^3 brick majorants: {frac:.4f} of samples match ({scattered.mean():.3f} scattered, "
           f"{match[scattered].mean():.4f} of those); mean z {z:.2f}")
     assert scattered.mean() > 0.1
     assert frac >= 0.9 and abs(z) < 3.0

@@ -64,7 +64,8 @@ def test_minimal_material_raymarch_closed_form():
     sun = np.array(common.uSunDirection[:], np.float64)
     c = float(np.dot(sun, d))
     w = math.exp(-float(np.float32(0.3)) * sigma)
-    phase = _hg(c, -0.15) * 2.16 * (1 - w) + _hg(c, 0.85) * w
+    f32 = lambda x: float(np.float32(x))  # the lobes are FP32 parameters (SkyhipCloudPhase)
+    phase = _hg(c, f32(-0.15)) * f32(2.16) * (1 - w) + _hg(c, f32(0.85)) * w
     t_sun = math.exp(-sigma * 2.0)  # 6 light steps: t reaches exactly 1.0
     # num_steps = uint(max(steps * dist / max_dist, 1)) with dist = min(chord, max_dist)
     assert steps[0] == int(64 * min(chord, 3.0) / 3.0)
@@ -196,3 +197,117 @@ def test_froxel_visibility_scales_the_aerial_perspective_term():
     np.testing.assert_allclose(got[selb, :3], (L1b[:, :3] - ((1.0 - vis) * lap * (1.0 - L1b[:, 3]))[:, None])[selb],
                                rtol=1e-9, atol=1e-12)
     common.uInvShadowFroxelMaxDistance = base_inv
+
+
+def _constant_chain(size, levels, channels, values, dims):
+    n = sum(max(1, size >> l) ** dims for l in range(levels))
+    return np.tile(np.array(values, np.uint8), n)
+
+
+def _material1_constant(kind, cov, typ, detail, worley=None):
+    """A Material1 (or Material1 + Worley) material over constant textures, with the coverage, type,
+    detail and Worley values given as UNORM8 integers; returns (material, host inputs)."""
+    mat = _abi.CloudMaterial()
+    mat.kind = kind
+    for si in (mat.common.uCloudMapSampleInfo, mat.common.uDetailSampleInfo, mat.worley.uWorleySampleInfo):
+        si.frequency, si.k_lod = 0.1, 1.0
+    mat.common.uLodBias = 0.0
+    mat.common.uDensity = 7.0
+    p = mat.material1
+    p.uBaseDensityThreshold, p.uBaseEdgeHardness, p.uBaseHeightHardness = 0.25, 3.0, 1.5
+    p.uDetailBase, p.uDetailScale, p.uHeightCut, p.uEdgeCur = 0.125, 0.75, 0.375, 0.9
+    mat.worley.uWorleyBase, mat.worley.uWorleyScale = 0.0625, 0.5
+    mat.cloud_map_size, mat.cloud_map_levels = 16, 5
+    mat.detail_size, mat.detail_levels = 8, 4
+    mat.worley_size, mat.worley_levels = 4, 3
+    cm = _constant_chain(16, 5, 2, [cov, typ], 2)
+    det = _constant_chain(8, 4, 1, [detail], 3)
+    wor = _constant_chain(4, 3, 1, [worley if worley is not None else 0], 3)
+    inputs = oracle.HostCloudInputs(mat, cloud_map=cm, detail=det, worley=wor, T=np.ones((2, 2, 4), np.float32),
+                                    ap_l=np.zeros((2, 2, 2, 4), np.float32), ap_t=np.ones((2, 2, 2, 4), np.float32),
+                                    blue_noise=np.zeros((1, 1), np.uint16), image_size=(1, 1))
+    return inputs
+
+
+def _material1_closed_form(cov, detail, h01, worley=None):
+    """VolumetricCloudDefaultMaterial1.glsl:14-28 with the block of _material1_constant, plus the
+    Worley erosion (w + base) * scale * mask when `worley` is given (SKYHIP_MATERIAL_DEFAULT1_WORLEY)."""
+    c, d = cov / 255.0, detail / 255.0
+    sat = lambda x: min(max(x, 0.0), 1.0)
+    thr, edge, hh = float(np.float32(0.25)), 3.0, 1.5
+    base = sat((c - thr) * edge) * sat((1 - h01) * hh)
+    if base == 0:
+        return 0.0
+    mask = max(sat(h01 - 0.375), sat(float(np.float32(0.9)) - c))
+    dens = sat(base - (d + 0.125) * 0.75 * mask)
+    if worley is not None:
+        dens = sat(dens - (worley / 255.0 + 0.0625) * 0.5 * mask)
+    return dens * 7.0 * h01
+
+
+def test_material1_erosion_and_worley_term_closed_form():
+    """Material1's erosion (detail + base) * scale * max(sat(h01 - cut), sat(edge_cut - cov)) and the
+    Worley material's second erosion, on constant textures, at two heights: one where the
+    height branch of the max is larger and one where the coverage branch is.  Dropping the mask,
+    taking the wrong branch, or applying the Worley term after the density scale fails this."""
+    _, common, blk = _blocks()
+    camp = np.array(common.uCameraPos[:], np.float64)
+    L = oracle.lib()
+    cases = set()
+    for z in (1.9, 3.4):  # h01 ~ 0.16 (coverage branch) and ~ 0.76 (height branch)
+        pos = camp + np.array([2.0, 0.5, 0.0])
+        pos[2] = z
+        h01 = (math.sqrt(pos[0] ** 2 + pos[1] ** 2 + (pos[2] + 6360.0) ** 2) - 6360.0 - 1.5) / 2.5
+        for cov, det, wor in ((200, 64, 100), (230, 10, 255), (180, 200, 30)):
+            c = cov / 255.0
+            mask_h, mask_c = max(h01 - 0.375, 0.0), max(float(np.float32(0.9)) - c, 0.0)
+            branch = "height" if mask_h > mask_c else "coverage"
+            inp = _material1_constant(_abi.MATERIAL_DEFAULT1, cov, 0, det)
+            got1 = L.oracle_cloud_sigma_t_f64(C.byref(common), C.byref(blk), C.byref(inp.material), *pos)
+            want1 = _material1_closed_form(cov, det, h01)
+            assert abs(got1 - want1) < 1e-6 * max(1.0, abs(want1)), (branch, got1, want1)
+            inp = _material1_constant(_abi.MATERIAL_DEFAULT1_WORLEY, cov, 0, det, wor)
+            got2 = L.oracle_cloud_sigma_t_f64(C.byref(common), C.byref(blk), C.byref(inp.material), *pos)
+            want2 = _material1_closed_form(cov, det, h01, wor)
+            assert abs(got2 - want2) < 1e-6 * max(1.0, abs(want2)), (branch, got2, want2)
+            if want1 > 0 and max(mask_h, mask_c) > 0:
+                cases.add(branch)
+                assert want2 < want1  # the Worley term erodes where the mask is non-zero
+    assert cases == {"height", "coverage"}
+
+
+def test_bound_phase_lobes_closed_form():
+    """The closed form of test_minimal_material_raymarch_closed_form with the cloud benchmark's
+    lobes bound (g = 0.8 forward, -0.3 back, back scale 2.16): the pixel's luminance follows the
+    bound lobes, not the reference's; unbinding restores the reference's."""
+    width, height = 33, 19
+    cam, common, blk = _blocks(width=width, height=height)
+    sigma = float(np.float32(0.05))
+    mat = _abi.CloudMaterial()
+    mat.kind = _abi.MATERIAL_MINIMAL
+    mat.minimal.uDensity = sigma
+    T, apl, apt, bn = _unit_images(width, height)
+    inputs = oracle.HostCloudInputs(mat, T=T, ap_l=apl, ap_t=apt, blue_noise=bn, image_size=(width, height))
+    atm = presets.earth_100km().buffer_data()
+    px = (height // 2) * width + width // 2
+    ref, _, _ = oracle.cloud_raymarch(atm, common, blk, inputs, width, [px])
+    oracle.bind_cloud_phase(presets.CONFIG2_PHASE)
+    try:
+        got, _, _ = oracle.cloud_raymarch(atm, common, blk, inputs, width, [px])
+    finally:
+        oracle.bind_cloud_phase(None)
+    again, _, _ = oracle.cloud_raymarch(atm, common, blk, inputs, width, [px])
+    np.testing.assert_array_equal(again, ref)
+    ivp = np.array(common.uInvMVP[:], np.float64).reshape(4, 4).T
+    u, v = (width // 2 + 0.5) / width, (height // 2 + 0.5) / height
+    p = ivp @ np.array([2 * u - 1, 2 * v - 1, 1.0, 1.0])
+    d = p[:3] / p[3] - np.array(common.uCameraPos[:], np.float64)
+    d /= np.linalg.norm(d)
+    c = float(np.dot(np.array(common.uSunDirection[:], np.float64), d))
+    w = math.exp(-float(np.float32(0.3)) * sigma)
+    f32 = lambda x: float(np.float32(x))
+    phase_ref = _hg(c, f32(-0.15)) * f32(2.16) * (1 - w) + _hg(c, f32(0.85)) * w
+    phase_new = _hg(c, f32(-0.3)) * f32(2.16) * (1 - w) + _hg(c, f32(0.8)) * w
+    assert got[0, 3] == ref[0, 3]  # transmittance does not depend on the phase
+    np.testing.assert_allclose(got[0, :3] / ref[0, :3], phase_new / phase_ref, rtol=1e-9)
+    assert abs(phase_new / phase_ref - 1.0) > 0.05

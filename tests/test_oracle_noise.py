@@ -99,3 +99,44 @@ def test_mipmap_box_filter():
     c3 = np.zeros(8 + 1, np.uint8)
     c3[:8] = np.arange(8)
     assert oracle.generate_mipmaps(c3, 3, 1, 2, 2, 2, 2)[8] == 4
+
+
+def _fbm_from_octaves(L, noise, p, ci):
+    """PerlinFBM / WorleyFBM (NoiseGen.comp:20-50) composed in numpy from the oracle's single-octave
+    noise: 8 octaves, frequency doubling from base_frequency, amplitude halving from 0.5, each octave
+    remapped with RemapTo01 and the sum normalised by the amplitudes' sum."""
+    res, a, f, sum_a = 0.0, 0.5, ci.base_frequency, 0.0
+    for _ in range(8):
+        n = noise(*p, f, ci.seed)
+        if noise is L.oracle_perlin_noise:
+            n = n * 0.5 + 0.5
+        res += min(max((n - ci.remap_min) / (ci.remap_max - ci.remap_min), 0.0), 1.0) * a
+        sum_a += a
+        f *= 2
+        a *= 0.5
+    return res / sum_a
+
+
+def test_worley_and_detail_texels_compose_the_fbm():
+    """The Worley detail volume's texel is UNORM8(WorleyFBM(texel centre)), and the 128^3 detail
+    texel is UNORM8(RemapFrom01(PerlinFBM, WorleyFBM, 1)) (NoiseGen.comp:80-86): both follow from
+    the single-octave noises pinned above.  A wrong octave weight, a frequency that does not
+    double, a missing normalisation or a swapped Perlin/Worley role fails this."""
+    L = oracle.lib()
+    size = 32
+    worley = _abi.NoiseCreateInfo(4, 2, 0.1, 0.9)
+    perlin = _abi.NoiseCreateInfo(3, 3, 0.2, 0.8)
+    texels = np.arange(0, size ** 3, 331)
+    got_w = oracle.worley_texels(worley, size, texels)
+    got_d = oracle.detail_texels(perlin, worley, size, texels)
+    for t, gw, gd in zip(texels, got_w, got_d):
+        x, y, z = t % size, (t // size) % size, t // (size * size)
+        p = ((x + 0.5) / size, (y + 0.5) / size, (z + 0.5) / size)
+        w = _fbm_from_octaves(L, L.oracle_worley_noise, p, worley)
+        pf = _fbm_from_octaves(L, L.oracle_perlin_noise, p, perlin)
+        d = min(max(w + pf * (1.0 - w), 0.0), 1.0)
+        # UNORM8 = round(clamp(f) * 255); values within 1e-9 of a half step may round either way.
+        for val, f in ((gw, w), (gd, d)):
+            x255 = f * 255.0
+            if abs(x255 - math.floor(x255) - 0.5) > 1e-6:
+                assert int(val) == int(math.floor(x255 + 0.5)), (t, val, f)

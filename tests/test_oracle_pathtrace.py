@@ -18,6 +18,7 @@
 import math
 
 import numpy as np
+import pytest
 
 import oracle
 from skyrendering_amd import _abi, presets
@@ -64,14 +65,13 @@ def test_grid_generator_structure():
         assert abs(math.log(val) - free.lognormal_sigma * free.fbm_inv_std * fbm) < 2e-5
 
 
-def test_brick_majorants_bound_trilinear_samples():
+@pytest.mark.parametrize("B", [16, 8])
+def test_brick_majorants_bound_trilinear_samples(B):
     ci = VoxelGridCreateInfo()
-    dims = (40, 32, 24)  # x is not a multiple of 8: a partial last brick
+    dims = (44, 32, 24)  # x is not a multiple of 8 or 16: a partial last brick
     g = oracle.voxel_grid_f16(ci.to_c(), dims)
     gf = g.view(np.float16).astype(np.float32)
-    M = oracle.brick_majorants(g)
-    B = 16  # the library's brick edge in texels
-    assert oracle.BRICK_TEXELS == B
+    M = oracle.brick_majorants(g, B) if B != oracle.BRICK_TEXELS else oracle.brick_majorants(g)
     nz, ny, nx = gf.shape
     ref = np.zeros_like(M)
     for bz in range(M.shape[0]):
@@ -190,6 +190,35 @@ def test_single_scattering_closed_form():
     assert np.all(out[~scattered, :3] == 0.0)
 
 
+def test_albedo_weights_the_collision_vertex():
+    """Single scattering in the homogeneous slab with albedo 0.5: a real collision scatters with
+    probability albedo, so the sun estimate at the first vertex carries it.  The random sequence
+    does not depend on albedo, so each sample is exactly half the albedo-1 sample, and the mean
+    is half the closed form of test_single_scattering_closed_form.  Applying albedo only to the
+    continuing throughput (one vertex late) leaves the single-bounce image unchanged and fails."""
+    sigma, D, zen = 0.4, 2.5, 40.0
+    common = cases.straight_up_common(sun_zenith_deg=zen)
+    mat = cases.minimal_material(sigma)
+    one = _run(common, mat, cases.params(sigma_t_max=1.7 * sigma, max_bounces=1), cases.unit_luts()).reshape(-1, 4)
+    half = _run(common, mat, cases.params(sigma_t_max=1.7 * sigma, max_bounces=1, albedo=0.5),
+                cases.unit_luts()).reshape(-1, 4)
+    np.testing.assert_array_equal(half[:, 3], one[:, 3])
+    np.testing.assert_allclose(half[:, :3], 0.5 * one[:, :3], rtol=1e-12, atol=0)
+    mu_s = float(np.float32(math.cos(math.radians(zen))))
+    k = 1.0 - 1.0 / mu_s
+    expect = 0.5 * cases.hg_mixture(mu_s) * math.exp(-sigma * D / mu_s) * (1 - math.exp(-sigma * D * k)) / k
+    x = half[:, 0]
+    assert abs(x.mean() - expect) < 4 * x.std() / math.sqrt(len(x)), (x.mean(), expect)
+    # Two bounces: the second vertex's sun term carries albedo^2, so the albedo-0.5 image lies
+    # strictly between 0.25 and 0.5 of the albedo-1 image where second vertices contribute.
+    one2 = _run(common, mat, cases.params(sigma_t_max=1.7 * sigma, max_bounces=2), cases.unit_luts()).reshape(-1, 4)
+    half2 = _run(common, mat, cases.params(sigma_t_max=1.7 * sigma, max_bounces=2, albedo=0.5),
+                 cases.unit_luts()).reshape(-1, 4)
+    second = one2[:, 0] - one[:, 0]
+    np.testing.assert_allclose(half2[:, 0] - half[:, 0], 0.25 * second, rtol=1e-9, atol=1e-15)
+    assert np.sum(second > 0) > 100
+
+
 def test_white_furnace_constant_environment():
     sigma = 0.4
     common = cases.straight_up_common()
@@ -231,7 +260,7 @@ def _optical_depth(common, mat, prm, o, d, t0, t1, n=4000):
     return h * sum(oracle.pt_sigma_t(common, mat, prm, o + d * (t0 + (i + 0.5) * h)) for i in range(n))
 
 
-def _tracking_z_scores(common, mat, prm, rays, majorants, n=4000):
+def _tracking_z_scores(common, mat, prm, rays, majorants, n=4000, brick=None):
     """Per ray: z-scores of the ratio-tracking mean against exp(-tau), and of the delta-tracking
     probability of a collision before 0.3, 0.6 and 1.0 of the ray against 1 - exp(-tau(x))."""
     seeds = [oracle.lib().oracle_pcg_hash(i + 77) for i in range(n)]
@@ -241,9 +270,9 @@ def _tracking_z_scores(common, mat, prm, rays, majorants, n=4000):
         d = (e - o) / np.linalg.norm(e - o)
         t1 = float(np.linalg.norm(e - o))
         tau = _optical_depth(common, mat, prm, o, d, 0.0, t1)
-        tr = np.array([oracle.pt_track(common, mat, prm, s, o, d, "ratio", majorants) for s in seeds])
+        tr = np.array([oracle.pt_track(common, mat, prm, s, o, d, "ratio", majorants, brick) for s in seeds])
         z = [(tr.mean() - math.exp(-tau)) / (max(tr.std(), 1e-3) / math.sqrt(n))]
-        tc = np.array([oracle.pt_track(common, mat, prm, s, o, d, "delta", majorants) for s in seeds])
+        tc = np.array([oracle.pt_track(common, mat, prm, s, o, d, "delta", majorants, brick) for s in seeds])
         for frac in (0.3, 0.6, 1.0):
             p = 1.0 - math.exp(-_optical_depth(common, mat, prm, o, d, 0.0, frac * t1, 1000))
             hit = np.mean((tc >= 0) & (tc <= frac * t1))
@@ -252,7 +281,8 @@ def _tracking_z_scores(common, mat, prm, rays, majorants, n=4000):
     return out
 
 
-def test_brick_tracking_is_unbiased():
+@pytest.mark.parametrize("brick", [16, 8])
+def test_brick_tracking_is_unbiased(brick):
     """Delta and ratio tracking over the 16^3-brick majorant grid (and over the single
     majorant) against exp(-optical depth) from quadrature of sigma along each ray.  The grid
     is constant per brick with a gradient across bricks and a three-brick empty gap, so the
@@ -260,13 +290,13 @@ def test_brick_tracking_is_unbiased():
     the half-texel shell outside the grid where CLAMP_TO_BORDER samples are still non-zero.
     The same statistics reject majorants scaled by 0.7 and majorants mirrored along x (an
     index error), so they have the power to see a wrong walk."""
-    B = oracle.BRICK_TEXELS
+    B = brick
     dims = (9 * B, 4 * B, 2 * B)
     bz, by, bx = np.meshgrid(np.arange(2), np.arange(4), np.arange(9), indexing="ij")
     vals = (0.125 * (bx + 3 * by + 7 * bz)).astype(np.float16)
     vals[:, :, 2:5] = 0
     g = np.ascontiguousarray(np.repeat(np.repeat(np.repeat(vals, B, 0), B, 1), B, 2).view(np.uint16))
-    M = oracle.brick_majorants(g)
+    M = oracle.brick_majorants(g, B)
     assert (M == 0).any() and len(np.unique(M)) > 10
     mat = _abi.CloudMaterial()
     mat.kind = _abi.MATERIAL_VOXEL_F16
@@ -282,8 +312,8 @@ def test_brick_tracking_is_unbiased():
             ((0.0, 0.3, 0.0), (1.0, 0.6, 1.0)), ((-shell, 0.05, 0.0), (-shell, 0.95, 1.0)),
             ((0.1, 0.5, 0.0), (0.9, 0.5, 1.0)), ((1.0 + shell, 0.9, 1.0), (1.0 + shell, 0.1, 0.0))]
     for majorants in (M, None):
-        for tau, z in _tracking_z_scores(common, mat, prm, rays, majorants):
+        for tau, z in _tracking_z_scores(common, mat, prm, rays, majorants, brick=B):
             assert tau > 0.3 and max(abs(x) for x in z) < 4.0, (majorants is None, tau, z)
     for wrong in (M * np.float32(0.7), np.ascontiguousarray(np.flip(M, 2))):
-        zs = _tracking_z_scores(common, mat, prm, rays, wrong)
+        zs = _tracking_z_scores(common, mat, prm, rays, wrong, brick=B)
         assert max(max(abs(x) for x in z) for _, z in zs) > 6.0
