@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-epochs", type=int, default=2400)
     ap.add_argument("--surrogate-no-bf16", action="store_true", help="skip the BF16 / SiLU network comparison")
+    ap.add_argument("--batch-autotune-reps", type=int, default=2,
+                    help="repetitions per autotuner candidate of the batched LUT build (0: default tuning)")
     ap.add_argument("--sections", default="all",
                     help="comma-separated sections besides the headline raymarch: " + ", ".join(SECTIONS))
     return ap.parse_args()
@@ -415,9 +417,9 @@ def bench_cloud_reconstruct(ctx, scene, reps=50):
     return out
 
 
-def bench_lut_batch(ctx, launches=5):
-    """The batched LUT workload (256 atmosphere sets x 8 suns, mixed sizes): autotuned, then
-    `launches` graph launches timed with HIP events."""
+def bench_lut_batch(ctx, launches=5, autotune_reps=2):
+    """The batched LUT workload (256 atmosphere sets x 8 suns, mixed sizes): autotuned (unless
+    autotune_reps is 0), then `launches` graph launches timed with HIP events."""
     from skyrendering_amd import presets
     from skyrendering_amd.batch import LutBatch
     sets, views = presets.batched_lut_workload()
@@ -428,7 +430,10 @@ def bench_lut_batch(ctx, launches=5):
     b.launch()
     ctx.record(9)
     default_ms = ctx.elapsed_ms(8, 9)
-    tuned, tuned_pass_ms = b.autotune(repetitions=2)
+    if autotune_reps > 0:
+        tuned, tuned_pass_ms = b.autotune(repetitions=autotune_reps)
+    else:
+        tuned, tuned_pass_ms = b.tuning, 0.0
     b.launch()
     ctx.synchronize()
     ctx.record(8)
@@ -752,7 +757,7 @@ def main():
         progress(f"path tracer {out['path_trace']['ms_per_frame']} ms per frame")
     batch = None
     if "lut_batch" in want:
-        batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx)
+        batch, b_sets, b_views, out["lut_batch"] = bench_lut_batch(ctx, autotune_reps=args.batch_autotune_reps)
         progress(f"LUT batch {out['lut_batch']['ms_per_launch']} ms per launch")
     if "surrogate" in want:
         out["surrogate"] = bench_surrogate(ctx, epochs=args.surrogate_epochs, compare_bf16=not args.surrogate_no_bf16,

@@ -157,10 +157,10 @@ _PROTO.update({
                                          C.c_long, vp, vp, vp]),
     "oracle_cloud_raymarch_forced_f64": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
                                                 P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32,
-                                                vp, C.c_long, vp, vp, vp, vp, vp, vp]),
+                                                vp, C.c_long, vp, vp, vp, vp, vp, vp, vp, vp]),
     "oracle_cloud_raymarch_forced_f32": (None, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
                                                 P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32,
-                                                vp, C.c_long, vp, vp, vp, vp, vp, vp]),
+                                                vp, C.c_long, vp, vp, vp, vp, vp, vp, vp, vp]),
     "oracle_cloud_trace_f64": (i32, [P(_abi.AtmosphereBufferData), P(_abi.CloudCommonBufferData),
                                      P(_abi.CloudBufferData), P(_abi.CloudMaterial), P(CloudImagesC), i32, i32, vp,
                                      i32]),
@@ -248,8 +248,10 @@ def cloud_raymarch(atm, common, cloud, inputs: HostCloudInputs, width, pixels, p
 def cloud_raymarch_forced(atm, common, cloud, inputs: HostCloudInputs, width, pixels, forced_steps, precision="f64"):
     """Checker mode: replays another implementation's per-pixel step decisions
     (uint32[n, 2]: executed, scheduled; each segment 0 | segment 1 << 16) and returns
-    (rgba, distance, steps, early-exit margin, step-count margin).  precision "f32" runs the
-    oracle's FP32 instantiation under the same decisions."""
+    (rgba, distance, steps, early-exit margin, step-count margin, texel margin, LOD margin): the
+    last two are each pixel's smallest distance of a NEAREST texel coordinate from an integer
+    (texel units) and of a level-of-detail from a switch point, over every fetch of its march.
+    precision "f32" runs the oracle's FP32 instantiation under the same decisions."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     fs = np.ascontiguousarray(forced_steps, dtype=np.uint32).reshape(-1, 2)
     dt = _dt(precision)
@@ -258,10 +260,13 @@ def cloud_raymarch_forced(atm, common, cloud, inputs: HostCloudInputs, width, pi
     steps = np.empty(len(px), np.int32)
     margin = np.empty(len(px), np.float64)
     step_margin = np.empty(len(px), np.float64)
+    texel_margin = np.empty(len(px), np.float64)
+    lod_margin = np.empty(len(px), np.float64)
     getattr(lib(), f"oracle_cloud_raymarch_forced_{precision}")(
         C.byref(atm), C.byref(common), C.byref(cloud), C.byref(inputs.material), C.byref(inputs.images), width,
-        _ptr(px), len(px), _ptr(fs), _ptr(rgba), _ptr(dist), _ptr(steps), _ptr(margin), _ptr(step_margin))
-    return rgba, dist, steps, margin, step_margin
+        _ptr(px), len(px), _ptr(fs), _ptr(rgba), _ptr(dist), _ptr(steps), _ptr(margin), _ptr(step_margin),
+        _ptr(texel_margin), _ptr(lod_margin))
+    return rgba, dist, steps, margin, step_margin, texel_margin, lod_margin
 
 
 # ---------------------------------------------------------------------------

@@ -80,16 +80,22 @@ void cloud_raymarch(const SkyhipAtmosphereBufferData* atm, const SkyhipCloudComm
                            const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
                            const CloudImages* images, int width, const int32_t* pixels, long count, O* out_rgba,
                            O* out_dist, int32_t* out_steps, const uint32_t* forced_steps = nullptr,
-                           double* out_margin = nullptr, double* out_step_margin = nullptr) {
+                           double* out_margin = nullptr, double* out_step_margin = nullptr,
+                           double* out_texel_margin = nullptr, double* out_lod_margin = nullptr) {
     Atmosphere<S> A(*atm);
     CloudCtx<S> Cc(common, cloud, material);
 #pragma omp parallel for schedule(dynamic, 16)
     for (long i = 0; i < count; ++i) {
         int p = pixels[i];
         S margin = 0, step_margin = 0;
+        DecisionMargins fetch_margins;
+        if (out_texel_margin || out_lod_margin) ActiveMargins() = &fetch_margins;
         CloudPixelOut<S> o = CloudRaymarchPixel<S>(A, Cc, *images, p % width, p / width,
                                                    forced_steps ? forced_steps + 2 * i : nullptr, &margin, nullptr, 0,
                                                    &step_margin);
+        ActiveMargins() = nullptr;
+        if (out_texel_margin) out_texel_margin[i] = fetch_margins.texel;
+        if (out_lod_margin) out_lod_margin[i] = fetch_margins.lod;
         if (out_margin) out_margin[i] = double(margin);
         if (out_step_margin) out_step_margin[i] = double(step_margin);
         for (int k = 0; k < 4; ++k) out_rgba[i * 4 + k] = O(o.rgba[k]);
@@ -338,9 +344,10 @@ void oracle_cloud_raymarch_forced_f64(const SkyhipAtmosphereBufferData* atm, con
                                       const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
                                       const CloudImages* images, int width, const int32_t* pixels, long count,
                                       const uint32_t* forced_steps, double* out_rgba, double* out_dist,
-                                      int32_t* out_steps, double* out_margin, double* out_step_margin) {
+                                      int32_t* out_steps, double* out_margin, double* out_step_margin,
+                                      double* out_texel_margin, double* out_lod_margin) {
     cloud_raymarch<double>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps,
-                           forced_steps, out_margin, out_step_margin);
+                           forced_steps, out_margin, out_step_margin, out_texel_margin, out_lod_margin);
 }
 
 // The FP32 instantiation of the checker mode: an independent FP32 evaluation under the same
@@ -349,9 +356,10 @@ void oracle_cloud_raymarch_forced_f32(const SkyhipAtmosphereBufferData* atm, con
                                       const SkyhipCloudBufferData* cloud, const SkyhipCloudMaterial* material,
                                       const CloudImages* images, int width, const int32_t* pixels, long count,
                                       const uint32_t* forced_steps, float* out_rgba, float* out_dist,
-                                      int32_t* out_steps, double* out_margin, double* out_step_margin) {
+                                      int32_t* out_steps, double* out_margin, double* out_step_margin,
+                                      double* out_texel_margin, double* out_lod_margin) {
     cloud_raymarch<float>(atm, common, cloud, material, images, width, pixels, count, out_rgba, out_dist, out_steps,
-                          forced_steps, out_margin, out_step_margin);
+                          forced_steps, out_margin, out_step_margin, out_texel_margin, out_lod_margin);
 }
 
 // Per-step trace of one pixel (8 doubles per step, see CloudRaymarchPixel); returns the step count.

@@ -151,6 +151,36 @@ template <class S> inline int nearest_mip_level(S lambda, int levels) {
     return int(dd);
 }
 
+// The checker's record of how close the sampler's discrete choices came to flipping: the
+// smallest |x - round(x)| / max(|x|, 1) over the texel-space coordinates x = u * size of NEAREST
+// fetches (a texel choice flips where x crosses an integer; FP32 rounding of the sample position
+// moves x by a few ulps of x, hence the relative measure), and the smallest distance of lambda from the
+// magnification switch (lambda = 0.5) and from the boundary between two levels that the clamp
+// to the last level does not hide (lambda + 0.5 an integer).  Recorded only while a record is
+// active on the calling thread (ActiveMargins), so other callers pay nothing.
+struct DecisionMargins {
+    double texel = 1e30, lod = 1e30;
+};
+inline DecisionMargins*& ActiveMargins() {
+    static thread_local DecisionMargins* active = nullptr;
+    return active;
+}
+template <class S> inline void note_lod(S lambda, int levels) {
+    DecisionMargins* m = ActiveMargins();
+    const double l = double(lambda);
+    if (!m || !std::isfinite(l)) return;
+    m->lod = std::min(m->lod, std::fabs(l - 0.5));
+    const double x = l + 0.5, n = std::round(x);
+    if (l > 0.0 && n - 1.0 < double(levels - 1)) m->lod = std::min(m->lod, std::fabs(x - n));
+}
+template <class S> inline void note_texel(S x) {
+    DecisionMargins* m = ActiveMargins();
+    if (m) {
+        const double v = double(x);
+        m->texel = std::min(m->texel, std::fabs(v - std::round(v)) / std::max(std::fabs(v), 1.0));
+    }
+}
+
 inline int wrap_repeat(long i, int n) {
     long m = i % n;
     return int(m < 0 ? m + n : m);
@@ -162,6 +192,7 @@ inline int wrap_repeat(long i, int n) {
 template <class S>
 inline void texlod2d_u8_repeat(const uint8_t* tex, int size, int levels, int channels, S u, S v, S lambda,
                                int nch, S* out) {
+    note_lod(lambda, levels);
     int lvl = nearest_mip_level(lambda, levels);
     if (lvl < 0) {
         S x = u * S(size) - S(0.5), y = v * S(size) - S(0.5);
@@ -179,6 +210,8 @@ inline void texlod2d_u8_repeat(const uint8_t* tex, int size, int levels, int cha
     }
     int s = std::max(1, size >> lvl);
     const uint8_t* base = tex + mip_offset(size, 2, channels, lvl);
+    note_texel(u * S(s));
+    note_texel(v * S(s));
     int xi = wrap_repeat(long(std::floor(u * S(s))), s), yi = wrap_repeat(long(std::floor(v * S(s))), s);
     for (int c = 0; c < nch; ++c) out[c] = S(base[(size_t(yi) * s + xi) * channels + c]) / S(255);
 }
@@ -188,6 +221,7 @@ inline void texlod2d_u8_repeat(const uint8_t* tex, int size, int levels, int cha
 // CLAMP_TO_BORDER with border colour 0 instead of REPEAT.
 template <class S>
 inline S texlod3d_u8(const uint8_t* tex, const int dim[3], int levels, S u, S v, S w, S lambda, bool border) {
+    note_lod(lambda, levels);
     int lvl = nearest_mip_level(lambda, levels);
     S coord[3] = {u, v, w};
     if (lvl < 0) {
@@ -233,6 +267,7 @@ inline S texlod3d_u8(const uint8_t* tex, const int dim[3], int levels, S u, S v,
     }
     int idx[3];
     for (int k = 0; k < 3; ++k) {
+        note_texel(coord[k] * S(d[k]));
         long i = long(std::floor(coord[k] * S(d[k])));
         if (border) {
             if (i < 0 || i >= d[k]) return S(0);

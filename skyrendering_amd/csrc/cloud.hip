@@ -334,11 +334,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
 }
 
 // Texture descriptor from the ABI material (levels packed back to back).
+static int log2_exact(int x) {  // log2 of a power of two, else -1
+    int l = 0;
+    while ((1 << l) < x) ++l;
+    return (1 << l) == x ? l : -1;
+}
+
 static void make_tex2d(Tex2D& t, const uint8_t* data, int size, int levels, int channels) {
     t.data = data;
     t.size = size;
     t.levels = levels;
     t.channels = channels;
+    t.log2_size = log2_exact(size);
     uint32_t off = 0;
     for (int l = 0; l < kMaxMipLevels; ++l) {
         t.offset[l] = off;
@@ -352,6 +359,7 @@ static void make_tex3d(Tex3D& t, const uint8_t* data, const int dim[3], int leve
     t.data = data;
     t.dim[0] = dim[0], t.dim[1] = dim[1], t.dim[2] = dim[2];
     t.levels = levels;
+    t.log2_size = (dim[0] == dim[1] && dim[1] == dim[2]) ? log2_exact(dim[0]) : -1;
     uint32_t off = 0;
     for (int l = 0; l < kMaxMipLevels; ++l) {
         t.offset[l] = off;
@@ -396,6 +404,18 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
                 make_tex3d(d.worley_tex, m->worley_texture, wdim, m->worley_levels);
             }
             if (!m->cloud_map || !m->detail) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "material texture is NULL");
+            // RG8 texels are read as 2-byte and RGBA8 texels as 4-byte loads.
+            if (reinterpret_cast<uintptr_t>(m->cloud_map) % 2 != 0 ||
+                (m->kind == SKYHIP_MATERIAL_DEFAULT0 && reinterpret_cast<uintptr_t>(m->displacement) % 4 != 0))
+                return fail(SKYHIP_ERR_INVALID_ARGUMENT, "cloud map must be 2-byte and displacement 4-byte aligned");
+            // lambda = log2(k_lod distance) + bias = log2(distance) + (log2(k_lod) + bias), rounded once.
+            auto lod_const = [&](const SkyhipSampleInfo& si) {
+                return float(std::log2(double(si.k_lod)) + double(m->common.uLodBias));
+            };
+            d.lod_cloud_map = lod_const(m->common.uCloudMapSampleInfo);
+            d.lod_detail = lod_const(m->common.uDetailSampleInfo);
+            d.lod_displacement = lod_const(m->common.uDisplacementSampleInfo);
+            if (m->kind == SKYHIP_MATERIAL_DEFAULT1_WORLEY) d.lod_worley = lod_const(m->worley.uWorleySampleInfo);
             if (!pow2(m->cloud_map_size) || !pow2(m->detail_size) || m->cloud_map_size > 16384 || m->detail_size > 1024)
                 return fail(SKYHIP_ERR_INVALID_SIZE, "cloud map / detail sizes must be powers of two");
             if (m->cloud_map_levels < 1 || m->cloud_map_levels > skyhip_mip_levels(m->cloud_map_size, m->cloud_map_size, 1) ||

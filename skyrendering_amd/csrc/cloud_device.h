@@ -20,6 +20,7 @@ constexpr int kMaxMipLevels = 16;
 struct Tex2D {
     const uint8_t* data;
     int size, levels, channels;
+    int log2_size;  // size is a power of two (REPEAT textures)
     uint32_t offset[kMaxMipLevels];
 };
 
@@ -27,6 +28,7 @@ struct Tex3D {
     const uint8_t* data;
     int dim[3];
     int levels;
+    int log2_size;  // cube edge log2 for REPEAT volumes (power-of-two cubes), -1 otherwise
     uint32_t offset[kMaxMipLevels];
 };
 
@@ -39,35 +41,76 @@ __device__ __forceinline__ int nearest_level(float lambda, int levels) {
 
 __device__ __forceinline__ int floor_to_int(float x) { return int(floorf(fminf(fmaxf(x, -1.0e9f), 1.0e9f))); }
 
-// textureLod(sampler2D REPEAT) of the first two channels of an RG8/RGBA8 texture.
+// a + t (b - a): the linear interpolation GL's LINEAR filter weights form (exact in real arithmetic).
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+// One texel of an RG8 (kChannels 2) or RGBA8 (4) texture as one 2- or 4-byte load; texel offsets are
+// 32-bit and the base is uniform, so the load takes the scalar-base form.
+template <int kChannels>
+__device__ __forceinline__ uint32_t texel_bits(const uint8_t* base, uint32_t index) {
+    const uint32_t byte_offset = index * uint32_t(kChannels);  // 32-bit offset from the uniform base
+    if (kChannels == 2) return *reinterpret_cast<const uint16_t*>(base + byte_offset);
+    return *reinterpret_cast<const uint32_t*>(base + byte_offset);
+}
+__device__ __forceinline__ float unorm_byte(uint32_t bits, int c) { return float((bits >> (8 * c)) & 0xffu); }
+
+// textureLod(sampler2D REPEAT) of an RG8 / RGBA8 texture whose size is a power of two; out[c]
+// for the first kChannels channels (t.channels == kChannels).
 template <int kChannels>
 __device__ inline void tex2d_repeat(const Tex2D& t, float u, float v, float lambda, float out[kChannels]) {
     if (!(lambda > 0.5f)) {
-        const int s = t.size, m = s - 1;
-        float x = u * float(s) - 0.5f, y = v * float(s) - 0.5f;
-        int ix = floor_to_int(x), iy = floor_to_int(y);
-        float ax = x - floorf(x), ay = y - floorf(y);
-        int x0 = ix & m, x1 = (ix + 1) & m, y0 = iy & m, y1 = (iy + 1) & m;
-        const uint8_t* d = t.data;
-        for (int c = 0; c < kChannels; ++c) {
-            float t00 = d[(y0 * s + x0) * t.channels + c], t10 = d[(y0 * s + x1) * t.channels + c];
-            float t01 = d[(y1 * s + x0) * t.channels + c], t11 = d[(y1 * s + x1) * t.channels + c];
-            out[c] = ((1.0f - ay) * ((1.0f - ax) * t00 + ax * t10) + ay * ((1.0f - ax) * t01 + ax * t11)) *
-                     (1.0f / 255.0f);
-        }
+        const int s = t.size, ls = t.log2_size;
+        const uint32_t m = uint32_t(s - 1);
+        const float x = u * float(s) - 0.5f, y = v * float(s) - 0.5f;
+        const int ix = floor_to_int(x), iy = floor_to_int(y);
+        const float ax = x - floorf(x), ay = y - floorf(y);
+        const uint32_t x0 = uint32_t(ix) & m, x1 = uint32_t(ix + 1) & m;
+        const uint32_t r0 = (uint32_t(iy) & m) << ls, r1 = (uint32_t(iy + 1) & m) << ls;
+        const uint32_t b00 = texel_bits<kChannels>(t.data, r0 | x0), b10 = texel_bits<kChannels>(t.data, r0 | x1);
+        const uint32_t b01 = texel_bits<kChannels>(t.data, r1 | x0), b11 = texel_bits<kChannels>(t.data, r1 | x1);
+        for (int c = 0; c < kChannels; ++c)
+            out[c] = lerpf(lerpf(unorm_byte(b00, c), unorm_byte(b10, c), ax),
+                           lerpf(unorm_byte(b01, c), unorm_byte(b11, c), ax), ay) * (1.0f / 255.0f);
         return;
     }
     const int l = nearest_level(lambda, t.levels);
-    const int s = max(1, t.size >> l), m = s - 1;
-    int xi = floor_to_int(u * float(s)) & m, yi = floor_to_int(v * float(s)) & m;
-    const uint8_t* d = t.data + t.offset[l] + (yi * s + xi) * t.channels;
-    for (int c = 0; c < kChannels; ++c) out[c] = float(d[c]) * (1.0f / 255.0f);
+    const int ls = max(t.log2_size - l, 0);
+    const int s = 1 << ls;
+    const uint32_t m = uint32_t(s - 1);
+    const uint32_t xi = uint32_t(floor_to_int(u * float(s))) & m, yi = uint32_t(floor_to_int(v * float(s))) & m;
+    const uint32_t b = texel_bits<kChannels>(t.data + t.offset[l], (yi << ls) | xi);
+    for (int c = 0; c < kChannels; ++c) out[c] = unorm_byte(b, c) * (1.0f / 255.0f);
 }
 
-// textureLod(sampler3D) of an R8 volume; kBorder selects CLAMP_TO_BORDER(0),
-// otherwise REPEAT (power-of-two dims).
-template <bool kBorder>
-__device__ inline float tex3d_r8(const Tex3D& t, float u, float v, float w, float lambda) {
+// textureLod(sampler3D REPEAT) of an R8 volume that is a power-of-two cube.
+__device__ inline float tex3d_r8_repeat(const Tex3D& t, float u, float v, float w, float lambda) {
+    const uint8_t* d = t.data;
+    if (!(lambda > 0.5f)) {
+        const int ls = t.log2_size, s = 1 << ls;
+        const uint32_t m = uint32_t(s - 1);
+        const float x = u * float(s) - 0.5f, y = v * float(s) - 0.5f, z = w * float(s) - 0.5f;
+        const int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
+        const float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
+        const uint32_t x0 = uint32_t(ix) & m, x1 = uint32_t(ix + 1) & m;
+        const uint32_t y0 = (uint32_t(iy) & m) << ls, y1 = (uint32_t(iy + 1) & m) << ls;
+        const uint32_t z0 = (uint32_t(iz) & m) << (2 * ls), z1 = (uint32_t(iz + 1) & m) << (2 * ls);
+        const float c00 = lerpf(float(d[z0 | y0 | x0]), float(d[z0 | y0 | x1]), ax);
+        const float c10 = lerpf(float(d[z0 | y1 | x0]), float(d[z0 | y1 | x1]), ax);
+        const float c01 = lerpf(float(d[z1 | y0 | x0]), float(d[z1 | y0 | x1]), ax);
+        const float c11 = lerpf(float(d[z1 | y1 | x0]), float(d[z1 | y1 | x1]), ax);
+        return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az) * (1.0f / 255.0f);
+    }
+    const int l = nearest_level(lambda, t.levels);
+    const int ls = max(t.log2_size - l, 0);
+    const uint32_t m = uint32_t((1 << ls) - 1);
+    const float s = float(1 << ls);
+    const uint32_t xi = uint32_t(floor_to_int(u * s)) & m, yi = uint32_t(floor_to_int(v * s)) & m,
+                   zi = uint32_t(floor_to_int(w * s)) & m;
+    return float(d[t.offset[l] + ((zi << (2 * ls)) | (yi << ls) | xi)]) * (1.0f / 255.0f);
+}
+
+// textureLod(sampler3D CLAMP_TO_BORDER(0)) of an R8 volume of any dimensions.
+__device__ inline float tex3d_r8_border(const Tex3D& t, float u, float v, float w, float lambda) {
     const uint8_t* d = t.data;
     if (!(lambda > 0.5f)) {
         const int sx = t.dim[0], sy = t.dim[1], sz = t.dim[2];
@@ -78,14 +121,8 @@ __device__ inline float tex3d_r8(const Tex3D& t, float u, float v, float w, floa
         for (int k = 0; k < 8; ++k) {
             int xx = ix + (k & 1), yy = iy + ((k >> 1) & 1), zz = iz + ((k >> 2) & 1);
             float wgt = ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az);
-            float texel;
-            if (kBorder) {
-                bool inside = xx >= 0 && xx < sx && yy >= 0 && yy < sy && zz >= 0 && zz < sz;
-                texel = inside ? float(d[((long long)zz * sy + yy) * sx + xx]) : 0.0f;
-            } else {
-                xx &= sx - 1, yy &= sy - 1, zz &= sz - 1;
-                texel = float(d[((long long)zz * sy + yy) * sx + xx]);
-            }
+            bool inside = xx >= 0 && xx < sx && yy >= 0 && yy < sy && zz >= 0 && zz < sz;
+            float texel = inside ? float(d[((long long)zz * sy + yy) * sx + xx]) : 0.0f;
             r += wgt * texel;
         }
         return r * (1.0f / 255.0f);
@@ -93,11 +130,7 @@ __device__ inline float tex3d_r8(const Tex3D& t, float u, float v, float w, floa
     const int l = nearest_level(lambda, t.levels);
     const int sx = max(1, t.dim[0] >> l), sy = max(1, t.dim[1] >> l), sz = max(1, t.dim[2] >> l);
     int xi = floor_to_int(u * float(sx)), yi = floor_to_int(v * float(sy)), zi = floor_to_int(w * float(sz));
-    if (kBorder) {
-        if (xi < 0 || xi >= sx || yi < 0 || yi >= sy || zi < 0 || zi >= sz) return 0.0f;
-    } else {
-        xi &= sx - 1, yi &= sy - 1, zi &= sz - 1;
-    }
+    if (xi < 0 || xi >= sx || yi < 0 || yi >= sy || zi < 0 || zi >= sz) return 0.0f;
     return float(d[t.offset[l] + ((long long)zi * sy + yi) * sx + xi]) * (1.0f / 255.0f);
 }
 
@@ -141,9 +174,12 @@ __device__ inline float grid_f16_trilinear(const GridF16& g, float u, float v, f
     return r;
 }
 
-// Material parameters as the kernels see them.
+// Material parameters as the kernels see them.  lod_* = log2(k_lod) + uLodBias of each
+// texture's SampleInfo, so that lambda = log2(k_lod distance) + bias is log2(distance) + lod_*
+// with one log2 per density evaluation.
 struct MaterialDev {
     int kind;
+    float lod_cloud_map, lod_detail, lod_displacement, lod_worley;
     SkyhipCloudMaterialCommonBufferData common;
     SkyhipCloudMaterial1BufferData m1;
     SkyhipCloudMaterial0BufferData m0;
@@ -178,13 +214,15 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
         float u = pos.x * M.voxel.uSampleFrequency[0] + M.voxel.uSampleBias[0];
         float v = pos.y * M.voxel.uSampleFrequency[1] + M.voxel.uSampleBias[1];
         float lod = __log2f(M.voxel.uSampleLodK * dist) + M.voxel.uLodBias;
-        return tex3d_r8<true>(M.voxel_tex, u, v, height01, lod) * M.voxel.uDensity;
+        return tex3d_r8_border(M.voxel_tex, u, v, height01, lod) * M.voxel.uDensity;
     }
+    // GetUVWLod (VolumetricCloudDefaultMaterialCommon.glsl:20-24) for every texture of the evaluation.
+    const float log2_dist = __log2f(dist);
     const SkyhipSampleInfo& ci = M.common.uCloudMapSampleInfo;
     const SkyhipSampleInfo& di = M.common.uDetailSampleInfo;
     float ct[2];
     tex2d_repeat<2>(M.cloud_map, pos.x * ci.frequency + ci.bias[0], pos.y * ci.frequency + ci.bias[1],
-                    __log2f(ci.k_lod * dist) + M.common.uLodBias, ct);
+                    log2_dist + M.lod_cloud_map, ct);
     if (kMat == SKYHIP_MATERIAL_DEFAULT1 || kMat == SKYHIP_MATERIAL_DEFAULT1_WORLEY) {
         // VolumetricCloudDefaultMaterial1.glsl:14-28
         const SkyhipCloudMaterial1BufferData& p = M.m1;
@@ -192,8 +230,8 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
         density *= clamp01((1.0f - height01) * p.uBaseHeightHardness);
         if (density == 0.0f) return 0.0f;
         if (kCount) wc->detail++;
-        float detail = tex3d_r8<false>(M.detail, pos.x * di.frequency + di.bias[0], pos.y * di.frequency + di.bias[1],
-                                       pos.z * di.frequency, __log2f(di.k_lod * dist) + M.common.uLodBias);
+        float detail = tex3d_r8_repeat(M.detail, pos.x * di.frequency + di.bias[0], pos.y * di.frequency + di.bias[1],
+                                       pos.z * di.frequency, log2_dist + M.lod_detail);
         const float mask = fmaxf(clamp01(height01 - p.uHeightCut), clamp01(p.uEdgeCur - ct[0]));
         detail = (detail + p.uDetailBase) * p.uDetailScale;
         detail *= mask;
@@ -205,9 +243,9 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
             const SkyhipCloudWorleyDetailBufferData& q = M.worley;
             const SkyhipSampleInfo& wi = q.uWorleySampleInfo;
             if (density > 0.0f || !(q.uWorleyBase >= 0.0f && q.uWorleyScale >= 0.0f)) {
-                float w = tex3d_r8<false>(M.worley_tex, pos.x * wi.frequency + wi.bias[0],
+                float w = tex3d_r8_repeat(M.worley_tex, pos.x * wi.frequency + wi.bias[0],
                                           pos.y * wi.frequency + wi.bias[1], pos.z * wi.frequency,
-                                          __log2f(wi.k_lod * dist) + M.common.uLodBias);
+                                          log2_dist + M.lod_worley);
                 w = (w + q.uWorleyBase) * q.uWorleyScale;
                 density = clamp01(density - w * mask);
             }
@@ -216,14 +254,14 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
     }
     // SKYHIP_MATERIAL_DEFAULT0, VolumetricCloudDefaultMaterial0.glsl:18-31
     const SkyhipSampleInfo& si = M.common.uDisplacementSampleInfo;
-    float lod = __log2f(si.k_lod * dist) + M.common.uLodBias;
+    float lod = log2_dist + M.lod_displacement;
     float u = pos.x * si.frequency + si.bias[0], v = pos.y * si.frequency + si.bias[1], w = pos.z * si.frequency;
     float d0[4], d1[4];
     tex2d_repeat<4>(M.displacement, u, v, lod, d0);
     tex2d_repeat<4>(M.displacement, u, w, lod, d1);
     F3 p2 = pos + M.m0.uDisplacementScale * F3{d0[0] + d1[2], d0[1], d1[3]};
-    float detail = tex3d_r8<false>(M.detail, p2.x * di.frequency + di.bias[0], p2.y * di.frequency + di.bias[1],
-                                   p2.z * di.frequency, __log2f(di.k_lod * length3(p2 - cam)) + M.common.uLodBias);
+    float detail = tex3d_r8_repeat(M.detail, p2.x * di.frequency + di.bias[0], p2.y * di.frequency + di.bias[1],
+                                   p2.z * di.frequency, __log2f(length3(p2 - cam)) + M.lod_detail);
     detail = detail * M.m0.uDetailParam[0] + M.m0.uDetailParam[1];
     float hit = clamp01(height01 / ct[1]);                  // CalHeightMask (:9-12)
     float mask = clamp01(hit * (hit - 1.0f) * -4.0f);
@@ -247,7 +285,7 @@ struct Shell {
 // CalHeight01 (VolumetricCloudCommon.glsl:36-40) via |p'|^2 - R^2 = x^2 + y^2 + z (z + 2R).
 __device__ __forceinline__ float height01(const Shell& S, F3 p, float* q_out = nullptr) {
     float q = p.x * p.x + p.y * p.y + p.z * (p.z + 2.0f * S.R);
-    float alt = q / (sqrtf(fmaxf(q + S.R * S.R, 0.0f)) + S.R);
+    float alt = q * __builtin_amdgcn_rcpf(sqrtf(fmaxf(q + S.R * S.R, 0.0f)) + S.R);  // 1-ulp reciprocal
     if (q_out) *q_out = q;
     return clamp01((alt - S.hb) * S.inv_thickness);
 }

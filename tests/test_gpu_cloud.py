@@ -12,14 +12,17 @@ mip choices (GL NEAREST_MIPMAP_NEAREST) and the sigma < 1e-5 skip.  Where FP32 a
 land on opposite sides of one of them the pixel changes by O(1).  So the oracle replays
 the GPU's per-pixel step counts and early exits, reports for each disagreement how far
 it is from the boundary (step counts: |x - round(x)|, bar 1e-3; early exits:
-|T - 0.01| / 0.01, counted as an outlier above 1e-3), and compares the images under
-identical decisions.  Metrics (stated with every result): the fraction of pixels whose
-per-pixel relative error exceeds 1e-3 (floor 1e-3 * max) -- bar 1% for NEAREST-mip
-scenes, 0.1% when every fetch is bilinear/trilinear -- and the relative L2 error over
-the image ||gpu - oracle||_2 / ||oracle||_2 on RGB and on T, bar 1e-3 (1e-2 for the
-scene whose rays cross the planet and sample clouds ~1000 km away, where a single
-flipped pixel dominates the L2).  The RGBA16F output's half rounding (<= 2^-11
-relative) is inside these bars.
+|T - 0.01| / 0.01), records how close each pixel's NEAREST texel and level-of-detail
+choices came to a boundary, and compares the images under identical step decisions.
+Metrics (stated with every result): the relative L2 error ||gpu - oracle||_2 / ||oracle||_2
+on RGB and on T over the pixels with no decision near a boundary (bar 1e-3 in every scene),
+the free-running relative L2 error (bar 1e-3; 1e-2 for the scene whose rays cross the planet
+and sample clouds ~1000 km away, where one flipped pixel dominates the L2), and the fraction
+of pixels whose per-pixel relative error exceeds 1e-3 (floor 1e-3 * max) with no exit or
+fetch flip to explain it (bar per scene, about twice the rate measured on MI355X: the
+remaining cases are opaque pixels tens of km away, whose FP32 texel coordinates resolve the
+filter weights to ~1e-4, and nearly transparent pixels, whose luminance scales with 1 - T).
+The RGBA16F output's half rounding (<= 2^-11 relative) is inside these bars.
 """
 import ctypes as C
 
@@ -89,6 +92,12 @@ def _metrics(g, ref):
     return l2_rgb, l2_t, per
 
 
+# FP32 rounding of a sample position moves a texel coordinate by a few ulps of the coordinate
+# (relative 1e-7 to 1e-6) and a level of detail by about 1e-6 (log2 of an FP32 distance).
+TEXEL_FLIP = 4e-6
+LOD_FLIP = 1e-5
+
+
 def _compare(scene, pixel_tol_frac, label, sample=None, l2_tol=1e-3):
     """Three evaluations of the same pixels against the FP64 oracle:
     1. free-running: the oracle takes its own early-exit decisions (T < 0.01);
@@ -97,10 +106,14 @@ def _compare(scene, pixel_tol_frac, label, sample=None, l2_tol=1e-3):
        are compared under identical decisions;
     3. the oracle's FP32 instantiation under the same replayed decisions, its RGBA rounded
        to half as the GPU's output is: what FP32 rounding alone does to this scene.
-    Asserted: step-count disagreements sit at integer boundaries; pixels over 1e-3 that no
-    early-exit flip explains number at most as many as the FP32 evaluation has (x1.5, plus 3
-    pixels) and below `pixel_tol_frac`; the replayed relative L2 errors are below `l2_tol` or
-    within 2x of the FP32 evaluation's; the free-running L2 errors are below 5 x `l2_tol`."""
+    The oracle also records each pixel's decision margins: how close any NEAREST texel choice
+    or level-of-detail switch along its march came to flipping.
+    Asserted: step-count disagreements sit at integer boundaries; early-exit flips (beyond 1e-3 of
+    T = 0.01) and the pixels over 1e-3 that neither an exit flip nor a fetch near a boundary
+    explains are each fewer than `pixel_tol_frac` (about twice the rate measured on MI355X); the
+    relative L2 error over the pixels with no decision near a boundary is below 1e-3; the
+    free-running relative L2 error is below `l2_tol`.  The FP32 evaluation is printed beside
+    them: a plain FP32 transcription (no cancellation-free forms) is about 100x further off."""
     counts = scene.count_work()
     rgba, dist = scene.image()
     packed = scene.steps_per_pixel()
@@ -112,8 +125,9 @@ def _compare(scene, pixel_tol_frac, label, sample=None, l2_tol=1e-3):
     try:
         atm, cb, cl = scene.atmosphere.buffer, scene.common, scene.cloud_block
         free, _, free_steps = oracle.cloud_raymarch(atm, cb, cl, inputs, w, px)
-        ref, ref_dist, _, margin, step_margin = oracle.cloud_raymarch_forced(atm, cb, cl, inputs, w, px, packed[px])
-        f32, _, _, _, _ = oracle.cloud_raymarch_forced(atm, cb, cl, inputs, w, px, packed[px], precision="f32")
+        ref, ref_dist, _, margin, step_margin, texel_m, lod_m = oracle.cloud_raymarch_forced(atm, cb, cl, inputs, w,
+                                                                                             px, packed[px])
+        f32 = oracle.cloud_raymarch_forced(atm, cb, cl, inputs, w, px, packed[px], precision="f32")[0]
     finally:
         oracle.bind_cloud_phase(None)
     f32 = f32.astype(np.float16).astype(np.float64)
@@ -122,8 +136,12 @@ def _compare(scene, pixel_tol_frac, label, sample=None, l2_tol=1e-3):
     l2_rgb32, l2_t32, per32 = _metrics(f32, ref)
     d_err = np.abs(dist.reshape(-1)[px] - ref_dist) / np.maximum(ref_dist, 1e-3)
     exit_flips = margin > 1e-3
+    # A NEAREST fetch whose texel coordinate lies within TEXEL_FLIP (relative) of an integer, or
+    # whose level of detail lies within LOD_FLIP of a switch point, may choose differently in FP32.
+    fetch_flips = (texel_m < TEXEL_FLIP) | (lod_m < LOD_FLIP)
     out_gpu = (per > 1e-3) & ~exit_flips
     out_f32 = (per32 > 1e-3) & ~exit_flips
+    unexplained = out_gpu & ~fetch_flips
     step_flips = int(np.sum(step_margin > 0))
     print(f"{label}: {len(px)} pixels; free-running rel L2 rgb {l2f_rgb:.2e} T {l2f_t:.2e}, pixels > 1e-3 "
           f"{np.mean(perf > 1e-3):.2e}; step-count decisions differing: {step_flips} pixels, max |x - round(x)| "
@@ -135,17 +153,36 @@ def _compare(scene, pixel_tol_frac, label, sample=None, l2_tol=1e-3):
     print(f"   replayed FP32 oracle: rel L2 rgb {l2_rgb32:.2e} T {l2_t32:.2e}, pixels > 1e-3 without an exit flip "
           f"{int(out_f32.sum())} ({out_f32.mean():.2e}), max {per32.max():.2e}; both over 1e-3: "
           f"{int(np.sum(out_gpu & out_f32))}")
+    keep = ~(exit_flips | fetch_flips)
+    l2k_rgb, l2k_t, _ = _metrics(g[keep], ref[keep])
+    print(f"   fetch decisions near a boundary: {fetch_flips.mean():.3f} of pixels (texel {np.mean(texel_m < TEXEL_FLIP):.3f}"
+          f", lod {np.mean(lod_m < LOD_FLIP):.3f}); outliers without an exit flip: {int(out_gpu.sum())}, of which near "
+          f"a fetch boundary {int((out_gpu & fetch_flips).sum())}, unexplained {int(unexplained.sum())}; rel L2 of the "
+          f"other pixels rgb {l2k_rgb:.2e} T {l2k_t:.2e}")
+    if out_gpu.any():
+        print(f"   outliers' smallest margins: texel {np.sort(texel_m[out_gpu])[:8]}, lod {np.sort(lod_m[out_gpu])[:8]}")
+    if unexplained.any():
+        u = np.where(unexplained)[0]
+        tq = np.quantile(ref[u, 3], [0, 0.1, 0.5, 0.9, 1])
+        dq = np.quantile(ref_dist[u], [0, 0.1, 0.5, 0.9, 1])
+        cond = ref[u, 3] / np.maximum(1 - ref[u, 3], 1e-9)
+        print(f"   unexplained outliers: T quantiles {np.round(tq, 4)}, distance quantiles {np.round(dq, 1)} km, "
+              f"T > 0.9: {int(np.sum(ref[u, 3] > 0.9))}, errors {np.round(np.quantile(per[u], [0.5, 0.9, 1]), 5)}, "
+              f"T rel err median {np.median(np.abs(g[u, 3] - ref[u, 3]) / ref[u, 3]):.2e}, cond median "
+              f"{np.median(cond):.1f}; all pixels: T > 0.9 {np.mean(ref[:, 3] > 0.9):.3f}")
     worst = np.argsort(-per)[:5]
     for i in worst:
-        print(f"   pixel {px[i] % w},{px[i] // w}: gpu {g[i]} oracle {ref[i]} fp32 {f32[i]} steps {packed[px[i]]}")
+        print(f"   pixel {px[i] % w},{px[i] // w}: gpu {g[i]} oracle {ref[i]} fp32 {f32[i]} steps {packed[px[i]]} "
+              f"texel margin {texel_m[i]:.2e} lod margin {lod_m[i]:.2e}")
     # Step counts depend on geometry only: any disagreement must sit at an integer boundary.
     assert step_margin.max() < 1e-3
     assert np.mean(exit_flips) < pixel_tol_frac
-    # Outliers that no early-exit flip explains: the texel and mip choices of NEAREST_MIPMAP_NEAREST
-    # fetches flip where FP32 and FP64 coordinates straddle a boundary, as they do for the FP32 oracle.
-    assert out_gpu.sum() <= 1.5 * out_f32.sum() + 3 and out_gpu.mean() < pixel_tol_frac
-    assert (l2_rgb < l2_tol or l2_rgb < 2.0 * l2_rgb32) and (l2_t < l2_tol or l2_t < 2.0 * l2_t32)
-    assert l2f_rgb < 5 * l2_tol and l2f_t < 5 * l2_tol
+    # The outliers left after exit flips and fetch flips are opaque pixels tens of km away (FP32
+    # texel coordinates of ~2000 texels resolve the filter weights to ~2.4e-4, which moves density
+    # on steep erosion edges) and nearly transparent pixels (luminance scales with 1 - T).
+    assert unexplained.mean() < pixel_tol_frac
+    assert l2k_rgb < 1e-3 and l2k_t < 1e-3
+    assert l2f_rgb < l2_tol and l2f_t < l2_tol
     return counts
 
 
@@ -153,13 +190,13 @@ def test_raymarch_benchmark_workload_parity(ctx):
     """The 1080p workload's scene (weather map, detail volume, LOD rule, 64 steps, 6 light
     steps) at 480x270, every pixel."""
     scene = CloudScene(ctx, 480, 270)
-    _compare(scene, 1e-2, "Material1 480x270 (NEAREST mips)")
+    _compare(scene, 5e-4, "Material1 480x270 (NEAREST mips)")
 
 
 def test_raymarch_1080p_sample_parity(ctx):
     scene = CloudScene(ctx, 1920, 1080)
     sample = np.random.default_rng(5).choice(1920 * 1080, 20000, replace=False)
-    _compare(scene, 1e-2, "Material1 1920x1080 (20000-pixel sample)", sample=sample)
+    _compare(scene, 8e-3, "Material1 1920x1080 (20000-pixel sample)", sample=sample)
 
 
 def test_worley_volume_matches_oracle(ctx):
@@ -179,13 +216,13 @@ def test_raymarch_config2_parity(ctx):
     """The cloud benchmark as stated: Material1 plus the 32^3 Worley detail volume, dual-lobe HG with
     g = 0.8 / -0.3, the seeded 128^2 blue-noise tile, at 480x270, every pixel."""
     scene = CloudScene(ctx, 480, 270, workload=presets.cloud_workload_config2())
-    _compare(scene, 1e-2, "config 2 (Worley 32^3, HG 0.8/-0.3) 480x270")
+    _compare(scene, 5e-4, "config 2 (Worley 32^3, HG 0.8/-0.3) 480x270")
 
 
 def test_raymarch_config2_1080p_sample_parity(ctx):
     scene = CloudScene(ctx, 1920, 1080, workload=presets.cloud_workload_config2())
     sample = np.random.default_rng(6).choice(1920 * 1080, 20000, replace=False)
-    _compare(scene, 1e-2, "config 2 (Worley 32^3, HG 0.8/-0.3) 1920x1080 (20000-pixel sample)", sample=sample)
+    _compare(scene, 8e-3, "config 2 (Worley 32^3, HG 0.8/-0.3) 1920x1080 (20000-pixel sample)", sample=sample)
 
 
 def test_raymarch_config2_phase_is_bound(ctx):
@@ -216,19 +253,19 @@ def test_raymarch_continuous_fetches_parity(ctx):
     w = presets.cloud_workload()
     w["camera"] = presets.bench_camera(pitch_deg=30.0)
     scene = CloudScene(ctx, 320, 180, workload=w, lod_bias=-30.0)
-    _compare(scene, 1e-3, "Material1 320x180 magnification only")
+    _compare(scene, 3e-4, "Material1 320x180 magnification only")
 
 
 def test_raymarch_noise_cloud_map_and_reference_blue_noise(ctx, blue_noise_64):
     """The reference's own texture pipeline: the NoiseGen cloud map instead of the uniform
     weather map, and the 64x64 blue-noise tile of data/BlueNoise (pos & 0x3f)."""
     scene = CloudScene(ctx, 320, 180, weather="noise", blue_noise=blue_noise_64, frame_id=7)
-    _compare(scene, 1e-2, "Material1 NoiseGen cloud map, blue-noise fixture, frame 7", l2_tol=1e-2)
+    _compare(scene, 5e-4, "Material1 NoiseGen cloud map, blue-noise fixture, frame 7", l2_tol=1e-2)
 
 
 def test_raymarch_minimal_material_parity(ctx):
     scene = CloudScene(ctx, 320, 180, material=VolumetricCloudMinimalMaterial(density=3.0))
-    _compare(scene, 1e-3, "Minimal material")
+    _compare(scene, 1e-4, "Minimal material")
 
 
 def test_raymarch_rejects_bad_input(ctx):
@@ -301,7 +338,7 @@ def test_raymarch_with_shadow_froxel_parity(ctx):
     oracle.bind_shadow_froxel(froxel)
     try:
         print(f"froxel {froxel.shape}: visibility {froxel.min() / 65535:.3f}-{froxel.max() / 65535:.3f}")
-        _compare(scene, 1e-2, "Material1 480x270, shadow froxel bound")
+        _compare(scene, 5e-4, "Material1 480x270, shadow froxel bound")
         bound, _ = scene.image()
     finally:
         bind_shadow_froxel(ctx, None)
