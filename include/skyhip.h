@@ -993,6 +993,17 @@ size_t skyhip_surrogate_workspace_bytes(int32_t count);
 int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, const float* params, const float* features,
                               const float* targets, const float* weights, int32_t count, float loss_scale,
                               float* out_grad, float* out_loss, void* workspace, size_t workspace_bytes);
+/* skyhip_surrogate_gradient without a workspace argument and without allocating: for callers that
+ * cannot allocate (code under HIP graph capture, tight training loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  The partial gradients live in a fixed device array, one
+ * per GPU, resolved when a context is created on that GPU and held until the process ends
+ * (13.5 MB); count must be at most SKYHIP_SURROGATE_NOALLOC_MAX_COUNT (larger counts fail with
+ * SKYHIP_ERR_INVALID_SIZE; use skyhip_surrogate_gradient).  All calls on one GPU share the array:
+ * calls that overlap on one GPU (on different streams or from different threads) race. */
+#define SKYHIP_SURROGATE_NOALLOC_MAX_COUNT 8192
+int skyhip_surrogate_gradient_noalloc(SkyhipContext* ctx, int32_t network, const float* params,
+                                      const float* features, const float* targets, const float* weights,
+                                      int32_t count, float loss_scale, float* out_grad, float* out_loss);
 /* One Adam step (step counts from 1) on device params / moments. */
 int skyhip_surrogate_adam(SkyhipContext* ctx, float* params, const float* grad, float* m, float* v, float lr,
                           float beta1, float beta2, float eps, int32_t step);

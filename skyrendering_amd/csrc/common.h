@@ -64,6 +64,8 @@ struct SkyhipContext {
 namespace skyhip {
 
 int check_ctx(SkyhipContext* ctx);
+// Resolves `device`'s copy of the allocation-free surrogate gradient's workspace (surrogate.hip).
+int surrogate_scratch_setup(int device);
 
 // ---------------------------------------------------------------------------
 // Device vector helpers (float3 arithmetic; HIP's float3 is a plain struct).

@@ -214,6 +214,12 @@ extern "C" int skyhip_create(int device, SkyhipContext** out_ctx) {
             return check_hip(e, "hipEventCreate");
         }
     }
+    // This device's copy of the allocation-free gradient's workspace (with the device current).
+    const int rc = skyhip::surrogate_scratch_setup(device);
+    if (rc != SKYHIP_OK) {
+        skyhip_destroy(ctx);
+        return rc;
+    }
     *out_ctx = ctx;
     return SKYHIP_OK;
 }

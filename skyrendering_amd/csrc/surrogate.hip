@@ -31,6 +31,7 @@
 //    check against the plain integrator at probe pixels, and the plain fallback for
 //    tiles whose residual exceeds the bound.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "lut_texels.h"
@@ -831,18 +832,14 @@ extern "C" size_t skyhip_surrogate_workspace_bytes(int32_t count) {
     return groups * kParams * sizeof(float) + groups * sizeof(float);
 }
 
-extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, const float* params,
-                                         const float* features, const float* targets, const float* weights,
-                                         int32_t count, float loss_scale, float* out_grad, float* out_loss,
-                                         void* workspace, size_t workspace_bytes) {
-    SKYHIP_CHECK(check_ctx(ctx));
-    SKYHIP_CHECK(check_network(network));
-    if (!params || !features || !targets || !out_grad || !workspace)
-        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (count < 1 || count > (1 << 26)) return fail(SKYHIP_ERR_INVALID_SIZE, "count outside [1, 2^26]");
-    if (workspace_bytes < skyhip_surrogate_workspace_bytes(count))
-        return fail(SKYHIP_ERR_INVALID_SIZE, "workspace smaller than skyhip_surrogate_workspace_bytes(count)");
-    if (!std::isfinite(loss_scale)) return fail(SKYHIP_ERR_NON_FINITE, "non-finite loss scale");
+namespace skyhip {
+
+// Launches the training kernels for `count` samples with the per-workgroup partial gradients
+// (div_up(count, kTS) x kParams floats, then one loss per workgroup) at `partial`; every
+// workgroup writes its whole slice, so the buffer needs no clearing.
+static int gradient_launch(SkyhipContext* ctx, int32_t network, const float* params, const float* features,
+                           const float* targets, const float* weights, int32_t count, float loss_scale,
+                           float* out_grad, float* out_loss, float* partial) {
     // The training kernels' LDS footprint exceeds the 64 KiB default (both element types are
     // two bytes, so both layouts have the same size).
     static_assert(sizeof(TrainLds<_Float16>) == sizeof(TrainLds<__bf16>), "LDS layouts differ");
@@ -856,7 +853,6 @@ extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, co
         lds_ready = true;
     }
     const int groups = div_up(count, kTS);
-    float* partial = reinterpret_cast<float*>(workspace);
     float* partial_loss = partial + (size_t)groups * kParams;
     if (network == SKYHIP_SURROGATE_F16_GATE)
         hipLaunchKernelGGL(surrogate_train_kernel<NetF16Gate>, dim3(groups), dim3(256), lds_bytes, ctx->stream,
@@ -869,6 +865,77 @@ extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, co
                        partial_loss, groups, out_grad, out_loss);
     SKYHIP_LAUNCH_CHECK("surrogate_reduce_kernel");
     return SKYHIP_OK;
+}
+
+static int check_gradient_args(SkyhipContext* ctx, int32_t network, const float* params, const float* features,
+                               const float* targets, const float* out_grad, int32_t count, float loss_scale) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    SKYHIP_CHECK(check_network(network));
+    if (!params || !features || !targets || !out_grad) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count < 1 || count > (1 << 26)) return fail(SKYHIP_ERR_INVALID_SIZE, "count outside [1, 2^26]");
+    if (!std::isfinite(loss_scale)) return fail(SKYHIP_ERR_NON_FINITE, "non-finite loss scale");
+    return SKYHIP_OK;
+}
+
+// The allocation-free gradient's partial gradients: one copy of this array per GPU, held until the
+// process ends (13.5 MB: SKYHIP_SURROGATE_NOALLOC_MAX_COUNT / 32 workgroups x (13,187 + 1) floats).
+// Host code never uses the array's name as an address: surrogate_scratch_setup resolves each
+// device's copy with hipGetSymbolAddress when a context is created on it.
+constexpr int kNoAllocMaxGroups = SKYHIP_SURROGATE_NOALLOC_MAX_COUNT / kTS;
+static_assert(SKYHIP_SURROGATE_NOALLOC_MAX_COUNT % kTS == 0, "whole workgroups");
+static __device__ __align__(256) float scratch[size_t(kNoAllocMaxGroups) * (kParams + 1)];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static float* g_scratch[kMaxScratchDevices] = {};
+
+// Called by skyhip_create with `device` current: resolves that device's copy of `scratch` once.
+int surrogate_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "device %d outside [0, %d)", device, kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return SKYHIP_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    if (ptr == nullptr) return fail(SKYHIP_ERR_DEVICE, "hipGetSymbolAddress(scratch) gave NULL");
+    g_scratch[device] = static_cast<float*>(ptr);
+    return SKYHIP_OK;
+}
+
+static float* scratch_for(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch[device];
+}
+
+}  // namespace skyhip
+
+extern "C" int skyhip_surrogate_gradient(SkyhipContext* ctx, int32_t network, const float* params,
+                                         const float* features, const float* targets, const float* weights,
+                                         int32_t count, float loss_scale, float* out_grad, float* out_loss,
+                                         void* workspace, size_t workspace_bytes) {
+    SKYHIP_CHECK(check_gradient_args(ctx, network, params, features, targets, out_grad, count, loss_scale));
+    if (!workspace) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (workspace_bytes < skyhip_surrogate_workspace_bytes(count))
+        return fail(SKYHIP_ERR_INVALID_SIZE, "workspace smaller than skyhip_surrogate_workspace_bytes(count)");
+    return gradient_launch(ctx, network, params, features, targets, weights, count, loss_scale, out_grad, out_loss,
+                           reinterpret_cast<float*>(workspace));
+}
+
+extern "C" int skyhip_surrogate_gradient_noalloc(SkyhipContext* ctx, int32_t network, const float* params,
+                                                 const float* features, const float* targets, const float* weights,
+                                                 int32_t count, float loss_scale, float* out_grad,
+                                                 float* out_loss) {
+    SKYHIP_CHECK(check_gradient_args(ctx, network, params, features, targets, out_grad, count, loss_scale));
+    if (count > SKYHIP_SURROGATE_NOALLOC_MAX_COUNT)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "count %d above the allocation-free maximum %d", count,
+                    SKYHIP_SURROGATE_NOALLOC_MAX_COUNT);
+    float* partial = scratch_for(ctx->device);
+    if (partial == nullptr)
+        return fail(SKYHIP_ERR_DEVICE, "the allocation-free workspace of device %d was not resolved", ctx->device);
+    return gradient_launch(ctx, network, params, features, targets, weights, count, loss_scale, out_grad, out_loss,
+                           partial);
 }
 
 extern "C" int skyhip_surrogate_adam(SkyhipContext* ctx, float* params, const float* grad, float* m, float* v,
