@@ -38,13 +38,19 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 # FP32 operations the kernels execute, measured with rocprofv3's SQ counters on this bench's
-# workloads (profiles/cloud_raymarch_1080p.md, profiles/earth_lut_set.md): per wave instruction
-# SQ_INSTS_VALU_{ADD,MUL,TRANS}_F32 count 64 lane operations and SQ_INSTS_VALU_FMA_F32 128, scaled
-# by the measured lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), then divided
-# by the executed sample-steps (raymarch) or integrator steps (LUT kernels) of the same launches.
+# workloads (profiles/cloud_raymarch_1080p.md, profiles/earth_lut_set.md,
+# profiles/lut_batch_256x8.md): SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP32_TRANS count
+# operations per wave instruction (an FMA counts 2), so the lane operations are 64 times that,
+# scaled by the measured lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), then
+# divided by the executed sample-steps (raymarch: 8.246e9 over 35,111,907) or integrator steps
+# (LUT set: 1.0386e9 over 4,914,688; batch: 7.183e11 over 3,024,532,480) of the same launches.
 RAYMARCH_KERNEL = "cloud_raymarch_kernel<5,false>"
-RAYMARCH_FP32_FLOP_PER_STEP = 150.0
-FLOP_PER_INTEGRATOR_STEP = 150.0
+RAYMARCH_FP32_FLOP_PER_STEP = 234.8
+LUT_SET_FLOP_PER_STEP = 211.3
+BATCH_FLOP_PER_STEP = 237.5
+# VALU issue: the raymarch issues 3.874e8 wave instructions per frame, 20 % of the 1024 SIMDs'
+# issue slots at 2.4 GHz over its 0.779 ms, at 62 % lane utilisation: the kernel waits on its
+# dependent texture fetches more than it computes.
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
@@ -55,8 +61,8 @@ FLOP_PER_INTEGRATOR_STEP = 150.0
 FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
-    "cloud_raymarch_kernel<5,false>": (8141.5, 27958.8),
-    "path_trace_kernel<4,true,false>": (6.9093e6, 2.0778e5),
+    "cloud_raymarch_kernel<5,false>": (7527.9, 24313.3),
+    "path_trace_kernel<4,true,false>": (6.6085e6, 1.9962e5),
 }
 
 
@@ -129,14 +135,14 @@ def lut_set_section(ctx, rounds, cpu_seconds):
     path (OpenMP over texels) building whole sets on the host cores."""
     from skyrendering_amd import presets
     ms, steps = bench_lut_set(ctx, rounds)
-    flop = steps * FLOP_PER_INTEGRATOR_STEP
+    flop = steps * LUT_SET_FLOP_PER_STEP
     achieved = flop / (ms * 1e-3) / 1e12
     out = {"workload": "Earth LUT set: transmittance 256x64 (40 steps), multiple scattering 32x32 x 64 directions "
                        "(30), sky view 192x108 (30), aerial perspective 32x32x32 (50.867, 203 km); 16 sun "
                        "zeniths 0-95 deg; FP32 (FP64 optical depth for the near-horizon transmittance columns)",
            "ms_per_set": round(ms, 4), "integrator_steps": steps,
            "gsteps_per_s": round(steps / (ms * 1e-3) / 1e9, 2),
-           "roofline": {"bound": "valu", "unit": "TFLOP/s", "flop_per_step": FLOP_PER_INTEGRATOR_STEP,
+           "roofline": {"bound": "valu", "unit": "TFLOP/s", "flop_per_step": LUT_SET_FLOP_PER_STEP,
                         "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                         "note": "latency-bound: the set is 4 dependent launches (T -> MS -> SV || AP) of "
@@ -449,10 +455,10 @@ def bench_lut_batch(ctx, launches=5, autotune_reps=2):
         "integrator_steps": steps, "gsteps_per_s": round(steps / (ms * 1e-3) / 1e9, 2),
         "tuning": tuned, "tuned_pass_sum_ms": round(tuned_pass_ms, 3),
         "roofline": {"bound": "valu", "roof": "fp32 vector (VALU) peak", "unit": "TFLOP/s",
-                     "flop_per_step": FLOP_PER_INTEGRATOR_STEP,
-                     "achieved": round(steps * FLOP_PER_INTEGRATOR_STEP / (ms * 1e-3) / 1e12, 3),
+                     "flop_per_step": BATCH_FLOP_PER_STEP,
+                     "achieved": round(steps * BATCH_FLOP_PER_STEP / (ms * 1e-3) / 1e12, 3),
                      "peak": FP32_PEAK_TFLOPS,
-                     "frac": round(steps * FLOP_PER_INTEGRATOR_STEP / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+                     "frac": round(steps * BATCH_FLOP_PER_STEP / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
     }
 
 
@@ -720,8 +726,9 @@ def main():
                                "elevation 20 deg",
                    "width": 1920, "height": 1080, "executed_steps_per_frame": steps,
                    "density_evals_per_frame": counts["evals"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "roof": "FP32 vector peak; the kernel is VALU-issue bound (textures are "
-                                              "L2-resident, HBM traffic is its inputs once and outputs once)",
+        "roofline": {"bound": "valu", "roof": "FP32 vector peak (textures are L2-resident; HBM traffic is the "
+                                              "inputs once and the outputs once); measured VALU issue 20 % of "
+                                              "slots at 62 % lane utilisation: fetch-latency limited",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                      "traffic_gbs": round(traffic / (ms * 1e-3) / 1e9, 1), "kernel": RAYMARCH_KERNEL,

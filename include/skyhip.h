@@ -486,8 +486,10 @@ int skyhip_aerial_perspective_lut(SkyhipContext* ctx, const SkyhipAtmosphereBuff
 /* One frame's atmosphere LUT set, in the reference's order: Atmosphere::UpdateLuts
  * (transmittance then multiple scattering, Atmosphere.cpp:101-124) followed by the LUT
  * passes of AtmosphereRenderer::Render (sky view and aerial perspective,
- * AtmosphereRenderer.cpp:221-233), which run concurrently on a forked stream and are
- * joined back before the call's work completes.  Dithering off (the reference default). */
+ * AtmosphereRenderer.cpp:221-233), the last two as one launch.  Built for latency: each
+ * texel's march is split over several lanes and the parts combined in order, so the FP32
+ * results agree with the per-pass entry points to rounding, not bit for bit.  Three kernel
+ * launches on the context's stream.  Dithering off (the reference default). */
 int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
                               const SkyhipAtmosphereRenderBufferData* render, float* transmittance_rgba,
                               int32_t transmittance_width, int32_t transmittance_height,
@@ -764,7 +766,11 @@ typedef struct SkyhipPathTracingParameters {
  *                    float[ceil(dim/B)^3] output of skyhip_voxel_grid_majorants_sized for a
  *                    VOXEL_F16 material (scaled by the voxel block's uDensity on use);
  *   brick_size:      the brick edge B those majorants were built with: 0 for the default
- *                    (skyhip_voxel_grid_brick_size, 16), else a power of two in [4, 64]. */
+ *                    (skyhip_voxel_grid_brick_size, 16), else a power of two in [4, 64];
+ *   cell_corners:    NULL, or the skyhip_voxel_grid_cells copy of the material's grid: the
+ *                    brick-majorant tracker then reads each density lookup's eight texels with
+ *                    one 16-byte load instead of eight 2-byte ones.  The same texels and the
+ *                    same arithmetic: results are identical with and without it. */
 typedef struct SkyhipPathTracingInputs {
     const float* transmittance;
     int32_t transmittance_width;
@@ -780,6 +786,7 @@ typedef struct SkyhipPathTracingInputs {
     int32_t ap_depth;
     int32_t brick_size;
     const float* brick_majorants;
+    const uint16_t* cell_corners;
 } SkyhipPathTracingInputs;
 
 /* The synthetic voxel-cloud density of the path-tracing workload (no reference
@@ -816,6 +823,15 @@ int skyhip_voxel_grid_majorants_sized(SkyhipContext* ctx, const uint16_t* grid, 
 /* skyhip_voxel_grid_majorants_sized with the default brick edge. */
 int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
                                 int32_t dim_z, float* out_majorants);
+/* Cell-corner copy of an FP16 grid, for trilinear lookups with one load (no reference
+ * counterpart: a device data layout of the path tracer).  Cell (i, j, k), i in [-1, dim_x - 1]
+ * (likewise j, k), holds the eight texels (i + a, j + b, k + c), a, b, c in {0, 1}, in the
+ * order a + 2b + 4c, with +0 for texels outside the grid (CLAMP_TO_BORDER(0)); cells are
+ * stored x fastest from (-1, -1, -1).  out_cells: device uint16_t[8 * (dim_x + 1) *
+ * (dim_y + 1) * (dim_z + 1)], 16-byte aligned (4.2 x the grid: 2.16 GB for 512^3); at most
+ * 2^31 cells. */
+int skyhip_voxel_grid_cells(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y, int32_t dim_z,
+                            uint16_t* out_cells);
 
 /* One or more frames of VolumetricCloud::PathTracing::Render's trace pass
  * (src/SkyRendering/VolumetricCloud.cpp:537-566, VolumetricCloudPathTracing.comp:255-284):

@@ -215,7 +215,7 @@ class PathTracingInputsC(C.Structure):
         ("transmittance", vp), ("transmittance_width", i32), ("transmittance_height", i32),
         ("sky_view_luminance", vp), ("sky_view_transmittance", vp), ("sky_view_width", i32),
         ("sky_view_height", i32), ("ap_luminance", vp), ("ap_transmittance", vp), ("ap_width", i32),
-        ("ap_height", i32), ("ap_depth", i32), ("brick_size", i32), ("brick_majorants", vp),
+        ("ap_height", i32), ("ap_depth", i32), ("brick_size", i32), ("brick_majorants", vp), ("cell_corners", vp),
     ]
 
 
@@ -254,7 +254,7 @@ assert C.sizeof(CloudMaterialCommonBufferData) == 64
 assert C.sizeof(CloudMaterial1BufferData) == 32
 assert C.sizeof(CloudVoxelBufferData) == 32
 assert C.sizeof(PathTracingParametersC) == 76
-assert C.sizeof(PathTracingInputsC) == 80
+assert C.sizeof(PathTracingInputsC) == 88
 assert C.sizeof(VoxelGridCreateInfoC) == 48
 
 P = C.POINTER

@@ -84,6 +84,85 @@ __global__ __launch_bounds__(256) void aerial_perspective_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Latency-bound form of one frame's LUT set (skyhip_atmosphere_lut_set).  A single set has
+// too few texels to fill 256 CUs one lane per texel (the 256x64 transmittance LUT is one wave
+// per CU, the 192x108 sky view 1.3), so each texel's march is split over consecutive lanes
+// and the parts combined in order, and the sky-view and aerial-perspective passes, which read
+// only T and MS, run as one launch.  Same quantities as the per-pass kernels; the FP32 sums
+// associate differently (parts add in order), which is why the batched builder, which must
+// match the per-pass kernels bit for bit, does not use these.
+// ---------------------------------------------------------------------------
+constexpr int kSetTransmittanceParts = 4;
+constexpr int kSetSkyViewParts = 4;
+constexpr int kSetAerialParts = 8;
+
+template <class S, int kParts>
+__device__ inline F3 transmittance_texel_split(const SkyhipAtmosphereBufferData& b, int ix, int iy, int w, int h,
+                                               int part) {
+    const int n = transmittance_step_count(b);
+    const int c = (n + kParts - 1) / kParts, first = min(part * c, n);
+    S od[3], dx;
+    transmittance_od<S>(b, ix, iy, w, h, first, max(0, min(c, n - first)), od, dx);
+    S acc[3] = {od[0], od[1], od[2]};
+    for (int p = 1; p < kParts; ++p)
+        for (int k = 0; k < 3; ++k) acc[k] += __shfl_down(od[k], p, 64);
+    return transmittance_from_od<S>(b, acc, dx);
+}
+
+__global__ __launch_bounds__(256) void set_transmittance_kernel(SkyhipAtmosphereBufferData b,
+                                                                float4* __restrict__ out, int w, int h) {
+    constexpr int P = kSetTransmittanceParts;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int t = g / P, part = g % P;
+    if (t >= w * h) return;  // whole groups of P lanes leave together
+    const int ix = t % w, iy = t / w;
+    const F3 v = ix >= transmittance_fp64_column(w) ? transmittance_texel_split<double, P>(b, ix, iy, w, h, part)
+                                                    : transmittance_texel_split<float, P>(b, ix, iy, w, h, part);
+    if (part == 0) out[t] = make_float4(v.x, v.y, v.z, 1.0f);
+}
+
+// Blocks [0, sv_blocks) build the sky view, kSetSkyViewParts lanes per texel; the rest the
+// aerial perspective, kSetAerialParts lanes per froxel (no blue-noise dither, as the set's
+// per-pass calls).
+__global__ __launch_bounds__(256) void set_sky_aerial_kernel(
+    SkyhipAtmosphereBufferData b, SkyhipAtmosphereRenderBufferData rbd, const float4* __restrict__ T, int tw, int th,
+    const float4* __restrict__ MS, int mw, int mh, float4* __restrict__ sv_l, float4* __restrict__ sv_t, int sw,
+    int sh, int sv_blocks, float4* __restrict__ ap_l, float4* __restrict__ ap_t, int aw, int ah, int ad) {
+    const AtmDev A = make_atm(b);
+    const RenderDev R = make_render(rbd);
+    F3 L, Tr;
+    if (int(blockIdx.x) < sv_blocks) {
+        constexpr int P = kSetSkyViewParts;
+        const int g = blockIdx.x * 256 + threadIdx.x;
+        const int part = g % P;
+        const bool valid = g / P < sw * sh;
+        const int t = valid ? g / P : 0;  // every lane of a group reaches the shuffles
+        const SkyViewRay ray = sky_view_ray(A, R, nullptr, 1, t % sw, t / sw, sw, sh);
+        scattered_luminance_split<P>(A, T, tw, th, MS, mw, mh, ray.s, ray.dist, R.sv_steps, ray.start_i,
+                                     ray.dist > 0.0f, part, L, Tr);
+        if (part == 0 && valid) {
+            sv_l[t] = make_float4(L.x, L.y, L.z, 0.0f);
+            sv_t[t] = make_float4(Tr.x, Tr.y, Tr.z, 0.0f);
+        }
+        return;
+    }
+    constexpr int P = kSetAerialParts;
+    const long long g = (long long)(blockIdx.x - sv_blocks) * 256 + threadIdx.x;
+    const int n_texels = aw * ah * ad;
+    const int part = int(g % P);
+    const bool valid = g / P < n_texels;
+    const int t = valid ? int(g / P) : 0;
+    const AerialRay ray = aerial_ray(A, R, rbd.inv_view_projection, nullptr, 1, t % aw, (t / aw) % ah, t / (aw * ah),
+                                     aw, ah, ad);
+    scattered_luminance_split<P>(A, T, tw, th, MS, mw, mh, ray.s, ray.marching_distance, R.ap_steps, ray.start_i,
+                                 ray.marching_distance > 0.0f, part, L, Tr);
+    if (part == 0 && valid) {
+        ap_l[t] = make_float4(L.x, L.y, L.z, 0.0f);
+        ap_t[t] = make_float4(Tr.x, Tr.y, Tr.z, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Validation and launch helpers
 // ---------------------------------------------------------------------------
 constexpr int kMaxLutDim = 8192;
@@ -265,18 +344,19 @@ extern "C" int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosph
     SKYHIP_CHECK(validate_lut(ap_transmittance_rgba, ap_width, ap_height, "aerial-perspective transmittance"));
     if (ap_depth < 2 || ap_depth > 1024) return fail(SKYHIP_ERR_INVALID_SIZE, "aerial-perspective depth outside [2, 1024]");
     hipStream_t st = ctx->stream;
-    SKYHIP_CHECK(launch_transmittance(st, *atmosphere, transmittance_rgba, tw, th));
+    hipLaunchKernelGGL(set_transmittance_kernel, dim3(div_up((long long)tw * th * kSetTransmittanceParts, 256)),
+                       dim3(256), 0, st, *atmosphere, reinterpret_cast<float4*>(transmittance_rgba), tw, th);
+    SKYHIP_LAUNCH_CHECK("set_transmittance_kernel");
     SKYHIP_CHECK(launch_multiscattering(st, *atmosphere, transmittance_rgba, tw, th, multiscattering_rgba, mw, mh));
-    // Sky view and aerial perspective both read T and MS only: run them side by side.
-    SKYHIP_HIP(hipEventRecord(ctx->fork_event, st));
-    SKYHIP_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_event, 0));
-    SKYHIP_CHECK(launch_aerial_perspective(ctx->aux_stream, *atmosphere, *render, transmittance_rgba, tw, th,
-                                           multiscattering_rgba, mw, mh, nullptr, 1, ap_luminance_rgba,
-                                           ap_transmittance_rgba, ap_width, ap_height, ap_depth));
-    SKYHIP_CHECK(launch_sky_view(st, *atmosphere, *render, transmittance_rgba, tw, th, multiscattering_rgba, mw, mh,
-                                 nullptr, 1, sky_view_luminance_rgba, sky_view_transmittance_rgba, sky_view_width,
-                                 sky_view_height));
-    SKYHIP_HIP(hipEventRecord(ctx->join_event, ctx->aux_stream));
-    SKYHIP_HIP(hipStreamWaitEvent(st, ctx->join_event, 0));
+    const int sv_blocks = int(div_up((long long)sky_view_width * sky_view_height * kSetSkyViewParts, 256));
+    const int ap_blocks = int(div_up((long long)ap_width * ap_height * ap_depth * kSetAerialParts, 256));
+    hipLaunchKernelGGL(set_sky_aerial_kernel, dim3(sv_blocks + ap_blocks), dim3(256), 0, st, *atmosphere, *render,
+                       reinterpret_cast<const float4*>(transmittance_rgba), tw, th,
+                       reinterpret_cast<const float4*>(multiscattering_rgba), mw, mh,
+                       reinterpret_cast<float4*>(sky_view_luminance_rgba),
+                       reinterpret_cast<float4*>(sky_view_transmittance_rgba), sky_view_width, sky_view_height,
+                       sv_blocks, reinterpret_cast<float4*>(ap_luminance_rgba),
+                       reinterpret_cast<float4*>(ap_transmittance_rgba), ap_width, ap_height, ap_depth);
+    SKYHIP_LAUNCH_CHECK("set_sky_aerial_kernel");
     return SKYHIP_OK;
 }

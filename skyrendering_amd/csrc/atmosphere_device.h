@@ -18,6 +18,11 @@
 
 #include "common.h"
 
+// Steps of the scattering integrator unrolled per loop iteration.
+#ifndef SKYHIP_INTEGRATOR_UNROLL
+#define SKYHIP_INTEGRATOR_UNROLL 2
+#endif
+
 namespace skyhip {
 
 constexpr float kPiF = 3.14159265358979323846f;
@@ -238,8 +243,15 @@ __device__ inline void scattered_luminance(const AtmDev& A, const float4* __rest
     L = f3(0.0f);
     Tr = f3(1.0f);
     Lf = f3(0.0f);
-    for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f)
-        scatter_step<kMS>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
+    // The float counter's trip count first, so the step loop has an integer induction variable
+    // the compiler can unroll: the unrolled steps' LUT fetches depend only on the counter, so
+    // they issue ahead of the previous step's arithmetic.  The counter itself still advances by
+    // repeated += 1, as the shader's does.
+    float i = kMS ? 0.5f : start_i;
+    int n = 0;
+    for (float j = i; j < steps; j += 1.0f) ++n;
+#pragma unroll SKYHIP_INTEGRATOR_UNROLL
+    for (int k = 0; k < n; ++k, i += 1.0f) scatter_step<kMS>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
 }
 
 // `count` consecutive steps of the same float counter, starting at counter value i0, with
@@ -252,6 +264,7 @@ __device__ inline void scattered_luminance_part(const AtmDev& A, const float4* _
     Tr = f3(1.0f);
     F3 Lf = f3(0.0f);
     float i = i0;
+#pragma unroll SKYHIP_INTEGRATOR_UNROLL
     for (int k = 0; k < count; ++k, i += 1.0f) scatter_step<false>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
 }
 

@@ -143,10 +143,17 @@ struct GridF16 {
 
 __device__ __forceinline__ float half_bits_to_float(uint16_t h) { return float(__builtin_bit_cast(_Float16, h)); }
 
-// Trilinear sample at normalised (u, v, w); texel-space x = u * dim - 0.5 as GL.
-__device__ inline float grid_f16_trilinear(const GridF16& g, float u, float v, float w) {
+// Trilinear interpolation of a cell's corners c[a + 2b + 4c] at fractions (ax, ay, az): x, then
+// y, then z, each as a lerp.
+__device__ __forceinline__ float trilinear_corners(const float c[8], float ax, float ay, float az) {
+    const float x0 = lerpf(c[0], c[1], ax), x1 = lerpf(c[2], c[3], ax);
+    const float x2 = lerpf(c[4], c[5], ax), x3 = lerpf(c[6], c[7], ax);
+    return lerpf(lerpf(x0, x1, ay), lerpf(x2, x3, ay), az);
+}
+
+// Trilinear sample at texel-space (x, y, z) = (u, v, w) * dim - 0.5 (GL's texel coordinates).
+__device__ inline float grid_f16_trilinear_texel(const GridF16& g, float x, float y, float z) {
     const int nx = g.dim[0], ny = g.dim[1], nz = g.dim[2];
-    float x = u * float(nx) - 0.5f, y = v * float(ny) - 0.5f, z = w * float(nz) - 0.5f;
     int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
     float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
     float c[8];
@@ -168,10 +175,37 @@ __device__ inline float grid_f16_trilinear(const GridF16& g, float u, float v, f
             c[k] = inside ? half_bits_to_float(g.data[((long long)zz * ny + yy) * nx + xx]) : 0.0f;
         }
     }
-    float r = 0.0f;
-    for (int k = 0; k < 8; ++k)
-        r += ((k & 1) ? ax : 1.0f - ax) * ((k & 2) ? ay : 1.0f - ay) * ((k & 4) ? az : 1.0f - az) * c[k];
-    return r;
+    return trilinear_corners(c, ax, ay, az);
+}
+
+// The same sample from the grid's cell-corner copy (skyhip_voxel_grid_cells): cell (i, j, k),
+// i in [-1, nx - 1], holds the eight texels (i + a, j + b, k + c) as FP16 in the order
+// a + 2b + 4c, zeros outside the grid, cells x fastest from (-1, -1, -1).  One 16-byte load;
+// samples whose cell lies outside the copy are 0, as all their texels are border texels.
+__device__ __forceinline__ float grid_cells_trilinear_texel(const uint4* __restrict__ cells, const int dim[3], float x,
+                                                            float y, float z) {
+    const int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
+    const float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
+    const bool inside = ix >= -1 && iy >= -1 && iz >= -1 && ix < dim[0] && iy < dim[1] && iz < dim[2];
+    const uint32_t cell = inside ? (uint32_t(iz + 1) * uint32_t(dim[1] + 1) + uint32_t(iy + 1)) *
+                                           uint32_t(dim[0] + 1) + uint32_t(ix + 1)
+                                 : 0u;
+    const uint4 q = cells[cell];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    float c[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        c[2 * k] = half_bits_to_float(uint16_t(w[k] & 0xffffu));
+        c[2 * k + 1] = half_bits_to_float(uint16_t(w[k] >> 16));
+    }
+    const float r = trilinear_corners(c, ax, ay, az);
+    return inside ? r : 0.0f;
+}
+
+// Trilinear sample at normalised (u, v, w).
+__device__ inline float grid_f16_trilinear(const GridF16& g, float u, float v, float w) {
+    return grid_f16_trilinear_texel(g, u * float(g.dim[0]) - 0.5f, v * float(g.dim[1]) - 0.5f,
+                                    w * float(g.dim[2]) - 0.5f);
 }
 
 // Material parameters as the kernels see them.  lod_* = log2(k_lod) + uLodBias of each

@@ -17,9 +17,13 @@ __device__ inline double sexp(double x) { return exp(x); }
 __device__ inline float ssqrt(float x) { return sqrtf(x); }
 __device__ inline double ssqrt(double x) { return sqrt(x); }
 
-// One transmittance texel, evaluated entirely in scalar type S.
+// The density integrals (Rayleigh, Mie, ozone) over steps [first, first + count) of one
+// transmittance texel's march (the shader's float counter runs 0.5, 1.5, ...), evaluated
+// entirely in scalar type S, and the step length dx.  Extinction is linear in the integrals,
+// so parts of the march add.
 template <class S>
-__device__ inline F3 transmittance_texel(const SkyhipAtmosphereBufferData& b, int ix, int iy, int w, int h) {
+__device__ inline void transmittance_od(const SkyhipAtmosphereBufferData& b, int ix, int iy, int w, int h, int first,
+                                        int count, S od[3], S& dx_out) {
     const S rb = b.bottom_radius, rt = b.top_radius;
     const S thickness = rt - rb;
     const S H2 = thickness * (rt + rb);
@@ -46,8 +50,9 @@ __device__ inline F3 transmittance_texel(const SkyhipAtmosphereBufferData& b, in
     const S rb2 = rb * rb;
     const S inv_hr = b.inv_rayleigh_exponential_distribution, inv_hm = b.inv_mie_exponential_distribution;
     const S oc = b.ozone_center_altitude, inv_ow = b.inv_ozone_width;
-    S od_r = 0, od_m = 0, od_o = 0;  // density integrals (extinction is linear in them)
-    for (float i = 0.5f; i < steps; i += 1.0f) {
+    S od_r = 0, od_m = 0, od_o = 0;
+    float i = 0.5f + float(first);  // exact: the counter's values are k + 0.5 far below 2^23
+    for (int k = 0; k < count; ++k, i += 1.0f) {
         S di = S(i) * dx;
         S q = di * (di + two_r_mu) + rho2;  // r_i^2 - R_b^2
         S rd = ssqrt(q + rb2);
@@ -62,13 +67,36 @@ __device__ inline F3 transmittance_texel(const SkyhipAtmosphereBufferData& b, in
         od_m += em;
         od_o += eo;
     }
+    od[0] = od_r;
+    od[1] = od_m;
+    od[2] = od_o;
+    dx_out = dx;
+}
+
+// Trip count of the transmittance march's float counter (0.5, 1.5, ... < steps).
+__device__ inline int transmittance_step_count(const SkyhipAtmosphereBufferData& b) {
+    int n = 0;
+    for (float i = 0.5f; i < b.transmittance_steps; i += 1.0f) ++n;
+    return n;
+}
+
+template <class S>
+__device__ inline F3 transmittance_from_od(const SkyhipAtmosphereBufferData& b, const S od[3], S dx) {
     F3 out;
     for (int c = 0; c < 3; ++c) {
-        S od = (S(b.rayleigh_scattering[c]) * od_r + (S(b.mie_scattering[c]) + S(b.mie_absorption[c])) * od_m +
-                S(b.ozone_absorption[c]) * od_o) * dx;
-        (&out.x)[c] = float(sexp(-od));
+        S v = (S(b.rayleigh_scattering[c]) * od[0] + (S(b.mie_scattering[c]) + S(b.mie_absorption[c])) * od[1] +
+               S(b.ozone_absorption[c]) * od[2]) * dx;
+        (&out.x)[c] = float(sexp(-v));
     }
     return out;
+}
+
+// One transmittance texel, evaluated entirely in scalar type S.
+template <class S>
+__device__ inline F3 transmittance_texel(const SkyhipAtmosphereBufferData& b, int ix, int iy, int w, int h) {
+    S od[3], dx;
+    transmittance_od<S>(b, ix, iy, w, h, 0, transmittance_step_count(b), od, dx);
+    return transmittance_from_od<S>(b, od, dx);
 }
 
 // Texels with x >= this column (the near-horizon end of the x_mu axis, where the
@@ -165,11 +193,14 @@ __device__ inline void start_from_space(const AtmDev& A, RayStart& s, float& mar
     }
 }
 
-// One sky-view texel (:74-186): luminance and transmittance.
-__device__ inline void sky_view_texel(const AtmDev& A, const RenderDev& R, const float4* __restrict__ T, int tw,
-                                      int th, const float4* __restrict__ MS, int mw, int mh,
-                                      const uint16_t* __restrict__ blue_noise, int bn_size, int ix, int iy, int w,
-                                      int h, F3& L, F3& Tr) {
+// The ray of one sky-view texel (:74-186): its start, marching distance (0: nothing to
+// integrate) and the float counter's start value.
+struct SkyViewRay {
+    RayStart s;
+    float dist, start_i;
+};
+__device__ inline SkyViewRay sky_view_ray(const AtmDev& A, const RenderDev& R, const uint16_t* __restrict__ blue_noise,
+                                          int bn_size, int ix, int iy, int w, int h) {
     const float r = R.r;
     const float alt0 = r - A.rb;
     // GetHorizonDownAngleFromR (:74-78) and GetCosLatLonFromSkyViewTextureIndex (:81-102).
@@ -199,18 +230,25 @@ __device__ inline void sky_view_texel(const AtmDev& A, const RenderDev& R, const
     const float sin_lon = clamp01(2.0f * x_lon * sqrtf(fmaxf(1.0f - x_lon * x_lon, 0.0f)));
     // GetViewDirectionFromCosLatLon (:104-111)
     const F3 view = R.up * cos_lat + R.front * (sin_lat * cos_lon) + R.right * (sin_lat * sin_lon);
-    RayStart s{r, alt0, cos_lat, dot(R.sun, R.up), dot(view, R.sun)};
-    const bool hit = ray_hits_ground(A, r, alt0, s.mu);
-    float dist = hit ? distance_to_bottom(A, r, alt0, s.mu) : distance_to_top(A, r, alt0, s.mu);
-    start_from_space(A, s, dist);
+    SkyViewRay ray{RayStart{r, alt0, cos_lat, dot(R.sun, R.up), dot(view, R.sun)}, 0.0f, 0.5f};
+    const bool hit = ray_hits_ground(A, r, alt0, ray.s.mu);
+    ray.dist = hit ? distance_to_bottom(A, r, alt0, ray.s.mu) : distance_to_top(A, r, alt0, ray.s.mu);
+    start_from_space(A, ray.s, ray.dist);
+    if (blue_noise) ray.start_i = float(blue_noise[(iy & (bn_size - 1)) * bn_size + (ix & (bn_size - 1))]) / 65535.0f;
+    return ray;
+}
+
+// One sky-view texel (:74-186): luminance and transmittance.
+__device__ inline void sky_view_texel(const AtmDev& A, const RenderDev& R, const float4* __restrict__ T, int tw,
+                                      int th, const float4* __restrict__ MS, int mw, int mh,
+                                      const uint16_t* __restrict__ blue_noise, int bn_size, int ix, int iy, int w,
+                                      int h, F3& L, F3& Tr) {
+    const SkyViewRay ray = sky_view_ray(A, R, blue_noise, bn_size, ix, iy, w, h);
     L = f3(0.0f);
     Tr = f3(1.0f);
     F3 Lf;
-    if (dist > 0.0f) {
-        float start_i = 0.5f;
-        if (blue_noise) start_i = float(blue_noise[(iy & (bn_size - 1)) * bn_size + (ix & (bn_size - 1))]) / 65535.0f;
-        scattered_luminance<false>(A, T, tw, th, MS, mw, mh, s, dist, R.sv_steps, start_i, L, Tr, Lf);
-    }
+    if (ray.dist > 0.0f)
+        scattered_luminance<false>(A, T, tw, th, MS, mw, mh, ray.s, ray.dist, R.sv_steps, ray.start_i, L, Tr, Lf);
 }
 
 // ProjectiveMul(inv_view_projection, (x, y, z)) - camera, normalised; the

@@ -62,6 +62,8 @@ struct PtArgs {
     float inv_brick;  // 1 / B, exact
     float tex_scale[3], tex_bias[3], grid_n[3];
     float majorant_scale;
+    // The grid's cell-corner copy (one 16-byte load per density lookup), or null
+    const uint4* cells;
     // The context's bound ray-scatter-visibility froxel, or null (visibility 1)
     const uint16_t* froxel;
     int fw, fh, fd;
@@ -197,8 +199,21 @@ struct BrickWalk {
     }
 };
 
+// The voxel material's density at ray parameter t of a brick walk, sampled in the walk's texel
+// space: the walk's X(t) = org + t k is u N per axis (z from the layer bottom), so the
+// trilinear texel coordinate is X(t) - 0.5; equal to sigma_at(o + d t) in exact arithmetic.
+// kCells reads the grid's cell-corner copy (same texels, same arithmetic).
+template <bool kCells>
+__device__ __forceinline__ float sigma_walk(const PtArgs& a, const BrickWalk& w, float t) {
+    const float x = fmaf(t, w.k[0], w.org[0]) - 0.5f, y = fmaf(t, w.k[1], w.org[1]) - 0.5f,
+                z = fmaf(t, w.k[2], w.org[2]) - 0.5f;
+    const float g = kCells ? grid_cells_trilinear_texel(a.cells, a.mat.grid.dim, x, y, z)
+                           : grid_f16_trilinear_texel(a.mat.grid, x, y, z);
+    return g * a.mat.voxel.uDensity;
+}
+
 // Delta tracking over [t0, t1] (:182-196); returns true with t at a real collision.
-template <int kMat, bool kBricks, bool kCount>
+template <int kMat, bool kBricks, bool kCount, bool kCells>
 __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, float t1, float& t, PtCount& wc) {
     if (!kBricks) {
         if (a.sigma_t_max <= 0.0f) return false;
@@ -226,7 +241,7 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
                 float tt = t + exp_sample(rng.next()) * inv_mu;
                 if (!(tt < seg_end)) break;
                 t = tt;
-                float s = sigma_at<kMat>(a, o + d * t);
+                float s = sigma_walk<kCells>(a, w, t);
                 if (kCount) wc.lookups++;
                 if (rng.next() < s * inv_mu) return true;
             }
@@ -240,7 +255,7 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
 }
 
 // TransmittanceEstimation (:135-151): ratio tracking toward the sun on a copy of the state.
-template <int kMat, bool kBricks, bool kCount>
+template <int kMat, bool kBricks, bool kCount, bool kCells>
 __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) {
     float t0, t1;
     region(a, o, d, t0, t1);
@@ -272,7 +287,7 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
                 float tt = t + exp_sample(rng.next()) * inv_mu;
                 if (!(tt < seg_end)) break;
                 t = tt;
-                float s = sigma_at<kMat>(a, o + d * t);
+                float s = sigma_walk<kCells>(a, w, t);
                 if (kCount) wc.lookups++;
                 tr *= 1.0f - fmaxf(0.0f, s * inv_mu);
             }
@@ -352,9 +367,21 @@ __device__ inline F3 lambert_sample(Rng& rng, F3 N) {
     return t0 * (sin_theta * sp) + t1 * (sin_theta * cp) + N * cos_theta;
 }
 
+// The kernel's arguments, read again from the kernarg segment (PtArgs is the kernel's only
+// argument, at offset 0).  The empty asm hides that the pointer is loop-invariant, so each use
+// after it issues its own scalar load instead of the whole block being loaded once and held in
+// (spilled) SGPRs across the tracking loops.
+__device__ __forceinline__ const PtArgs& kernel_args() {
+    auto p = (const __attribute__((address_space(4))) PtArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const PtArgs*)p;
+}
+
 // Trace (:160-249)
-template <int kMat, bool kBricks, bool kCount>
-__device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, float& scattered_t, PtCount& wc) {
+template <int kMat, bool kBricks, bool kCount, bool kCells>
+__device__ F3 trace(Rng& rng, F3 view, bool& has_scattered, float& scattered_t, PtCount& wc) {
+    const PtArgs& a0 = kernel_args();
+    const PtArgs& a = a0;
     F3 L = f3(0.0f), thr = f3(1.0f);
     has_scattered = false;
     scattered_t = 0.0f;
@@ -363,12 +390,13 @@ __device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, flo
     region(a, o, d, c0, c1);
     if (c0 >= c1) return L;
     o = o + d * c0;
-    for (int istep = 0; istep < a.max_bounces && fmaxf(thr.x, fmaxf(thr.y, thr.z)) > 0.0f; ++istep) {
+    for (int istep = 0; istep < a0.max_bounces && fmaxf(thr.x, fmaxf(thr.y, thr.z)) > 0.0f; ++istep) {
+        const PtArgs& a = kernel_args();
         float t0, t1;
         region(a, o, d, t0, t1);
         if (t0 >= t1) break;
         float t;
-        if (!delta_track<kMat, kBricks, kCount>(a, rng, o, d, t0, t1, t, wc)) {
+        if (!delta_track<kMat, kBricks, kCount, kCells>(a, rng, o, d, t0, t1, t, wc)) {
             if (a.env == SKYHIP_ENV_OFF || !has_scattered) break;
             if (a.env == SKYHIP_ENV_CONST_ENVIRONMENT_MAP) {
                 L += thr * environment(a, d);
@@ -387,7 +415,7 @@ __device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, flo
             N = N * (1.0f / length3(N));
             if (kCount) wc.vertices++;
             F3 bsdf = a.A.albedo * ((1.0f / kPiF) * dot(N, a.sun));
-            L += thr * (ratio_track<kMat, kBricks, kCount>(a, rng, o, a.sun, wc) * sun_illuminance(a, o) * bsdf);
+            L += thr * (ratio_track<kMat, kBricks, kCount, kCells>(a, rng, o, a.sun, wc) * sun_illuminance(a, o) * bsdf);
             if (a.env == SKYHIP_ENV_GROUND_SINGLE_BOUNCE) break;
             d = lambert_sample(rng, N);
             thr *= a.A.albedo;
@@ -400,7 +428,7 @@ __device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, flo
             // estimate at this vertex and the continuation both carry it.
             thr = thr * a.albedo;
             float ph = pt_phase(a, dot(d, a.sun));
-            L += thr * (ratio_track<kMat, kBricks, kCount>(a, rng, o, a.sun, wc) * ph * sun_illuminance(a, o));
+            L += thr * (ratio_track<kMat, kBricks, kCount, kCells>(a, rng, o, a.sun, wc) * ph * sun_illuminance(a, o));
             float vop;
             d = hg_sample(a, rng, d, vop);
             thr = thr * vop;
@@ -413,7 +441,7 @@ __device__ F3 trace(const PtArgs& a, Rng& rng, F3 view, bool& has_scattered, flo
 // take to scratch): more waves hide more of the tracking's fetch latency.  Measured on the
 // 1024^2 brick-majorant workload: 3 waves (no cap) 14.2 ms per frame, 4 waves 12.0, 5 waves
 // 11.2, 6 waves 11.4, 7 and 8 waves 12.0-12.2; every variant gives the same image.
-template <int kMat, bool kBricks, bool kCount>
+template <int kMat, bool kBricks, bool kCount, bool kCells>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void path_trace_kernel(PtArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int x = a.region[0] + blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
@@ -433,7 +461,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
             rng.s = rng.hash(seed_xy + a.first_frame + uint32_t(f));  // :260
             bool hs;
             float st;
-            F3 L = trace<kMat, kBricks, kCount>(a, rng, view, hs, st, wc);
+            F3 L = trace<kMat, kBricks, kCount, kCells>(rng, view, hs, st, wc);
             if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98) and the froxel visibility (:277)
                 float r = a.cam.z + a.R, mu = view.z;
                 float t = st;
@@ -503,32 +531,53 @@ __global__ __launch_bounds__(256) void majorant_kernel(const uint16_t* __restric
     out[i] = m;
 }
 
+// The cell-corner copy: cell (i, j, k) from (-1, -1, -1), x fastest, gets the eight texels
+// (i + a, j + b, k + c) in the order a + 2b + 4c, 0 outside the grid.
+__global__ __launch_bounds__(256) void cells_kernel(const uint16_t* __restrict__ g, int nx, int ny, int nz,
+                                                    uint4* __restrict__ out, unsigned n) {
+    const unsigned c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+    const int cx = int(c % unsigned(nx + 1)) - 1, cy = int((c / unsigned(nx + 1)) % unsigned(ny + 1)) - 1,
+              cz = int(c / (unsigned(nx + 1) * unsigned(ny + 1))) - 1;
+    uint32_t h[8];
+    for (int k = 0; k < 8; ++k) {
+        const int x = cx + (k & 1), y = cy + ((k >> 1) & 1), z = cz + (k >> 2);
+        const bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
+        h[k] = in ? g[((long long)z * ny + y) * nx + x] : 0u;
+    }
+    out[c] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+}
+
 template <bool kBricks, bool kCount>
 static void launch_path_trace(hipStream_t st, const PtArgs& a) {
     dim3 grid(div_up(a.region[2] - a.region[0], 16), div_up(a.region[3] - a.region[1], 16));
     switch (a.mat.kind) {
         case SKYHIP_MATERIAL_DEFAULT0:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT0, false, kCount>), grid, dim3(256), 0, st,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT0, false, kCount, false>), grid, dim3(256), 0, st,
                                a);
             break;
         case SKYHIP_MATERIAL_DEFAULT1:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1, false, kCount>), grid, dim3(256), 0, st,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1, false, kCount, false>), grid, dim3(256), 0, st,
                                a);
             break;
         case SKYHIP_MATERIAL_DEFAULT1_WORLEY:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY, false, kCount>), grid, dim3(256), 0,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY, false, kCount, false>), grid, dim3(256), 0,
                                st, a);
             break;
         case SKYHIP_MATERIAL_MINIMAL:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_MINIMAL, false, kCount>), grid, dim3(256), 0, st,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_MINIMAL, false, kCount, false>), grid, dim3(256), 0, st,
                                a);
             break;
         case SKYHIP_MATERIAL_VOXEL:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL, false, kCount>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL, false, kCount, false>), grid, dim3(256), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount>), grid, dim3(256), 0,
-                               st, a);
+            if (kBricks && a.cells)
+                hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount, kBricks>), grid,
+                                   dim3(256), 0, st, a);
+            else
+                hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount, false>), grid,
+                                   dim3(256), 0, st, a);
             break;
     }
 }
@@ -560,6 +609,22 @@ extern "C" int skyhip_voxel_grid_majorants_sized(SkyhipContext* ctx, const uint1
 extern "C" int skyhip_voxel_grid_majorants(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
                                            int32_t dim_z, float* out_majorants) {
     return skyhip_voxel_grid_majorants_sized(ctx, grid, dim_x, dim_y, dim_z, kDefaultBrick, out_majorants);
+}
+
+extern "C" int skyhip_voxel_grid_cells(SkyhipContext* ctx, const uint16_t* grid, int32_t dim_x, int32_t dim_y,
+                                       int32_t dim_z, uint16_t* out_cells) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!grid || !out_cells) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (dim_x < 2 || dim_y < 2 || dim_z < 2 || dim_x > 4096 || dim_y > 4096 || dim_z > 4096)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "grid dims outside [2, 4096]");
+    if ((reinterpret_cast<uintptr_t>(out_cells) & 15) != 0)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "out_cells must be 16-byte aligned");
+    const long long n = (long long)(dim_x + 1) * (dim_y + 1) * (dim_z + 1);
+    if (n > (1ll << 31)) return fail(SKYHIP_ERR_INVALID_SIZE, "more than 2^31 cells");
+    hipLaunchKernelGGL(cells_kernel, dim3(div_up(n, 256)), dim3(256), 0, ctx->stream, grid, dim_x, dim_y, dim_z,
+                       reinterpret_cast<uint4*>(out_cells), (unsigned)n);
+    SKYHIP_LAUNCH_CHECK("cells_kernel");
+    return SKYHIP_OK;
 }
 
 extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
@@ -676,6 +741,9 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
         a.tex_scale[2] = a.inv_thickness * a.grid_n[2];
         a.tex_bias[2] = 0.0f;
         a.majorant_scale = vb.uDensity;
+        a.cells = reinterpret_cast<const uint4*>(in->cell_corners);
+        if (a.cells && (reinterpret_cast<uintptr_t>(a.cells) & 15) != 0)
+            return fail(SKYHIP_ERR_INVALID_ARGUMENT, "cell_corners must be 16-byte aligned");
         if (!(vb.uDensity >= 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "voxel density < 0");
     }
     for (int k = 0; k < 4; ++k) a.region[k] = region[k];

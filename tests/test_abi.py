@@ -149,6 +149,8 @@ def test_material_kinds_and_new_blocks_match_the_header():
     # The Worley fields are appended: every earlier field keeps its offset.
     assert _abi.CloudMaterial.worley.offset > _abi.CloudMaterial.voxel_grid_padding.offset
     assert _abi.PathTracingInputsC.brick_size.offset == _abi.PathTracingInputsC.ap_depth.offset + 4
+    # cell_corners is appended after brick_majorants.
+    assert _abi.PathTracingInputsC.cell_corners.offset == _abi.PathTracingInputsC.brick_majorants.offset + 8
 
 
 def test_phase_binding_and_brick_size_reject_bad_input_before_device_work():
@@ -156,3 +158,4 @@ def test_phase_binding_and_brick_size_reject_bad_input_before_device_work():
     assert L.skyhip_bind_cloud_phase(None, None) == _abi.ERR_INVALID_ARGUMENT
     assert L.skyhip_voxel_grid_majorants_sized(None, None, 8, 8, 8, 8, None) == _abi.ERR_INVALID_ARGUMENT
     assert L.skyhip_noise_worley(None, None, None, 32) == _abi.ERR_INVALID_ARGUMENT
+    assert L.skyhip_voxel_grid_cells(None, None, 8, 8, 8, None) == _abi.ERR_INVALID_ARGUMENT

@@ -130,7 +130,8 @@ def test_update_luts_rejects_bad_sizes(ctx):
 
 
 def test_lut_set_single_call_matches_oracle(ctx):
-    """skyhip_atmosphere_lut_set (T -> MS -> SV || AP on a forked stream) at the Earth LUT-set sizes."""
+    """skyhip_atmosphere_lut_set (T -> MS -> SV + AP in one launch, marches split over lanes) at the Earth
+    LUT-set sizes."""
     from skyrendering_amd.lutset import LutSet
     atm_p, renders, cam, sizes = presets.earth_lut_workload()
     ls = LutSet(ctx, sizes)

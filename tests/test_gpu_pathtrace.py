@@ -90,6 +90,36 @@ def test_voxel_grid_generator_matches_oracle(ctx):
         assert np.array_equal(gb.download_majorants(), oracle.brick_majorants(got, brick)), brick
 
 
+def _cells_reference(grid):
+    """Cell-corner copy built in numpy: cell (i, j, k) from (-1, -1, -1), corners a + 2b + 4c."""
+    dz, dy, dx = grid.shape
+    pad = np.zeros((dz + 2, dy + 2, dx + 2), np.uint16)
+    pad[1:-1, 1:-1, 1:-1] = grid
+    out = np.empty((dz + 1, dy + 1, dx + 1, 8), np.uint16)
+    for k in range(8):
+        a, b, c = k & 1, (k >> 1) & 1, k >> 2
+        out[..., k] = pad[c:c + dz + 1, b:b + dy + 1, a:a + dx + 1]
+    return out
+
+
+def test_cell_corner_copy_and_identical_samples(ctx):
+    """skyhip_voxel_grid_cells equals the numpy construction (including the zero border) on a
+    ragged grid, and brick-majorant tracking gives bit-identical samples with and without it."""
+    dims = (37, 29, 23)
+    g = VoxelGrid(ctx, dims, brick=8).generate(VoxelGridCreateInfo())
+    assert np.array_equal(g.download_cells(), _cells_reference(g.download()))
+    w = cases.small_grid_workload()
+    with_cells = PathTraceScene(ctx, 32, 32, workload=w, bricks=True)
+    assert with_cells.inputs.cell_corners
+    plain_grid = VoxelGrid(ctx, with_cells.grid.dims, brick=with_cells.grid.brick, cells=False)
+    plain_grid.upload(with_cells.grid.download())
+    without = PathTraceScene(ctx, 32, 32, workload=cases.small_grid_workload(), grid=plain_grid, bricks=True)
+    assert not without.inputs.cell_corners
+    a, b = _gpu_samples(with_cells, 3), _gpu_samples(without, 3)
+    print(f"cell-corner lookups: {np.mean(a[..., 3] == 0):.3f} of samples scattered; identical: {np.array_equal(a, b)}")
+    assert np.array_equal(a, b)
+
+
 def test_single_majorant_samples_match_oracle(ctx):
     w = cases.small_grid_workload()
     scene = PathTraceScene(ctx, 32, 32, workload=w, bricks=False)
