@@ -332,6 +332,12 @@ typedef struct SkyhipCloudMaterial {
     const uint8_t* worley_texture;
     int32_t worley_size;
     int32_t worley_levels;
+    /* Optional cell-corner copies of level 0 (skyhip_texture_cells_2d_rg8 / _3d_r8), NULL to
+     * read texels one by one: a LINEAR (magnification) fetch then takes one 8-byte load.  Same
+     * texels and arithmetic: results are identical with and without them.  8-byte aligned. */
+    const uint8_t* cloud_map_cells; /* cloud_map_size^2 cells of 8 B */
+    const uint8_t* detail_cells;    /* detail_size^3 cells of 8 B */
+    const uint8_t* worley_cells;    /* worley_size^3 cells of 8 B (DEFAULT1_WORLEY) */
 } SkyhipCloudMaterial;
 
 /* NoiseCreateInfo, src/SkyRendering/VolumetricCloudDefaultMaterial.h:57-62 and
@@ -679,6 +685,15 @@ int skyhip_noise_displacement(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* p
  * that DETAIL_MAP_GEN (:80-86) mixes into the detail volume, here stored on its own.  No
  * reference counterpart. */
 int skyhip_noise_worley(SkyhipContext* ctx, const SkyhipNoiseCreateInfo* worley, uint8_t* out_r8, int32_t size);
+
+/* Cell-corner copies of a REPEAT texture's level 0 for one-load LINEAR fetches (no reference
+ * counterpart: a device data layout of the raymarch).  2D RG8, size s a power of two: cell
+ * (x, y) holds the texels (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1) (indices mod s), two
+ * bytes each, in that order; cells x fastest, out_cells: s^2 x 8 B.  3D R8, edge s a power of
+ * two: cell (x, y, z) holds the texels (x + a, y + b, z + c) mod s in the order a + 2b + 4c;
+ * out_cells: s^3 x 8 B.  Both 8-byte aligned. */
+int skyhip_texture_cells_2d_rg8(SkyhipContext* ctx, const uint8_t* level0, int32_t size, uint8_t* out_cells);
+int skyhip_texture_cells_3d_r8(SkyhipContext* ctx, const uint8_t* level0, int32_t size, uint8_t* out_cells);
 /* glGenerateTextureMipmap (DynamicTexture::Generate, VolumetricCloudDefaultMaterial.h:39-49)
  * for a packed UNORM8 chain: fills levels 1..levels-1 from level 0 with a
  * 2x2 (dims = 2) or 2x2x2 (dims = 3) box filter, rounded half up. */

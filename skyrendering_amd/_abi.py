@@ -190,7 +190,7 @@ class CloudMaterial(C.Structure):
         ("voxel_texture", vp), ("voxel_dim", _arr(i32, 3)), ("voxel_levels", i32),
         ("voxel_grid", vp), ("voxel_grid_dim", _arr(i32, 3)), ("voxel_grid_padding", i32),
         ("worley", CloudWorleyDetailBufferData), ("worley_texture", vp), ("worley_size", i32),
-        ("worley_levels", i32),
+        ("worley_levels", i32), ("cloud_map_cells", vp), ("detail_cells", vp), ("worley_cells", vp),
     ]
 
 

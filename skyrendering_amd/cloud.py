@@ -26,6 +26,8 @@ _abi.register("skyhip_noise_displacement", C.c_int, [C.c_void_p, C.POINTER(_abi.
                                                       C.c_int32])
 _abi.register("skyhip_noise_worley", C.c_int, [C.c_void_p, C.POINTER(_abi.NoiseCreateInfo), C.c_void_p, C.c_int32])
 _abi.register("skyhip_generate_mipmaps", C.c_int, [C.c_void_p, C.c_void_p] + [C.c_int32] * 6)
+_abi.register("skyhip_texture_cells_2d_rg8", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p])
+_abi.register("skyhip_texture_cells_3d_r8", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p])
 _abi.register("skyhip_cloud_raymarch", C.c_int, [
     C.c_void_p, C.POINTER(_abi.AtmosphereBufferData), C.POINTER(_abi.CloudCommonBufferData),
     C.POINTER(_abi.CloudBufferData), C.POINTER(_abi.CloudMaterial), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
@@ -247,10 +249,14 @@ class VolumetricCloudMinimalMaterial:
 
 class CloudTextures:
     """Device mip chains of the default material's textures (cloud map RG8 2D, detail R8 3D,
-    displacement RGBA8 2D), generated by the noise kernels or uploaded from the host."""
+    displacement RGBA8 2D, optional Worley R8 3D), generated by the noise kernels or uploaded
+    from the host, and with `cells` the cell-corner copies of the cloud map's, detail's and
+    Worley volume's level 0 (one 8-byte load per LINEAR fetch; 4x / 8x / 8x level 0)."""
 
-    def __init__(self, ctx: Context, cloud_map_size=512, detail_size=128, displacement_size=128, worley_size=None):
+    def __init__(self, ctx: Context, cloud_map_size=512, detail_size=128, displacement_size=128, worley_size=None,
+                 cells=True):
         self.ctx = ctx
+        self.cells = cells
         self.cloud_map_size, self.detail_size, self.displacement_size = cloud_map_size, detail_size, displacement_size
         self.worley_size = worley_size
         self.worley = None
@@ -264,6 +270,9 @@ class CloudTextures:
         self.detail = ctx.alloc(mip_chain_bytes(detail_size, detail_size, detail_size, 1, self.detail_levels))
         self.displacement = ctx.alloc(
             mip_chain_bytes(displacement_size, displacement_size, 1, 4, self.displacement_levels))
+        self.cloud_map_cells = ctx.alloc(cloud_map_size ** 2 * 8) if cells else None
+        self.detail_cells = ctx.alloc(detail_size ** 3 * 8) if cells else None
+        self.worley_cells = ctx.alloc(worley_size ** 3 * 8) if (cells and worley_size) else None
 
     def generate(self, m: DefaultMaterialCommon):
         """DynamicTexture::Generate for all three textures (VolumetricCloudDefaultMaterial.h:39-49)."""
@@ -302,6 +311,18 @@ class CloudTextures:
         if self.worley is not None:
             n = self.worley_size
             _abi.check(L.skyhip_generate_mipmaps(h, self.worley.ptr, 3, 1, n, n, n, self.worley_levels))
+        self.build_cells()
+
+    def build_cells(self):
+        """The level-0 cell-corner copies (kept in step with the textures by build_mips)."""
+        if not self.cells:
+            return
+        L = _abi.lib()
+        h = self.ctx.handle
+        _abi.check(L.skyhip_texture_cells_2d_rg8(h, self.cloud_map.ptr, self.cloud_map_size, self.cloud_map_cells.ptr))
+        _abi.check(L.skyhip_texture_cells_3d_r8(h, self.detail.ptr, self.detail_size, self.detail_cells.ptr))
+        if self.worley_cells is not None:
+            _abi.check(L.skyhip_texture_cells_3d_r8(h, self.worley.ptr, self.worley_size, self.worley_cells.ptr))
 
     def download(self):
         return (self.cloud_map.download(np.uint8, (self.cloud_map.nbytes,)),
@@ -320,6 +341,9 @@ class CloudTextures:
         if self.worley is not None:
             mat.worley_texture, mat.worley_size, mat.worley_levels = self.worley.ptr, self.worley_size, \
                 self.worley_levels
+        if self.cells:
+            mat.cloud_map_cells, mat.detail_cells = self.cloud_map_cells.ptr, self.detail_cells.ptr
+            mat.worley_cells = self.worley_cells.ptr if self.worley_cells is not None else None
 
 
 @dataclass

@@ -121,6 +121,52 @@ __global__ __launch_bounds__(256) void set_transmittance_kernel(SkyhipAtmosphere
     if (part == 0) out[t] = make_float4(v.x, v.y, v.z, 1.0f);
 }
 
+// Multiple scattering with each direction's march split over kSetMsParts waves: wave p of a
+// texel's workgroup integrates part p of all 64 directions, waves 1.. leave their partial sums
+// in LDS, and wave 0 combines them in order and reduces over the directions as the per-pass
+// kernel does.
+constexpr int kSetMsParts = 4;
+__global__ __launch_bounds__(64 * kSetMsParts) void set_multiscattering_kernel(SkyhipAtmosphereBufferData b,
+                                                                               const float4* __restrict__ T, int tw,
+                                                                               int th, float4* __restrict__ out,
+                                                                               int w, int h) {
+    constexpr int P = kSetMsParts;
+    __shared__ float part_sums[P - 1][9][64];
+    const AtmDev A = make_atm(b);
+    const int texel = blockIdx.x, part = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const MsRay ray = ms_ray(A, texel % w, texel / w, w, h, lane);
+    int n = 0;
+    for (float i = 0.5f; i < A.ms_steps; i += 1.0f) ++n;
+    const int c = (n + P - 1) / P, first = min(part * c, n);
+    const ScatterRay sc = scatter_ray<true>(A, ray.s, ray.dist, A.ms_steps);
+    F3 L, Tr, Lf;
+    scattered_luminance_part_ms(A, T, tw, th, ray.s, sc, 0.5f + float(first), max(0, min(c, n - first)), L, Tr, Lf);
+    if (part > 0) {
+        const float v[9] = {L.x, L.y, L.z, Tr.x, Tr.y, Tr.z, Lf.x, Lf.y, Lf.z};
+        for (int k = 0; k < 9; ++k) part_sums[part - 1][k][lane] = v[k];
+    }
+    __syncthreads();
+    if (part != 0) return;
+    F3 aL = f3(0.0f), aT = f3(1.0f), aF = f3(0.0f);  // the parts after this one, combined
+    for (int p = P - 1; p >= 1; --p) {
+        const F3 pL = {part_sums[p - 1][0][lane], part_sums[p - 1][1][lane], part_sums[p - 1][2][lane]};
+        const F3 pT = {part_sums[p - 1][3][lane], part_sums[p - 1][4][lane], part_sums[p - 1][5][lane]};
+        const F3 pF = {part_sums[p - 1][6][lane], part_sums[p - 1][7][lane], part_sums[p - 1][8][lane]};
+        if (p == P - 1) {
+            aL = pL, aT = pT, aF = pF;
+        } else {
+            aL = pL + pT * aL;
+            aF = pF + pT * aF;
+            aT = pT * aT;
+        }
+    }
+    L = L + Tr * aL;
+    Lf = Lf + Tr * aF;
+    Tr = Tr * aT;
+    const float4 v = multiscattering_reduce(A, T, tw, th, ray, L, Tr, Lf);
+    if (lane == 0) out[texel] = v;
+}
+
 // Blocks [0, sv_blocks) build the sky view, kSetSkyViewParts lanes per texel; the rest the
 // aerial perspective, kSetAerialParts lanes per froxel (no blue-noise dither, as the set's
 // per-pass calls).
@@ -347,7 +393,10 @@ extern "C" int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosph
     hipLaunchKernelGGL(set_transmittance_kernel, dim3(div_up((long long)tw * th * kSetTransmittanceParts, 256)),
                        dim3(256), 0, st, *atmosphere, reinterpret_cast<float4*>(transmittance_rgba), tw, th);
     SKYHIP_LAUNCH_CHECK("set_transmittance_kernel");
-    SKYHIP_CHECK(launch_multiscattering(st, *atmosphere, transmittance_rgba, tw, th, multiscattering_rgba, mw, mh));
+    hipLaunchKernelGGL(set_multiscattering_kernel, dim3(mw * mh), dim3(64 * kSetMsParts), 0, st, *atmosphere,
+                       reinterpret_cast<const float4*>(transmittance_rgba), tw, th,
+                       reinterpret_cast<float4*>(multiscattering_rgba), mw, mh);
+    SKYHIP_LAUNCH_CHECK("set_multiscattering_kernel");
     const int sv_blocks = int(div_up((long long)sky_view_width * sky_view_height * kSetSkyViewParts, 256));
     const int ap_blocks = int(div_up((long long)ap_width * ap_height * ap_depth * kSetAerialParts, 256));
     hipLaunchKernelGGL(set_sky_aerial_kernel, dim3(sv_blocks + ap_blocks), dim3(256), 0, st, *atmosphere, *render,

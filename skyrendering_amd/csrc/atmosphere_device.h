@@ -18,11 +18,6 @@
 
 #include "common.h"
 
-// Steps of the scattering integrator unrolled per loop iteration.
-#ifndef SKYHIP_INTEGRATOR_UNROLL
-#define SKYHIP_INTEGRATOR_UNROLL 2
-#endif
-
 namespace skyhip {
 
 constexpr float kPiF = 3.14159265358979323846f;
@@ -243,15 +238,8 @@ __device__ inline void scattered_luminance(const AtmDev& A, const float4* __rest
     L = f3(0.0f);
     Tr = f3(1.0f);
     Lf = f3(0.0f);
-    // The float counter's trip count first, so the step loop has an integer induction variable
-    // the compiler can unroll: the unrolled steps' LUT fetches depend only on the counter, so
-    // they issue ahead of the previous step's arithmetic.  The counter itself still advances by
-    // repeated += 1, as the shader's does.
-    float i = kMS ? 0.5f : start_i;
-    int n = 0;
-    for (float j = i; j < steps; j += 1.0f) ++n;
-#pragma unroll SKYHIP_INTEGRATOR_UNROLL
-    for (int k = 0; k < n; ++k, i += 1.0f) scatter_step<kMS>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
+    for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f)
+        scatter_step<kMS>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
 }
 
 // `count` consecutive steps of the same float counter, starting at counter value i0, with
@@ -264,8 +252,20 @@ __device__ inline void scattered_luminance_part(const AtmDev& A, const float4* _
     Tr = f3(1.0f);
     F3 Lf = f3(0.0f);
     float i = i0;
-#pragma unroll SKYHIP_INTEGRATOR_UNROLL
     for (int k = 0; k < count; ++k, i += 1.0f) scatter_step<false>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
+}
+
+// The same for the multiple-scattering variant (isotropic phase, no MS term, and the
+// multiple-scattering transfer Lf beside L): parts combine as L = L_a + Tr_a L_b,
+// Lf = Lf_a + Tr_a Lf_b, Tr = Tr_a Tr_b.
+__device__ inline void scattered_luminance_part_ms(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                                   const RayStart& s, const ScatterRay& c, float i0, int count, F3& L,
+                                                   F3& Tr, F3& Lf) {
+    L = f3(0.0f);
+    Tr = f3(1.0f);
+    Lf = f3(0.0f);
+    float i = i0;
+    for (int k = 0; k < count; ++k, i += 1.0f) scatter_step<true>(A, T, tw, th, nullptr, 0, 0, s, c, i, L, Tr, Lf);
 }
 
 // scattered_luminance<false> spread over kParts consecutive lanes of a wave (kParts a power of

@@ -402,7 +402,13 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
                 d.worley = m->worley;
                 int wdim[3] = {m->worley_size, m->worley_size, m->worley_size};
                 make_tex3d(d.worley_tex, m->worley_texture, wdim, m->worley_levels);
+                d.worley_tex.cells = reinterpret_cast<const uint2*>(m->worley_cells);
             }
+            // Cell-corner copies are read as 8-byte loads.
+            if (reinterpret_cast<uintptr_t>(m->cloud_map_cells) % 8 != 0 ||
+                reinterpret_cast<uintptr_t>(m->detail_cells) % 8 != 0 ||
+                reinterpret_cast<uintptr_t>(m->worley_cells) % 8 != 0)
+                return fail(SKYHIP_ERR_INVALID_ARGUMENT, "texture cell copies must be 8-byte aligned");
             if (!m->cloud_map || !m->detail) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "material texture is NULL");
             // RG8 texels are read as 2-byte and RGBA8 texels as 4-byte loads.
             if (reinterpret_cast<uintptr_t>(m->cloud_map) % 2 != 0 ||
@@ -422,8 +428,10 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
                 m->detail_levels < 1 || m->detail_levels > skyhip_mip_levels(m->detail_size, m->detail_size, m->detail_size))
                 return fail(SKYHIP_ERR_INVALID_SIZE, "bad mip level count");
             make_tex2d(d.cloud_map, m->cloud_map, m->cloud_map_size, m->cloud_map_levels, 2);
+            d.cloud_map.cells = reinterpret_cast<const uint2*>(m->cloud_map_cells);
             int dim[3] = {m->detail_size, m->detail_size, m->detail_size};
             make_tex3d(d.detail, m->detail, dim, m->detail_levels);
+            d.detail.cells = reinterpret_cast<const uint2*>(m->detail_cells);
             if (m->kind == SKYHIP_MATERIAL_DEFAULT0) {
                 if (!m->displacement || !pow2(m->displacement_size) || m->displacement_levels < 1 ||
                     m->displacement_levels > skyhip_mip_levels(m->displacement_size, m->displacement_size, 1))

@@ -22,6 +22,7 @@ struct Tex2D {
     int size, levels, channels;
     int log2_size;  // size is a power of two (REPEAT textures)
     uint32_t offset[kMaxMipLevels];
+    const uint2* cells;  // RG8 level-0 cell-corner copy, or null
 };
 
 struct Tex3D {
@@ -30,6 +31,7 @@ struct Tex3D {
     int levels;
     int log2_size;  // cube edge log2 for REPEAT volumes (power-of-two cubes), -1 otherwise
     uint32_t offset[kMaxMipLevels];
+    const uint2* cells;  // R8 level-0 cell-corner copy (REPEAT volumes), or null
 };
 
 // GL NEAREST_MIPMAP_NEAREST level for lambda > 0.5 (the caller handles
@@ -66,8 +68,14 @@ __device__ inline void tex2d_repeat(const Tex2D& t, float u, float v, float lamb
         const float ax = x - floorf(x), ay = y - floorf(y);
         const uint32_t x0 = uint32_t(ix) & m, x1 = uint32_t(ix + 1) & m;
         const uint32_t r0 = (uint32_t(iy) & m) << ls, r1 = (uint32_t(iy + 1) & m) << ls;
-        const uint32_t b00 = texel_bits<kChannels>(t.data, r0 | x0), b10 = texel_bits<kChannels>(t.data, r0 | x1);
-        const uint32_t b01 = texel_bits<kChannels>(t.data, r1 | x0), b11 = texel_bits<kChannels>(t.data, r1 | x1);
+        uint32_t b00, b10, b01, b11;
+        if (kChannels == 2 && t.cells) {  // the cell's four RG8 texels in one 8-byte load
+            const uint2 q = t.cells[r0 | x0];
+            b00 = q.x & 0xffffu, b10 = q.x >> 16, b01 = q.y & 0xffffu, b11 = q.y >> 16;
+        } else {
+            b00 = texel_bits<kChannels>(t.data, r0 | x0), b10 = texel_bits<kChannels>(t.data, r0 | x1);
+            b01 = texel_bits<kChannels>(t.data, r1 | x0), b11 = texel_bits<kChannels>(t.data, r1 | x1);
+        }
         for (int c = 0; c < kChannels; ++c)
             out[c] = lerpf(lerpf(unorm_byte(b00, c), unorm_byte(b10, c), ax),
                            lerpf(unorm_byte(b01, c), unorm_byte(b11, c), ax), ay) * (1.0f / 255.0f);
@@ -91,13 +99,22 @@ __device__ inline float tex3d_r8_repeat(const Tex3D& t, float u, float v, float 
         const float x = u * float(s) - 0.5f, y = v * float(s) - 0.5f, z = w * float(s) - 0.5f;
         const int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
         const float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
-        const uint32_t x0 = uint32_t(ix) & m, x1 = uint32_t(ix + 1) & m;
-        const uint32_t y0 = (uint32_t(iy) & m) << ls, y1 = (uint32_t(iy + 1) & m) << ls;
-        const uint32_t z0 = (uint32_t(iz) & m) << (2 * ls), z1 = (uint32_t(iz + 1) & m) << (2 * ls);
-        const float c00 = lerpf(float(d[z0 | y0 | x0]), float(d[z0 | y0 | x1]), ax);
-        const float c10 = lerpf(float(d[z0 | y1 | x0]), float(d[z0 | y1 | x1]), ax);
-        const float c01 = lerpf(float(d[z1 | y0 | x0]), float(d[z1 | y0 | x1]), ax);
-        const float c11 = lerpf(float(d[z1 | y1 | x0]), float(d[z1 | y1 | x1]), ax);
+        const uint32_t x0 = uint32_t(ix) & m, y0 = (uint32_t(iy) & m) << ls, z0 = (uint32_t(iz) & m) << (2 * ls);
+        float c00, c10, c01, c11;
+        if (t.cells) {  // the cell's eight texels, byte a + 2b + 4c, in one 8-byte load
+            const uint2 q = t.cells[z0 | y0 | x0];
+            c00 = lerpf(float(q.x & 0xffu), float((q.x >> 8) & 0xffu), ax);
+            c10 = lerpf(float((q.x >> 16) & 0xffu), float(q.x >> 24), ax);
+            c01 = lerpf(float(q.y & 0xffu), float((q.y >> 8) & 0xffu), ax);
+            c11 = lerpf(float((q.y >> 16) & 0xffu), float(q.y >> 24), ax);
+        } else {
+            const uint32_t x1 = uint32_t(ix + 1) & m, y1 = (uint32_t(iy + 1) & m) << ls,
+                           z1 = (uint32_t(iz + 1) & m) << (2 * ls);
+            c00 = lerpf(float(d[z0 | y0 | x0]), float(d[z0 | y0 | x1]), ax);
+            c10 = lerpf(float(d[z0 | y1 | x0]), float(d[z0 | y1 | x1]), ax);
+            c01 = lerpf(float(d[z1 | y0 | x0]), float(d[z1 | y0 | x1]), ax);
+            c11 = lerpf(float(d[z1 | y1 | x0]), float(d[z1 | y1 | x1]), ax);
+        }
         return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az) * (1.0f / 255.0f);
     }
     const int l = nearest_level(lambda, t.levels);
@@ -226,6 +243,9 @@ struct MaterialDev {
 };
 
 __device__ __forceinline__ float length3(F3 a) { return sqrtf(dot(a, a)); }
+// The hardware square root (v_sqrt_f32, within 1 ulp) where the result feeds a LOD or a
+// height fraction: the correctly rounded sqrtf expands to about 20 instructions.
+__device__ __forceinline__ float sqrt_1ulp(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Work tally of the counting kernel variants (per lane).
 struct WorkCount {
@@ -243,7 +263,8 @@ __device__ inline float sample_sigma_t(const MaterialDev& M, F3 pos, float heigh
         return grid_f16_trilinear(M.grid, pos.x * M.voxel.uSampleFrequency[0] + M.voxel.uSampleBias[0],
                                   pos.y * M.voxel.uSampleFrequency[1] + M.voxel.uSampleBias[1], height01) *
                M.voxel.uDensity;
-    const float dist = length3(pos - cam);
+    const F3 rel = pos - cam;
+    const float dist = sqrt_1ulp(dot(rel, rel));  // LOD distance
     if (kMat == SKYHIP_MATERIAL_VOXEL) {  // VolumetricCloudMaterialVoxel.glsl:12-16
         float u = pos.x * M.voxel.uSampleFrequency[0] + M.voxel.uSampleBias[0];
         float v = pos.y * M.voxel.uSampleFrequency[1] + M.voxel.uSampleBias[1];
@@ -319,7 +340,7 @@ struct Shell {
 // CalHeight01 (VolumetricCloudCommon.glsl:36-40) via |p'|^2 - R^2 = x^2 + y^2 + z (z + 2R).
 __device__ __forceinline__ float height01(const Shell& S, F3 p, float* q_out = nullptr) {
     float q = p.x * p.x + p.y * p.y + p.z * (p.z + 2.0f * S.R);
-    float alt = q * __builtin_amdgcn_rcpf(sqrtf(fmaxf(q + S.R * S.R, 0.0f)) + S.R);  // 1-ulp reciprocal
+    float alt = q * __builtin_amdgcn_rcpf(sqrt_1ulp(fmaxf(q + S.R * S.R, 0.0f)) + S.R);  // 1-ulp sqrt, reciprocal
     if (q_out) *q_out = q;
     return clamp01((alt - S.hb) * S.inv_thickness);
 }

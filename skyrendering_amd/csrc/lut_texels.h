@@ -110,10 +110,13 @@ __device__ inline float4 transmittance_texel_rgba(const SkyhipAtmosphereBufferDa
     return make_float4(v.x, v.y, v.z, 1.0f);
 }
 
-// One multiple-scattering texel by the 64 lanes of a wave (lane = sphere direction);
-// the result is valid in lane 0.  Every lane of the wave must call it.
-__device__ inline float4 multiscattering_texel(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
-                                               int ix, int iy, int w, int h, int lane) {
+// The ray of one multiple-scattering direction (lane = sphere direction index, 0..63).
+struct MsRay {
+    RayStart s;
+    float dist;
+    bool hit;
+};
+__device__ inline MsRay ms_ray(const AtmDev& A, int ix, int iy, int w, int h, int lane) {
     // GetAltitudeMuSFromMultiscatteringTextureIndex
     const float altitude = (float(iy) / float(h - 1)) * A.thickness;
     const float mu_s = (float(ix) / float(w - 1)) * 2.0f - 1.0f;
@@ -132,17 +135,19 @@ __device__ inline float4 multiscattering_texel(const AtmDev& A, const float4* __
     const float nu = view.y * mu_s + view.z * sun_z;
     const bool hit = ray_hits_ground(A, r, altitude, mu);
     const float dist = hit ? distance_to_bottom(A, r, altitude, mu) : distance_to_top(A, r, altitude, mu);
-    F3 L, Tr, Lf;
-    scattered_luminance<true>(A, T, tw, th, nullptr, 0, 0, RayStart{r, altitude, mu, mu_s, nu}, dist, A.ms_steps,
-                              0.5f, L, Tr, Lf);
-    if (hit) {
-        // ComputeGroundLuminance (:297-306) at the ground point, |p - c| = R_b.
-        float mu_g = (r * mu_s + dist * nu) / A.rb;
+    return {RayStart{r, altitude, mu, mu_s, nu}, dist, hit};
+}
+
+// The ground term (ComputeGroundLuminance, :297-306, at |p - c| = R_b) and the reduction over the
+// 64 directions of a wave, in the reference's fixed-order shared-memory tree (:392-427): lane i
+// adds lane i + s for s = 32, 16, ..., 1, which is exactly this shuffle chain.  Valid in lane 0.
+__device__ inline float4 multiscattering_reduce(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                                const MsRay& ray, F3 L, F3 Tr, F3 Lf) {
+    if (ray.hit) {
+        float mu_g = (ray.s.r * ray.s.mu_s0 + ray.dist * ray.s.nu) / A.rb;
         F3 sv = sun_visibility(A, T, tw, th, A.rb, 0.0f, 0.0f, mu_g);
         L += Tr * ((1.0f / kPiF) * clamp01(mu_g)) * A.albedo * sv;
     }
-    // The reference's fixed-order shared-memory tree (:392-427): lane i adds
-    // lane i + s for s = 32, 16, ..., 1, which is exactly this shuffle chain.
     for (int s = 32; s >= 1; s >>= 1) {
         L.x += __shfl_down(L.x, s, 64);
         L.y += __shfl_down(L.y, s, 64);
@@ -153,6 +158,16 @@ __device__ inline float4 multiscattering_texel(const AtmDev& A, const float4* __
     }
     F3 l2 = L * (1.0f / 64.0f), fms = Lf * (1.0f / 64.0f);
     return make_float4(l2.x / (1.0f - fms.x), l2.y / (1.0f - fms.y), l2.z / (1.0f - fms.z), 1.0f);
+}
+
+// One multiple-scattering texel by the 64 lanes of a wave (lane = sphere direction);
+// the result is valid in lane 0.  Every lane of the wave must call it.
+__device__ inline float4 multiscattering_texel(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
+                                               int ix, int iy, int w, int h, int lane) {
+    const MsRay ray = ms_ray(A, ix, iy, w, h, lane);
+    F3 L, Tr, Lf;
+    scattered_luminance<true>(A, T, tw, th, nullptr, 0, 0, ray.s, ray.dist, A.ms_steps, 0.5f, L, Tr, Lf);
+    return multiscattering_reduce(A, T, tw, th, ray, L, Tr, Lf);
 }
 
 // The renderer's block as the kernels see it.

@@ -302,6 +302,66 @@ extern "C" int skyhip_noise_worley(SkyhipContext* ctx, const SkyhipNoiseCreateIn
     return SKYHIP_OK;
 }
 
+// Cell-corner copies of a REPEAT texture's level 0 (size s a power of two, indices mod s):
+// a 2D RG8 cell (x, y) packs texels (x, y), (x + 1, y) in .x and (x, y + 1), (x + 1, y + 1) in
+// .y, two bytes each; a 3D R8 cell (x, y, z) packs the eight texels (x + a, y + b, z + c) as
+// byte a + 2b + 4c.
+__global__ __launch_bounds__(256) void cells_2d_rg8_kernel(const uint16_t* __restrict__ t, int ls,
+                                                           uint2* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, s = 1u << ls, m = s - 1u;
+    if (i >= s * s) return;
+    const uint32_t x = i & m, y = i >> ls, x1 = (x + 1u) & m, y1 = (y + 1u) & m;
+    const uint32_t r0 = y << ls, r1 = y1 << ls;
+    out[i] = make_uint2(uint32_t(t[r0 | x]) | (uint32_t(t[r0 | x1]) << 16), uint32_t(t[r1 | x]) | (uint32_t(t[r1 | x1]) << 16));
+}
+
+__global__ __launch_bounds__(256) void cells_3d_r8_kernel(const uint8_t* __restrict__ t, int ls,
+                                                          uint2* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, s = 1u << ls, m = s - 1u;
+    if (i >= s * s * s) return;
+    const uint32_t x = i & m, y = (i >> ls) & m, z = i >> (2 * ls);
+    uint32_t w[2] = {0u, 0u};
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t xx = (x + uint32_t(k & 1)) & m, yy = (y + uint32_t((k >> 1) & 1)) & m,
+                       zz = (z + uint32_t(k >> 2)) & m;
+        w[k >> 2] |= uint32_t(t[(zz << (2 * ls)) | (yy << ls) | xx]) << (8 * (k & 3));
+    }
+    out[i] = make_uint2(w[0], w[1]);
+}
+
+static int cells_args(SkyhipContext* ctx, const uint8_t* level0, int32_t size, uint8_t* out, int max_size,
+                      int align_in, int& ls) {
+    SKYHIP_CHECK(check_ctx(ctx));
+    if (!level0 || !out) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (size < 2 || size > max_size || (size & (size - 1)) != 0)
+        return fail(SKYHIP_ERR_INVALID_SIZE, "texture size %d is not a power of two in [2, %d]", size, max_size);
+    if (reinterpret_cast<uintptr_t>(out) % 8 != 0 || reinterpret_cast<uintptr_t>(level0) % align_in != 0)
+        return fail(SKYHIP_ERR_INVALID_ARGUMENT, "misaligned texture or cell buffer");
+    ls = 0;
+    while ((1 << ls) < size) ++ls;
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_texture_cells_2d_rg8(SkyhipContext* ctx, const uint8_t* level0, int32_t size,
+                                           uint8_t* out_cells) {
+    int ls;
+    SKYHIP_CHECK(cells_args(ctx, level0, size, out_cells, 16384, 2, ls));
+    hipLaunchKernelGGL(cells_2d_rg8_kernel, dim3(div_up((long long)size * size, 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const uint16_t*>(level0), ls, reinterpret_cast<uint2*>(out_cells));
+    SKYHIP_LAUNCH_CHECK("cells_2d_rg8_kernel");
+    return SKYHIP_OK;
+}
+
+extern "C" int skyhip_texture_cells_3d_r8(SkyhipContext* ctx, const uint8_t* level0, int32_t size,
+                                          uint8_t* out_cells) {
+    int ls;
+    SKYHIP_CHECK(cells_args(ctx, level0, size, out_cells, 1024, 1, ls));
+    hipLaunchKernelGGL(cells_3d_r8_kernel, dim3(div_up((long long)size * size * size, 256)), dim3(256), 0,
+                       ctx->stream, level0, ls, reinterpret_cast<uint2*>(out_cells));
+    SKYHIP_LAUNCH_CHECK("cells_3d_r8_kernel");
+    return SKYHIP_OK;
+}
+
 extern "C" int skyhip_generate_mipmaps(SkyhipContext* ctx, uint8_t* texture, int32_t dims, int32_t channels,
                                        int32_t width, int32_t height, int32_t depth, int32_t levels) {
     SKYHIP_CHECK(check_ctx(ctx));

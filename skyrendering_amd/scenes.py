@@ -20,7 +20,7 @@ from .device import Context
 
 class CloudScene:
     def __init__(self, ctx: Context, width=1920, height=1080, workload=None, material=None,
-                 weather="uniform", blue_noise=None, frame_id=0, lod_bias=None):
+                 weather="uniform", blue_noise=None, frame_id=0, lod_bias=None, texture_cells=True):
         w = workload or presets.cloud_workload()
         self.ctx = ctx
         self.width, self.height = width, height
@@ -46,7 +46,7 @@ class CloudScene:
         if not isinstance(self.material, VolumetricCloudMinimalMaterial):
             m = self.material
             self.textures = CloudTextures(ctx, m.cloud_map_size, m.detail_size, m.displacement_size,
-                                          worley_size=getattr(m, "worley_size", None))
+                                          worley_size=getattr(m, "worley_size", None), cells=texture_cells)
             self.textures.generate(m)
             if weather == "uniform":
                 self.textures.upload_level0(cloud_map=presets.uniform_weather_map(m.cloud_map_size,

@@ -212,6 +212,27 @@ def test_worley_volume_matches_oracle(ctx):
     assert np.array_equal(got, oracle.generate_mipmaps(chain, 3, 1, 32, 32, 32, tex.worley_levels))
 
 
+def test_texture_cells_match_numpy_and_leave_images_unchanged(ctx):
+    """The level-0 cell-corner copies (RG8 2D and R8 3D, REPEAT wrap) equal their numpy
+    construction, and the config-2 raymarch gives the same image bits with and without them."""
+    m = presets.cloud_workload_config2()["material"]
+    tex = CloudTextures(ctx, 512, 128, 128, worley_size=32).generate(m)
+    cm = tex.cloud_map.download(np.uint8, (512 * 512 * 2,))[:512 * 512 * 2].view(np.uint16).reshape(512, 512)
+    want2 = np.stack([cm, np.roll(cm, -1, 1), np.roll(cm, -1, 0), np.roll(np.roll(cm, -1, 0), -1, 1)], -1)
+    assert np.array_equal(tex.cloud_map_cells.download(np.uint16, (512, 512, 4)), want2)
+    for vol, n, cells in ((tex.detail, 128, tex.detail_cells), (tex.worley, 32, tex.worley_cells)):
+        v = vol.download(np.uint8, (n ** 3,)).reshape(n, n, n)
+        want3 = np.stack([np.roll(v, (-(k >> 2), -((k >> 1) & 1), -(k & 1)), (0, 1, 2)) for k in range(8)], -1)
+        assert np.array_equal(cells.download(np.uint8, (n, n, n, 8)), want3), n
+    images = []
+    for cells in (True, False):
+        scene = CloudScene(ctx, 480, 270, workload=presets.cloud_workload_config2(), texture_cells=cells)
+        scene.run()
+        rgba, dist = scene.image()
+        images.append((rgba, dist))
+    assert np.array_equal(images[0][0], images[1][0]) and np.array_equal(images[0][1], images[1][1])
+
+
 def test_raymarch_config2_parity(ctx):
     """The cloud benchmark as stated: Material1 plus the 32^3 Worley detail volume, dual-lobe HG with
     g = 0.8 / -0.3, the seeded 128^2 blue-noise tile, at 480x270, every pixel."""
