@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void aerial_perspective_kernel(
 // match the per-pass kernels bit for bit, does not use these.
 // ---------------------------------------------------------------------------
 constexpr int kSetTransmittanceParts = 4;
-constexpr int kSetSkyViewParts = 4;
+constexpr int kSetSkyViewParts = 2;
 constexpr int kSetAerialParts = 8;
 
 template <class S, int kParts>
