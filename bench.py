@@ -42,15 +42,14 @@ HBM_PEAK_GBS = 8000.0
 # profiles/lut_batch_256x8.md): SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP32_TRANS count
 # operations per wave instruction (an FMA counts 2), so the lane operations are 64 times that,
 # scaled by the measured lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), then
-# divided by the executed sample-steps (raymarch: 8.246e9 over 35,111,907) or integrator steps
-# (LUT set: 1.0386e9 over 4,914,688; batch: 7.183e11 over 3,024,532,480) of the same launches.
+# divided by the executed sample-steps (raymarch: 7.314e9 over 35,111,907) or integrator steps
+# (LUT set, counted on the one-lane-per-texel kernels so that the split kernels' repeated ray
+# set-up is not counted as work: 1.0386e9 over 4,914,688; batch: 7.185e11 over 3,024,532,480)
+# of the same launches.
 RAYMARCH_KERNEL = "cloud_raymarch_kernel<5,false>"
-RAYMARCH_FP32_FLOP_PER_STEP = 234.8
+RAYMARCH_FP32_FLOP_PER_STEP = 208.3
 LUT_SET_FLOP_PER_STEP = 211.3
 BATCH_FLOP_PER_STEP = 237.5
-# VALU issue: the raymarch issues 3.874e8 wave instructions per frame, 20 % of the 1024 SIMDs'
-# issue slots at 2.4 GHz over its 0.779 ms, at 62 % lane utilisation: the kernel waits on its
-# dependent texture fetches more than it computes.
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
@@ -61,8 +60,8 @@ BATCH_FLOP_PER_STEP = 237.5
 FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
-    "cloud_raymarch_kernel<5,false>": (7527.9, 24313.3),
-    "path_trace_kernel<4,true,false>": (6.6085e6, 1.9962e5),
+    "cloud_raymarch_kernel<5,false>": (36115.9, 25697.5),
+    "path_trace_kernel<4,true,false,true>": (3.9300e6, 9.8996e4),
 }
 
 
@@ -145,8 +144,9 @@ def lut_set_section(ctx, rounds, cpu_seconds):
            "roofline": {"bound": "valu", "unit": "TFLOP/s", "flop_per_step": LUT_SET_FLOP_PER_STEP,
                         "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                        "note": "latency-bound: the set is 4 dependent launches (T -> MS -> SV || AP) of "
-                                "16k-33k lanes; the FP32 operations per step are measured by SQ counters"}}
+                        "note": "latency-bound: the set is 3 dependent launches (T -> MS -> SV + AP) whose "
+                                "marches are split over lanes and waves to fill the CUs; the FP32 operations per "
+                                "step are measured by SQ counters"}}
     if cpu_seconds > 0:
         import oracle
         atm, renders, cam, sizes = presets.earth_lut_workload()
@@ -214,8 +214,8 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     # Algorithmic bytes per frame: 8 FP16 texels per trilinear density lookup, one FP32 majorant
     # per brick segment, and the RGBA32F accumulator read and written once per pixel.
     alg_bytes = counts["lookups"] * 16 + counts["segments"] * 4 + n * 32
-    traffic = profiled_traffic("path_trace_kernel<4,true,false>")
-    roofline = {"bound": "hbm", "unit": "GB/s", "kernel": "path_trace_kernel<4,true,false>",
+    traffic = profiled_traffic("path_trace_kernel<4,true,false,true>")
+    roofline = {"bound": "hbm", "unit": "GB/s", "kernel": "path_trace_kernel<4,true,false,true>",
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "achieved": round(alg_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                 "frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -727,9 +727,9 @@ def main():
                                "elevation 20 deg",
                    "width": 1920, "height": 1080, "executed_steps_per_frame": steps,
                    "density_evals_per_frame": counts["evals"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "roof": "FP32 vector peak (textures are L2-resident; HBM traffic is the "
-                                              "inputs once and the outputs once); measured VALU issue 20 % of "
-                                              "slots at 62 % lane utilisation: fetch-latency limited",
+        "roofline": {"bound": "valu", "roof": "FP32 vector peak (the textures and their cell-corner copies stay "
+                                              "in L2 and the 256 MB last-level cache; 98 MB of HBM traffic per "
+                                              "frame); 62 % lane utilisation (rays end at different steps)",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                      "traffic_gbs": round(traffic / (ms * 1e-3) / 1e9, 1), "kernel": RAYMARCH_KERNEL,
