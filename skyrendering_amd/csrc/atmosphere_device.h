@@ -111,14 +111,24 @@ __device__ inline float mie_phase(float g, float c) {
 }
 
 // Bilinear CLAMP_TO_EDGE fetch of RGB from an RGBA32F LUT at texel-space
-// coordinates (0 .. N-1 covers texel centres 0 .. N-1).
+// coordinates (0 .. N-1 covers texel centres 0 .. N-1).  kCells: `lut` is the LUT's cell-corner
+// copy (lut_cells_kernel), where texel (x, y) is followed by the rest of its bilinear footprint:
+// 4 float4 per texel, (x, y), (x1, y), (x, y1), (x1, y1) with x1 = min(x + 1, w - 1) and y1 likewise,
+// read from one address.  The same texels and the same arithmetic either way.
+template <bool kCells = false>
 __device__ inline F3 lut_bilinear(const float4* __restrict__ lut, int w, int h, float fx, float fy) {
     fx = fminf(fmaxf(fx, 0.0f), float(w - 1));
     fy = fminf(fmaxf(fy, 0.0f), float(h - 1));
     int x0 = int(fx), y0 = int(fy);
-    int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
     float ax = fx - float(x0), ay = fy - float(y0);
-    float4 a = lut[y0 * w + x0], b = lut[y0 * w + x1], c = lut[y1 * w + x0], d = lut[y1 * w + x1];
+    float4 a, b, c, d;
+    if (kCells) {
+        const float4* q = lut + 4 * (y0 * w + x0);
+        a = q[0], b = q[1], c = q[2], d = q[3];
+    } else {
+        int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+        a = lut[y0 * w + x0], b = lut[y0 * w + x1], c = lut[y1 * w + x0], d = lut[y1 * w + x1];
+    }
     float bx = 1.0f - ax, by = 1.0f - ay;
     return {by * (bx * a.x + ax * b.x) + ay * (bx * c.x + ax * d.x),
             by * (bx * a.y + ax * b.y) + ay * (bx * c.y + ax * d.y),
@@ -128,6 +138,7 @@ __device__ inline F3 lut_bilinear(const float4* __restrict__ lut, int w, int h, 
 // GetSunVisibility (Atmosphere.glsl:105-117) at a point given by its radius r,
 // q = r^2 - R_b^2 (negative below the ground, where the reference's aerial-
 // perspective froxels can reach), altitude and mu_s.
+template <bool kCells = false>
 __device__ inline F3 sun_visibility(const AtmDev& A, const float4* __restrict__ T, int tw, int th, float r,
                                     float q, float altitude, float mu_s) {
     // GetTransmittanceTextureUvFromRMu (:90-103): rho = SafeSqrt(r^2 - R_b^2),
@@ -139,7 +150,7 @@ __device__ inline F3 sun_visibility(const AtmDev& A, const float4* __restrict__ 
     float d_max = rho + A.H;
     float x_mu = fdiv_fast(d - d_min, d_max - d_min);
     float x_r = rho * A.inv_H;
-    F3 t = lut_bilinear(T, tw, th, x_mu * float(tw - 1), x_r * float(th - 1));
+    F3 t = lut_bilinear<kCells>(T, tw, th, x_mu * float(tw - 1), x_r * float(th - 1));
     const float inv_r = __builtin_amdgcn_rcpf(r);
     float sin_theta_h = A.rb * inv_r;
     float cos_theta_h = -rho * inv_r;
@@ -150,11 +161,12 @@ __device__ inline F3 sun_visibility(const AtmDev& A, const float4* __restrict__ 
 }
 
 // GetMultiscatteringContribution (Atmosphere.glsl:169-178).
+template <bool kCells = false>
 __device__ inline F3 ms_contribution(const AtmDev& A, const float4* __restrict__ MS, int mw, int mh, float altitude,
                                      float mu_s) {
     float x_mu_s = mu_s * 0.5f + 0.5f;
     float x_r = altitude * A.inv_thickness;
-    return lut_bilinear(MS, mw, mh, x_mu_s * float(mw - 1), x_r * float(mh - 1));
+    return lut_bilinear<kCells>(MS, mw, mh, x_mu_s * float(mw - 1), x_r * float(mh - 1));
 }
 
 // Geometry of a ray that starts at radius r (altitude alt0 = r - R_b) with
@@ -196,8 +208,8 @@ __device__ inline ScatterRay scatter_ray(const AtmDev& A, const RayStart& s, flo
             2.0f * s.r * s.mu, s.alt0 * (s.r + A.rb), A.rb * A.rb};
 }
 
-// One step of the integrator at float counter value i.
-template <bool kMS>
+// One step of the integrator at float counter value i (kCells: T and MS are cell-corner copies).
+template <bool kMS, bool kCells = false>
 __device__ __forceinline__ void scatter_step(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
                                              const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
                                              const ScatterRay& c, float i, F3& L, F3& Tr, F3& Lf) {
@@ -214,10 +226,10 @@ __device__ __forceinline__ void scatter_step(const AtmDev& A, const float4* __re
     const F3 tr = {__expf(-ext.x * dx), __expf(-ext.y * dx), __expf(-ext.z * dx)};
     F3 one_minus_t = {one_minus_exp(ext.x * dx, tr.x), one_minus_exp(ext.y * dx, tr.y),
                       one_minus_exp(ext.z * dx, tr.z)};
-    F3 Li = (sr * c.rp + sm * c.mp) * sun_visibility(A, T, tw, th, r_d, q, alt, mu_s);
+    F3 Li = (sr * c.rp + sm * c.mp) * sun_visibility<kCells>(A, T, tw, th, r_d, q, alt, mu_s);
     F3 inv_ext = {__builtin_amdgcn_rcpf(ext.x), __builtin_amdgcn_rcpf(ext.y), __builtin_amdgcn_rcpf(ext.z)};
     if (!kMS) {
-        Li += A.ms_mask * ms_contribution(A, MS, mw, mh, alt, mu_s) * (sr + sm);
+        Li += A.ms_mask * ms_contribution<kCells>(A, MS, mw, mh, alt, mu_s) * (sr + sm);
         Li *= A.solar;
     }
     L += Tr * Li * one_minus_t * inv_ext;
@@ -229,7 +241,7 @@ __device__ __forceinline__ void scatter_step(const AtmDev& A, const float4* __re
 // MULTISCATTERING_COMPUTE_PROGRAM variant.  The loop keeps the shader's float
 // counter (i = start_i; i < steps; ++i), so fractional step counts iterate
 // exactly as the GLSL does.
-template <bool kMS>
+template <bool kMS, bool kCells = false>
 __device__ inline void scattered_luminance(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
                                            const float4* __restrict__ MS, int mw, int mh, const RayStart& s,
                                            float marching_distance, float steps, float start_i, F3& L, F3& Tr,
@@ -239,7 +251,7 @@ __device__ inline void scattered_luminance(const AtmDev& A, const float4* __rest
     Tr = f3(1.0f);
     Lf = f3(0.0f);
     for (float i = kMS ? 0.5f : start_i; i < steps; i += 1.0f)
-        scatter_step<kMS>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
+        scatter_step<kMS, kCells>(A, T, tw, th, MS, mw, mh, s, c, i, L, Tr, Lf);
 }
 
 // `count` consecutive steps of the same float counter, starting at counter value i0, with

@@ -49,6 +49,29 @@ __device__ __forceinline__ int next_tile(unsigned* ctr, int* s_tile) {
     return tile;
 }
 
+// Cell-corner copies of every item's T (kind 0) or MS (kind 1) LUT: texel (x, y) of an item is
+// followed by the rest of its bilinear footprint, (x, y), (x1, y), (x, y1), (x1, y1) with
+// x1 = min(x + 1, w - 1) and y1 likewise (4 float4 per texel, lut_bilinear<true>).  Items stride
+// over blockIdx.y, texels over blockIdx.x.
+__global__ __launch_bounds__(256) void batch_lut_cells_kernel(const BatchSetDev* __restrict__ sets, int n_sets,
+                                                              int kind, const float4* __restrict__ pool,
+                                                              float4* __restrict__ cells) {
+    for (int item = blockIdx.y; item < n_sets; item += gridDim.y) {
+        const BatchSetDev S = sets[item];
+        const int w = kind == 0 ? S.tw : S.mw, h = kind == 0 ? S.th : S.mh;
+        const long long off = kind == 0 ? S.t_off : S.ms_off;
+        const float4* src = pool + off;
+        for (int t = blockIdx.x * 256 + threadIdx.x; t < w * h; t += gridDim.x * 256) {
+            const int x = t % w, y = t / w, x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1);
+            float4* dst = cells + 4 * (off + t);
+            dst[0] = src[y * w + x];
+            dst[1] = src[y * w + x1];
+            dst[2] = src[y1 * w + x];
+            dst[3] = src[y1 * w + x1];
+        }
+    }
+}
+
 template <int kBlock>
 __global__ __launch_bounds__(kBlock) void batch_transmittance_kernel(
     const SkyhipAtmosphereBufferData* __restrict__ atm, const BatchSetDev* __restrict__ sets,
@@ -71,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void batch_transmittance_kernel(
 template <int kWaves>
 __global__ __launch_bounds__(64 * kWaves) void batch_multiscattering_kernel(
     const SkyhipAtmosphereBufferData* __restrict__ atm, const BatchSetDev* __restrict__ sets,
-    const Tile* __restrict__ tiles, int n_tiles, const float4* __restrict__ t_pool, float4* __restrict__ pool,
+    const Tile* __restrict__ tiles, int n_tiles, const float4* __restrict__ t_cells, float4* __restrict__ pool,
     unsigned* ctr) {
     __shared__ int s_tile;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -83,8 +106,8 @@ __global__ __launch_bounds__(64 * kWaves) void batch_multiscattering_kernel(
         const int texel = int(tl.first) + wave;
         if (texel < S.mw * S.mh) {  // wave-uniform
             const AtmDev A = make_atm(atm[tl.item]);
-            float4 v = multiscattering_texel(A, t_pool + S.t_off, S.tw, S.th, texel % S.mw, texel / S.mw, S.mw, S.mh,
-                                             lane);
+            float4 v = multiscattering_texel<true>(A, t_cells + 4 * S.t_off, S.tw, S.th, texel % S.mw, texel / S.mw,
+                                                   S.mw, S.mh, lane);
             if (lane == 0) pool[S.ms_off + texel] = v;
         }
     }
@@ -94,7 +117,7 @@ template <int kBlock>
 __global__ __launch_bounds__(kBlock) void batch_sky_view_kernel(
     const SkyhipAtmosphereBufferData* __restrict__ atm, const SkyhipAtmosphereRenderBufferData* __restrict__ renders,
     const BatchSetDev* __restrict__ sets, const BatchViewDev* __restrict__ views, const Tile* __restrict__ tiles,
-    int n_tiles, int tile_size, const float4* __restrict__ t_pool, const float4* __restrict__ ms_pool,
+    int n_tiles, int tile_size, const float4* __restrict__ t_cells, const float4* __restrict__ ms_cells,
     float4* __restrict__ l_pool, float4* __restrict__ tr_pool, unsigned* ctr) {
     __shared__ int s_tile;
     for (;;) {
@@ -110,8 +133,8 @@ __global__ __launch_bounds__(kBlock) void batch_sky_view_kernel(
             const int t = int(tl.first) + k;
             if (t >= n) break;
             F3 L, Tr;
-            sky_view_texel(A, R, t_pool + S.t_off, S.tw, S.th, ms_pool + S.ms_off, S.mw, S.mh, nullptr, 1, t % V.sw,
-                           t / V.sw, V.sw, V.sh, L, Tr);
+            sky_view_texel<true>(A, R, t_cells + 4 * S.t_off, S.tw, S.th, ms_cells + 4 * S.ms_off, S.mw, S.mh,
+                                 nullptr, 1, t % V.sw, t / V.sw, V.sw, V.sh, L, Tr);
             l_pool[V.sv_off + t] = make_float4(L.x, L.y, L.z, 0.0f);
             tr_pool[V.sv_off + t] = make_float4(Tr.x, Tr.y, Tr.z, 0.0f);
         }
@@ -138,6 +161,8 @@ struct SkyhipLutBatch {
     BatchSetDev* d_sets = nullptr;
     BatchViewDev* d_views = nullptr;
     float4 *t_pool = nullptr, *ms_pool = nullptr, *l_pool = nullptr, *tr_pool = nullptr;
+    float4 *t_cells = nullptr, *ms_cells = nullptr;  // cell-corner copies of the T and MS pools (4x)
+    int max_t_texels = 0, max_ms_texels = 0;
     unsigned* d_ctr = nullptr;
     Tile *d_t_tiles = nullptr, *d_ms_tiles = nullptr, *d_sv_tiles = nullptr;
     int n_t_tiles = 0, n_ms_tiles = 0, n_sv_tiles = 0;
@@ -216,6 +241,10 @@ unsigned grid_for(const SkyhipLutBatch* b, int per_cu, int n_tiles) {
     return unsigned(std::max(1, std::min(b->ctx->cu_count * per_cu, std::max(n_tiles, 1))));
 }
 
+dim3 cells_grid(int max_texels, int n_sets) {
+    return dim3(std::min(div_up(max_texels, 256), 4096u), unsigned(std::min(n_sets, 32768)));
+}
+
 void enqueue_t(SkyhipLutBatch* b, hipStream_t st) {
     const SkyhipLutBatchTuning& t = b->tuning;
     dim3 g(grid_for(b, t.transmittance_blocks_per_cu, b->n_t_tiles));
@@ -228,12 +257,15 @@ void enqueue_t(SkyhipLutBatch* b, hipStream_t st) {
 #undef SKYHIP_T
 }
 
+// The multiple-scattering pass, with the transmittance cell copy it reads built first.
 void enqueue_ms(SkyhipLutBatch* b, hipStream_t st) {
     const SkyhipLutBatchTuning& t = b->tuning;
+    hipLaunchKernelGGL(batch_lut_cells_kernel, cells_grid(b->max_t_texels, b->n_sets), dim3(256), 0, st, b->d_sets,
+                       b->n_sets, 0, b->t_pool, b->t_cells);
     dim3 g(grid_for(b, t.multiscattering_blocks_per_cu, b->n_ms_tiles));
 #define SKYHIP_MS(W)                                                                                              \
     hipLaunchKernelGGL(batch_multiscattering_kernel<W>, g, dim3(64 * W), 0, st, b->d_atm, b->d_sets,            \
-                       b->d_ms_tiles, b->n_ms_tiles, b->t_pool, b->ms_pool, b->d_ctr)
+                       b->d_ms_tiles, b->n_ms_tiles, b->t_cells, b->ms_pool, b->d_ctr)
     if (t.multiscattering_waves == 1) SKYHIP_MS(1);
     else if (t.multiscattering_waves == 2) SKYHIP_MS(2);
     else if (t.multiscattering_waves == 4) SKYHIP_MS(4);
@@ -241,12 +273,16 @@ void enqueue_ms(SkyhipLutBatch* b, hipStream_t st) {
 #undef SKYHIP_MS
 }
 
+// The sky-view pass, with the multiple-scattering cell copy it reads built first (the
+// transmittance one comes from the multiple-scattering pass).
 void enqueue_sv(SkyhipLutBatch* b, hipStream_t st) {
     const SkyhipLutBatchTuning& t = b->tuning;
+    hipLaunchKernelGGL(batch_lut_cells_kernel, cells_grid(b->max_ms_texels, b->n_sets), dim3(256), 0, st, b->d_sets,
+                       b->n_sets, 1, b->ms_pool, b->ms_cells);
     dim3 g(grid_for(b, t.sky_view_blocks_per_cu, b->n_sv_tiles));
 #define SKYHIP_SV(B)                                                                                              \
     hipLaunchKernelGGL(batch_sky_view_kernel<B>, g, dim3(B), 0, st, b->d_atm, b->d_render, b->d_sets, b->d_views, \
-                       b->d_sv_tiles, b->n_sv_tiles, t.sky_view_tile, b->t_pool, b->ms_pool, b->l_pool,          \
+                       b->d_sv_tiles, b->n_sv_tiles, t.sky_view_tile, b->t_cells, b->ms_cells, b->l_pool,        \
                        b->tr_pool, b->d_ctr)
     if (t.sky_view_block == 64) SKYHIP_SV(64);
     else if (t.sky_view_block == 128) SKYHIP_SV(128);
@@ -366,6 +402,8 @@ extern "C" int skyhip_lut_batch_create(SkyhipContext* ctx, const SkyhipLutBatchS
         b->sets.push_back(d);
         t_total += (long long)d.tw * d.th;
         ms_total += (long long)d.mw * d.mh;
+        b->max_t_texels = std::max(b->max_t_texels, d.tw * d.th);
+        b->max_ms_texels = std::max(b->max_ms_texels, d.mw * d.mh);
         b->steps += (unsigned long long)d.tw * d.th * loop_iterations(S.atmosphere.transmittance_steps) +
                     64ull * d.mw * d.mh * loop_iterations(S.atmosphere.multiscattering_steps);
     }
@@ -395,6 +433,8 @@ extern "C" int skyhip_lut_batch_create(SkyhipContext* ctx, const SkyhipLutBatchS
         (rc = alloc(reinterpret_cast<void**>(&b->d_views), sizeof(BatchViewDev) * view_count)) ||
         (rc = alloc(reinterpret_cast<void**>(&b->t_pool), sizeof(float4) * t_total)) ||
         (rc = alloc(reinterpret_cast<void**>(&b->ms_pool), sizeof(float4) * ms_total)) ||
+        (rc = alloc(reinterpret_cast<void**>(&b->t_cells), 4 * sizeof(float4) * t_total)) ||
+        (rc = alloc(reinterpret_cast<void**>(&b->ms_cells), 4 * sizeof(float4) * ms_total)) ||
         (rc = alloc(reinterpret_cast<void**>(&b->l_pool), sizeof(float4) * sv_total)) ||
         (rc = alloc(reinterpret_cast<void**>(&b->tr_pool), sizeof(float4) * sv_total)) ||
         (rc = alloc(reinterpret_cast<void**>(&b->d_ctr), 16)))
@@ -422,7 +462,8 @@ extern "C" int skyhip_lut_batch_destroy(SkyhipLutBatch* b) {
     release_graph(b);
     free_tiles(b);
     for (void* p : {(void*)b->d_atm, (void*)b->d_render, (void*)b->d_sets, (void*)b->d_views, (void*)b->t_pool,
-                    (void*)b->ms_pool, (void*)b->l_pool, (void*)b->tr_pool, (void*)b->d_ctr})
+                    (void*)b->ms_pool, (void*)b->t_cells, (void*)b->ms_cells, (void*)b->l_pool, (void*)b->tr_pool,
+                    (void*)b->d_ctr})
         if (p) (void)hipFree(p);
     if (b->capture) (void)hipStreamDestroy(b->capture);
     delete b;

@@ -141,11 +141,12 @@ __device__ inline MsRay ms_ray(const AtmDev& A, int ix, int iy, int w, int h, in
 // The ground term (ComputeGroundLuminance, :297-306, at |p - c| = R_b) and the reduction over the
 // 64 directions of a wave, in the reference's fixed-order shared-memory tree (:392-427): lane i
 // adds lane i + s for s = 32, 16, ..., 1, which is exactly this shuffle chain.  Valid in lane 0.
+template <bool kCells = false>
 __device__ inline float4 multiscattering_reduce(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
                                                 const MsRay& ray, F3 L, F3 Tr, F3 Lf) {
     if (ray.hit) {
         float mu_g = (ray.s.r * ray.s.mu_s0 + ray.dist * ray.s.nu) / A.rb;
-        F3 sv = sun_visibility(A, T, tw, th, A.rb, 0.0f, 0.0f, mu_g);
+        F3 sv = sun_visibility<kCells>(A, T, tw, th, A.rb, 0.0f, 0.0f, mu_g);
         L += Tr * ((1.0f / kPiF) * clamp01(mu_g)) * A.albedo * sv;
     }
     for (int s = 32; s >= 1; s >>= 1) {
@@ -161,13 +162,15 @@ __device__ inline float4 multiscattering_reduce(const AtmDev& A, const float4* _
 }
 
 // One multiple-scattering texel by the 64 lanes of a wave (lane = sphere direction);
-// the result is valid in lane 0.  Every lane of the wave must call it.
+// the result is valid in lane 0.  Every lane of the wave must call it.  kCells: T is the
+// transmittance LUT's cell-corner copy.
+template <bool kCells = false>
 __device__ inline float4 multiscattering_texel(const AtmDev& A, const float4* __restrict__ T, int tw, int th,
                                                int ix, int iy, int w, int h, int lane) {
     const MsRay ray = ms_ray(A, ix, iy, w, h, lane);
     F3 L, Tr, Lf;
-    scattered_luminance<true>(A, T, tw, th, nullptr, 0, 0, ray.s, ray.dist, A.ms_steps, 0.5f, L, Tr, Lf);
-    return multiscattering_reduce(A, T, tw, th, ray, L, Tr, Lf);
+    scattered_luminance<true, kCells>(A, T, tw, th, nullptr, 0, 0, ray.s, ray.dist, A.ms_steps, 0.5f, L, Tr, Lf);
+    return multiscattering_reduce<kCells>(A, T, tw, th, ray, L, Tr, Lf);
 }
 
 // The renderer's block as the kernels see it.
@@ -253,7 +256,9 @@ __device__ inline SkyViewRay sky_view_ray(const AtmDev& A, const RenderDev& R, c
     return ray;
 }
 
-// One sky-view texel (:74-186): luminance and transmittance.
+// One sky-view texel (:74-186): luminance and transmittance (kCells: T and MS are cell-corner
+// copies).
+template <bool kCells = false>
 __device__ inline void sky_view_texel(const AtmDev& A, const RenderDev& R, const float4* __restrict__ T, int tw,
                                       int th, const float4* __restrict__ MS, int mw, int mh,
                                       const uint16_t* __restrict__ blue_noise, int bn_size, int ix, int iy, int w,
@@ -263,7 +268,8 @@ __device__ inline void sky_view_texel(const AtmDev& A, const RenderDev& R, const
     Tr = f3(1.0f);
     F3 Lf;
     if (ray.dist > 0.0f)
-        scattered_luminance<false>(A, T, tw, th, MS, mw, mh, ray.s, ray.dist, R.sv_steps, ray.start_i, L, Tr, Lf);
+        scattered_luminance<false, kCells>(A, T, tw, th, MS, mw, mh, ray.s, ray.dist, R.sv_steps, ray.start_i, L, Tr,
+                                           Lf);
 }
 
 // ProjectiveMul(inv_view_projection, (x, y, z)) - camera, normalised; the
