@@ -59,6 +59,8 @@ struct SkyhipContext {
     int froxel_w = 0, froxel_h = 0, froxel_d = 0;
     // The raymarch's dual-lobe phase (the reference's constant lobes by default).
     SkyhipCloudPhase phase = {-0.15f, 2.16f, 0.85f, 0.0f};
+    // The persistent path tracer's tile counter (device memory, allocated on first use).
+    unsigned* pt_queue = nullptr;
 };
 
 namespace skyhip {

@@ -76,6 +76,7 @@ struct PtArgs {
     float4* accum;
     uint8_t* mask;
     unsigned long long* counters;  // samples, density lookups, majorant segments, path vertices, shadow rays
+    unsigned* queue;               // path_trace_refill_kernel's tile counter (zero at launch)
 };
 
 struct PtCount {
@@ -212,8 +213,9 @@ __device__ __forceinline__ float sigma_walk(const PtArgs& a, const BrickWalk& w,
     return g * a.mat.voxel.uDensity;
 }
 
-// Delta tracking over [t0, t1] (:182-196); returns true with t at a real collision.
-template <int kMat, bool kBricks, bool kCount, bool kCells>
+// Delta tracking over [t0, t1] (:182-196); returns true with t at a real collision.  kFlat
+// selects the loop form of the brick walk (below); both make the same draws.
+template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
 __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, float t1, float& t, PtCount& wc) {
     if (!kBricks) {
         if (a.sigma_t_max <= 0.0f) return false;
@@ -230,32 +232,71 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
     BrickWalk w;
     t = t0;
     if (!w.begin(a, o, d, t, t1)) return false;
-    int it = 0;
-    for (int seg = 0; seg < kMaxSegments; ++seg) {
-        const float seg_end = w.segment_end();
-        const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
-        if (kCount) wc.segments++;
-        if (mu > 0.0f) {
-            const float inv_mu = 1.0f / mu;
-            for (; it < kMaxTrackingSteps; ++it) {
-                float tt = t + exp_sample(rng.next()) * inv_mu;
-                if (!(tt < seg_end)) break;
+    if constexpr (!kFlat) {
+        // A loop over bricks around a loop over the tentative collisions in each.
+        int it = 0;
+        for (int seg = 0; seg < kMaxSegments; ++seg) {
+            const float seg_end = w.segment_end();
+            const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+            if (kCount) wc.segments++;
+            if (mu > 0.0f) {
+                const float inv_mu = 1.0f / mu;
+                for (; it < kMaxTrackingSteps; ++it) {
+                    float tt = t + exp_sample(rng.next()) * inv_mu;
+                    if (!(tt < seg_end)) break;
+                    t = tt;
+                    float s = sigma_walk<kCells>(a, w, t);
+                    if (kCount) wc.lookups++;
+                    if (rng.next() < s * inv_mu) return true;
+                }
+                if (it >= kMaxTrackingSteps) return false;
+            }
+            if (!(seg_end < w.t_end)) return false;
+            t = seg_end;
+            w.advance(a);
+        }
+        return false;
+    }
+    // One loop over tentative collisions and brick crossings together: an iteration draws the next
+    // tentative collision in the current brick and looks it up, or, when the draw falls past the
+    // brick, moves to the next brick.  Lanes of a wave then differ only in how many iterations
+    // they take, where the nested form makes the wave wait in every brick for its lane with the
+    // most collisions there, but every iteration carries both bodies.  Same draws in the same
+    // order as the nested form (a draw past the brick is discarded), so the same samples.
+    float seg_end = w.segment_end();
+    float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+    float inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+    if (kCount) wc.segments++;
+    int it = 0, seg = 0;
+    for (;;) {
+        bool cross = true;
+        if (mu > 0.0f && it < kMaxTrackingSteps) {
+            const float tt = t + exp_sample(rng.next()) * inv_mu;
+            if (tt < seg_end) {
+                cross = false;
                 t = tt;
-                float s = sigma_walk<kCells>(a, w, t);
+                ++it;
+                const float s = sigma_walk<kCells>(a, w, t);
                 if (kCount) wc.lookups++;
                 if (rng.next() < s * inv_mu) return true;
             }
-            if (it >= kMaxTrackingSteps) return false;
         }
-        if (!(seg_end < w.t_end)) return false;
-        t = seg_end;
-        w.advance(a);
+        if (cross) {
+            if (mu > 0.0f && it >= kMaxTrackingSteps) return false;
+            if (!(seg_end < w.t_end) || ++seg >= kMaxSegments) return false;
+            t = seg_end;
+            w.advance(a);
+            seg_end = w.segment_end();
+            mu = a.majorants[w.brick(a)] * a.majorant_scale;
+            inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+            if (kCount) wc.segments++;
+        }
     }
-    return false;
 }
 
-// TransmittanceEstimation (:135-151): ratio tracking toward the sun on a copy of the state.
-template <int kMat, bool kBricks, bool kCount, bool kCells>
+// TransmittanceEstimation (:135-151): ratio tracking toward the sun on a copy of the state
+// (kFlat as in delta_track).
+template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
 __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) {
     float t0, t1;
     region(a, o, d, t0, t1);
@@ -273,23 +314,60 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
         }
         return clamp01(tr);
     }
+    // Two tentative collisions per lookup iteration, their lookups issued together: the second is
+    // drawn ahead, and when it falls past the brick (or the step bound) the state goes back to
+    // before its draw, so the next iteration draws it again, overshoots and crosses as the
+    // one-at-a-time loop does.  Same draws, same products, same order, in either loop form.
     BrickWalk w;
     float t = t0;
     if (!w.begin(a, o, d, t, t1)) return tr;
-    int it = 0;
-    for (int seg = 0; seg < kMaxSegments; ++seg) {
-        const float seg_end = w.segment_end();
-        const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
-        if (kCount) wc.segments++;
-        if (mu > 0.0f) {
-            const float inv_mu = 1.0f / mu;
-            // Two tentative collisions per iteration, their lookups issued together: the second
-            // is drawn ahead, and when it falls past the segment (or the step bound) the state
-            // goes back to before its draw, so the next iteration draws it again, overshoots and
-            // leaves as the one-at-a-time loop does.  Same draws, same products, same order.
-            while (it < kMaxTrackingSteps) {
-                const float tt = t + exp_sample(rng.next()) * inv_mu;
-                if (!(tt < seg_end)) break;
+    if constexpr (!kFlat) {
+        int it = 0;
+        for (int seg = 0; seg < kMaxSegments; ++seg) {
+            const float seg_end = w.segment_end();
+            const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+            if (kCount) wc.segments++;
+            if (mu > 0.0f) {
+                const float inv_mu = 1.0f / mu;
+                while (it < kMaxTrackingSteps) {
+                    const float tt = t + exp_sample(rng.next()) * inv_mu;
+                    if (!(tt < seg_end)) break;
+                    const uint32_t before_second = rng.s;
+                    const float tt2 = tt + exp_sample(rng.next()) * inv_mu;
+                    const bool two = tt2 < seg_end && it + 1 < kMaxTrackingSteps;
+                    const float s = sigma_walk<kCells>(a, w, tt);
+                    const float s2 = sigma_walk<kCells>(a, w, two ? tt2 : tt);
+                    tr *= 1.0f - fmaxf(0.0f, s * inv_mu);
+                    if (two) {
+                        tr *= 1.0f - fmaxf(0.0f, s2 * inv_mu);
+                        t = tt2;
+                        it += 2;
+                    } else {
+                        rng.s = before_second;
+                        t = tt;
+                        it += 1;
+                    }
+                    if (kCount) wc.lookups += two ? 2u : 1u;
+                }
+                if (it >= kMaxTrackingSteps || tr == 0.0f) break;
+            }
+            if (!(seg_end < w.t_end)) break;
+            t = seg_end;
+            w.advance(a);
+        }
+        return clamp01(tr);
+    }
+    float seg_end = w.segment_end();
+    float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+    float inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+    if (kCount) wc.segments++;
+    int it = 0, seg = 0;
+    for (;;) {
+        bool cross = true;
+        if (mu > 0.0f && it < kMaxTrackingSteps) {
+            const float tt = t + exp_sample(rng.next()) * inv_mu;
+            if (tt < seg_end) {
+                cross = false;
                 const uint32_t before_second = rng.s;
                 const float tt2 = tt + exp_sample(rng.next()) * inv_mu;
                 const bool two = tt2 < seg_end && it + 1 < kMaxTrackingSteps;
@@ -307,11 +385,17 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
                 }
                 if (kCount) wc.lookups += two ? 2u : 1u;
             }
-            if (it >= kMaxTrackingSteps || tr == 0.0f) break;
         }
-        if (!(seg_end < w.t_end)) break;
-        t = seg_end;
-        w.advance(a);
+        if (cross) {
+            if (mu > 0.0f && (it >= kMaxTrackingSteps || tr == 0.0f)) break;
+            if (!(seg_end < w.t_end) || ++seg >= kMaxSegments) break;
+            t = seg_end;
+            w.advance(a);
+            seg_end = w.segment_end();
+            mu = a.majorants[w.brick(a)] * a.majorant_scale;
+            inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+            if (kCount) wc.segments++;
+        }
     }
     return clamp01(tr);
 }
@@ -393,71 +477,126 @@ __device__ __forceinline__ const PtArgs& kernel_args() {
     return *(const PtArgs*)p;
 }
 
-// Trace (:160-249)
-template <int kMat, bool kBricks, bool kCount, bool kCells>
-__device__ F3 trace(Rng& rng, F3 view, bool& has_scattered, float& scattered_t, PtCount& wc) {
-    const PtArgs& a0 = kernel_args();
-    const PtArgs& a = a0;
-    F3 L = f3(0.0f), thr = f3(1.0f);
-    has_scattered = false;
-    scattered_t = 0.0f;
-    F3 o = a.cam, d = view;
+// One path between bounces: position, direction, throughput, radiance so far, and whether and
+// where (distance from the camera) it first scattered.
+struct PathState {
+    F3 o, d, thr, L;
+    float scattered_t;
+    int istep;
+    bool has_scattered;
+};
+
+// Trace's set-up (:160-181): the camera ray along `view`, clipped to the cloud region.  False
+// when the path makes no bounce (it misses the region, or the bounce limit is 0).
+__device__ __forceinline__ bool begin_path(const PtArgs& a, F3 view, PathState& p) {
+    p.L = f3(0.0f);
+    p.thr = f3(1.0f);
+    p.has_scattered = false;
+    p.scattered_t = 0.0f;
+    p.istep = 0;
+    p.o = a.cam;
+    p.d = view;
     float c0, c1;
-    region(a, o, d, c0, c1);
-    if (c0 >= c1) return L;
-    o = o + d * c0;
-    for (int istep = 0; istep < a0.max_bounces && fmaxf(thr.x, fmaxf(thr.y, thr.z)) > 0.0f; ++istep) {
-        const PtArgs& a = kernel_args();
-        float t0, t1;
-        region(a, o, d, t0, t1);
-        if (t0 >= t1) break;
-        float t;
-        if (!delta_track<kMat, kBricks, kCount, kCells>(a, rng, o, d, t0, t1, t, wc)) {
-            if (a.env == SKYHIP_ENV_OFF || !has_scattered) break;
-            if (a.env == SKYHIP_ENV_CONST_ENVIRONMENT_MAP) {
-                L += thr * environment(a, d);
-                break;
-            }
-            float q = o.x * o.x + o.y * o.y + o.z * (o.z + 2.0f * a.R) + (a.R - a.A.rb) * (a.R + a.A.rb);
-            float r = sqrtf(fmaxf(q + a.A.rb * a.A.rb, 0.0f));
-            float alt = q / (r + a.A.rb);
-            float mu = (d.x * o.x + d.y * o.y + d.z * (o.z + a.R)) / r;
-            if (!ray_hits_ground(a.A, r, alt, mu)) {
-                L += thr * environment(a, d);
-                break;
-            }
-            o = o + d * distance_to_bottom(a.A, r, alt, mu);
-            F3 N = F3{o.x, o.y, o.z + a.R};
-            N = N * (1.0f / length3(N));
-            if (kCount) wc.vertices++;
-            F3 bsdf = a.A.albedo * ((1.0f / kPiF) * dot(N, a.sun));
-            L += thr * (ratio_track<kMat, kBricks, kCount, kCells>(a, rng, o, a.sun, wc) * sun_illuminance(a, o) * bsdf);
-            if (a.env == SKYHIP_ENV_GROUND_SINGLE_BOUNCE) break;
-            d = lambert_sample(rng, N);
-            thr *= a.A.albedo;
-        } else {
-            if (!has_scattered) scattered_t = length3(o - a.cam);  // the segment start, as :232-233
-            has_scattered = true;
-            o = o + d * t;
-            if (kCount) wc.vertices++;
-            // A real collision scatters with probability albedo (the reference's is 1): the sun
-            // estimate at this vertex and the continuation both carry it.
-            thr = thr * a.albedo;
-            float ph = pt_phase(a, dot(d, a.sun));
-            L += thr * (ratio_track<kMat, kBricks, kCount, kCells>(a, rng, o, a.sun, wc) * ph * sun_illuminance(a, o));
-            float vop;
-            d = hg_sample(a, rng, d, vop);
-            thr = thr * vop;
+    region(a, p.o, p.d, c0, c1);
+    if (c0 >= c1) return false;
+    p.o = p.o + p.d * c0;
+    return a.max_bounces > 0;
+}
+
+// One iteration of Trace's bounce loop (:182-247).  False when the path ends in it (it leaves
+// the region, escapes, or reaches the bounce limit or zero throughput).
+template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
+__device__ __forceinline__ bool bounce(Rng& rng, PathState& p, PtCount& wc) {
+    const PtArgs& a = kernel_args();
+    float t0, t1;
+    region(a, p.o, p.d, t0, t1);
+    if (t0 >= t1) return false;
+    float t;
+    if (!delta_track<kMat, kBricks, kCount, kCells, kFlat>(a, rng, p.o, p.d, t0, t1, t, wc)) {
+        if (a.env == SKYHIP_ENV_OFF || !p.has_scattered) return false;
+        if (a.env == SKYHIP_ENV_CONST_ENVIRONMENT_MAP) {
+            p.L += p.thr * environment(a, p.d);
+            return false;
         }
+        const F3 o = p.o, d = p.d;
+        float q = o.x * o.x + o.y * o.y + o.z * (o.z + 2.0f * a.R) + (a.R - a.A.rb) * (a.R + a.A.rb);
+        float r = sqrtf(fmaxf(q + a.A.rb * a.A.rb, 0.0f));
+        float alt = q / (r + a.A.rb);
+        float mu = (d.x * o.x + d.y * o.y + d.z * (o.z + a.R)) / r;
+        if (!ray_hits_ground(a.A, r, alt, mu)) {
+            p.L += p.thr * environment(a, d);
+            return false;
+        }
+        p.o = o + d * distance_to_bottom(a.A, r, alt, mu);
+        F3 N = F3{p.o.x, p.o.y, p.o.z + a.R};
+        N = N * (1.0f / length3(N));
+        if (kCount) wc.vertices++;
+        F3 bsdf = a.A.albedo * ((1.0f / kPiF) * dot(N, a.sun));
+        p.L += p.thr * (ratio_track<kMat, kBricks, kCount, kCells, kFlat>(a, rng, p.o, a.sun, wc) * sun_illuminance(a, p.o) *
+                        bsdf);
+        if (a.env == SKYHIP_ENV_GROUND_SINGLE_BOUNCE) return false;
+        p.d = lambert_sample(rng, N);
+        p.thr *= a.A.albedo;
+    } else {
+        if (!p.has_scattered) p.scattered_t = length3(p.o - a.cam);  // the segment start, as :232-233
+        p.has_scattered = true;
+        p.o = p.o + p.d * t;
+        if (kCount) wc.vertices++;
+        // A real collision scatters with probability albedo (the reference's is 1): the sun
+        // estimate at this vertex and the continuation both carry it.
+        p.thr = p.thr * a.albedo;
+        float ph = pt_phase(a, dot(p.d, a.sun));
+        p.L += p.thr * (ratio_track<kMat, kBricks, kCount, kCells, kFlat>(a, rng, p.o, a.sun, wc) * ph * sun_illuminance(a, p.o));
+        float vop;
+        p.d = hg_sample(a, rng, p.d, vop);
+        p.thr = p.thr * vop;
     }
-    return L;
+    return ++p.istep < a.max_bounces && fmaxf(p.thr.x, fmaxf(p.thr.y, p.thr.z)) > 0.0f;
+}
+
+// Trace (:160-249)
+template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
+__device__ F3 trace(Rng& rng, F3 view, bool& has_scattered, float& scattered_t, PtCount& wc) {
+    PathState p;
+    if (begin_path(kernel_args(), view, p))
+        while (bounce<kMat, kBricks, kCount, kCells, kFlat>(rng, p, wc)) {
+        }
+    has_scattered = p.has_scattered;
+    scattered_t = p.scattered_t;
+    return p.L;
+}
+
+// A finished sample's aerial perspective (GetAerialPerspective, VolumetricCloudCommon.glsl:81-98,
+// and the froxel visibility, :277) and its addition to the pixel's accumulator (:266-284).
+// (u, v) is the pixel centre in [0, 1]^2 and view_z the z of its view direction.
+__device__ __forceinline__ float4 add_sample(const PtArgs& a, float u, float v, float view_z, F3 L, bool hs, float st,
+                                             float4 acc) {
+    if (hs) {
+        float r = a.cam.z + a.R, mu = view_z;
+        float t = st;
+        if (r > a.A.rt) {
+            float disc = r * r * (mu * mu - 1.0f) + a.A.rt * a.A.rt;
+            t = (mu < 0.0f && disc >= 0.0f) ? t - fmaxf(-r * mu - sqrtf(disc), 0.0f) : 0.0f;
+        }
+        float w = sqrtf(t / a.ap_max_distance);
+        float fx = u * float(a.aw - 1), fy = v * float(a.ah - 1), fz = w * float(a.ad - 1);
+        F3 apt = lut3d_trilinear(a.ap_t, a.aw, a.ah, a.ad, fx, fy, fz);
+        F3 apl = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, fx, fy, fz);
+        if (a.froxel) apl = apl * ray_scatter_visibility(a.froxel, a.fw, a.fh, a.fd, u, v, st, a.inv_froxel_max_distance);
+        L = L * apt + apl;
+    }
+    acc.x += L.x;
+    acc.y += L.y;
+    acc.z += L.z;
+    acc.w += hs ? 0.0f : 1.0f;
+    return acc;
 }
 
 // Five waves per SIMD (96 VGPRs; the compiler spills the rest of the 154 the path state would
 // take to scratch): more waves hide more of the tracking's fetch latency.  Measured on the
 // 1024^2 brick-majorant workload: 3 waves (no cap) 14.2 ms per frame, 4 waves 12.0, 5 waves
 // 11.2, 6 waves 11.4, 7 and 8 waves 12.0-12.2; every variant gives the same image.
-template <int kMat, bool kBricks, bool kCount, bool kCells>
+template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void path_trace_kernel(PtArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int x = a.region[0] + blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
@@ -477,26 +616,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
             rng.s = rng.hash(seed_xy + a.first_frame + uint32_t(f));  // :260
             bool hs;
             float st;
-            F3 L = trace<kMat, kBricks, kCount, kCells>(rng, view, hs, st, wc);
-            if (hs) {  // GetAerialPerspective (VolumetricCloudCommon.glsl:81-98) and the froxel visibility (:277)
-                float r = a.cam.z + a.R, mu = view.z;
-                float t = st;
-                if (r > a.A.rt) {
-                    float disc = r * r * (mu * mu - 1.0f) + a.A.rt * a.A.rt;
-                    t = (mu < 0.0f && disc >= 0.0f) ? t - fmaxf(-r * mu - sqrtf(disc), 0.0f) : 0.0f;
-                }
-                float w = sqrtf(t / a.ap_max_distance);
-                float fx = u * float(a.aw - 1), fy = v * float(a.ah - 1), fz = w * float(a.ad - 1);
-                F3 apt = lut3d_trilinear(a.ap_t, a.aw, a.ah, a.ad, fx, fy, fz);
-                F3 apl = lut3d_trilinear(a.ap_l, a.aw, a.ah, a.ad, fx, fy, fz);
-                if (a.froxel)
-                    apl = apl * ray_scatter_visibility(a.froxel, a.fw, a.fh, a.fd, u, v, st, a.inv_froxel_max_distance);
-                L = L * apt + apl;
-            }
-            acc.x += L.x;
-            acc.y += L.y;
-            acc.z += L.z;
-            acc.w += hs ? 0.0f : 1.0f;
+            F3 L = trace<kMat, kBricks, kCount, kCells, kFlat>(rng, view, hs, st, wc);
+            acc = add_sample(a, u, v, view.z, L, hs, st, acc);
         }
         a.accum[pix] = acc;
         if (a.mask) a.mask[pix] = 1;
@@ -510,6 +631,87 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
             if (lane == 0) atomicAdd(&a.counters[i], s);
         }
     }
+}
+
+// The per-frame form of path_trace_kernel, persistent, with lanes refilled at every bounce.  The
+// frame's pixels form 8x8 tiles; a wave takes tiles from a global counter and hands their pixels
+// to its lanes.  After each bounce the lanes whose path has ended are found with a ballot, ranked
+// with a population count of the lower lanes, and given the next pixels of the wave's tile (a new
+// tile when it runs out), so a lane starts its next path while the rest of the wave is still
+// tracking, where path_trace_kernel's lanes step through the bounces together and a wave lasts as
+// long as its longest path.  Every path is the same computation with the same random sequence,
+// and each pixel gets one path per launch, so the accumulator receives the same additions.  The
+// loop ends when the counter has passed the last tile and no lane has a path.
+template <int kMat, bool kBricks, bool kCells, bool kFlat>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void path_trace_refill_kernel(PtArgs) {
+    const PtArgs& a0 = kernel_args();
+    const int lane = threadIdx.x & 63;
+    const int rx = a0.region[0], ry = a0.region[1], rx1 = a0.region[2], ry1 = a0.region[3];
+    const int tiles_x = (rx1 - rx + 7) >> 3;
+    const int n_tiles = tiles_x * ((ry1 - ry + 7) >> 3);
+    Rng rng;
+    rng.pcg = a0.prng;
+    PathState p;
+    PtCount wc;
+    int px = 0, py = 0;
+    float view_z = 0.0f;
+    bool alive = false, exhausted = false;
+    int next = 0, end = 0;  // the wave's unassigned pixels: items [next, end) of the frame (64 per tile)
+    for (;;) {
+        while (!exhausted) {  // wave-uniform: next, end and exhausted come from ballots and lane 0
+            const unsigned long long idle = __ballot(!alive);
+            if (idle == 0ull) break;
+            if (next >= end) {
+                int tile = 0;
+                if (lane == 0) tile = int(atomicAdd(a0.queue, 1u));
+                tile = __shfl(tile, 0);
+                if (tile >= n_tiles) {
+                    exhausted = true;
+                    break;
+                }
+                next = tile * 64;
+                end = next + 64;
+            }
+            const int avail = end - next;
+            const int rank = __popcll(idle & ((1ull << lane) - 1ull));
+            if (!alive && rank < avail) {
+                const int item = next + rank, tile = item >> 6, j = item & 63;
+                const int x = rx + (tile % tiles_x) * 8 + (j & 7), y = ry + (tile / tiles_x) * 8 + (j >> 3);
+                if (x < rx1 && y < ry1) {
+                    const PtArgs& a = kernel_args();
+                    const float u = (float(x) + 0.5f) / float(a.width), v = (float(y) + 0.5f) / float(a.height);
+                    float frag_dist;
+                    const F3 view = unproject(a.inv_mvp, u * 2.0f - 1.0f, v * 2.0f - 1.0f, 1.0f, a.cam, &frag_dist);
+                    rng.s = rng.hash(rng.hash(rng.hash(uint32_t(x)) + uint32_t(y)) + a.first_frame);  // :260
+                    px = x;
+                    py = y;
+                    view_z = view.z;
+                    alive = begin_path(a, view, p);
+                    if (!alive) {
+                        const long long pix = (long long)y * a.width + x;
+                        a.accum[pix] = add_sample(a, u, v, view_z, p.L, p.has_scattered, p.scattered_t, a.accum[pix]);
+                        if (a.mask) a.mask[pix] = 1;
+                    }
+                }
+            }
+            next += min(__popcll(idle), avail);
+        }
+        if (__ballot(alive) == 0ull) break;  // only once the tiles are exhausted
+        if (alive) {
+            alive = bounce<kMat, kBricks, false, kCells, kFlat>(rng, p, wc);
+            if (!alive) {
+                const PtArgs& a = kernel_args();
+                const float u = (float(px) + 0.5f) / float(a.width), v = (float(py) + 0.5f) / float(a.height);
+                const long long pix = (long long)py * a.width + px;
+                a.accum[pix] = add_sample(a, u, v, view_z, p.L, p.has_scattered, p.scattered_t, a.accum[pix]);
+                if (a.mask) a.mask[pix] = 1;
+            }
+        }
+    }
+}
+
+__global__ void queue_reset_kernel(unsigned* q) {
+    if (threadIdx.x == 0) q[0] = 0u;
 }
 
 // Display pass (:288-296).
@@ -564,36 +766,61 @@ __global__ __launch_bounds__(256) void cells_kernel(const uint16_t* __restrict__
     out[c] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
+// Workgroups of path_trace_refill_kernel per CU: 5 of 4 waves, its occupancy at the 5-wave cap.
+constexpr int kRefillBlocksPerCu = 5;
+// Brick-majorant frames with the cell-corner copy run on one of two forms, chosen by the brick
+// edge (same samples either way).  Measured on the 1024^2, 512^3-grid workload (ms per frame,
+// one box, per-pixel kernel / refill kernel, nested / flat tracking loops): 16^3 bricks 8.64-8.70
+// per-pixel nested, 7.96-8.01 per-pixel flat, 8.46-8.52 refill nested, 8.09-8.18 refill flat;
+// 8^3 bricks 9.23-9.47, 9.29, 8.59-8.67, 9.37-9.51.  Small bricks make crossings outnumber
+// lookups (240 M against 82 M per frame at 8^3), which the flat loop pays for in every
+// iteration; there the nested loops with lanes refilled per bounce win.
+constexpr int kRefillMaxBrick = 8;
+
 template <bool kBricks, bool kCount>
-static void launch_path_trace(hipStream_t st, const PtArgs& a) {
+static void launch_path_trace(hipStream_t st, const PtArgs& a, int cu_count) {
     dim3 grid(div_up(a.region[2] - a.region[0], 16), div_up(a.region[3] - a.region[1], 16));
     switch (a.mat.kind) {
         case SKYHIP_MATERIAL_DEFAULT0:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT0, false, kCount, false>), grid, dim3(256), 0, st,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT0, false, kCount, false, false>), grid, dim3(256), 0, st,
                                a);
             break;
         case SKYHIP_MATERIAL_DEFAULT1:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1, false, kCount, false>), grid, dim3(256), 0, st,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1, false, kCount, false, false>), grid, dim3(256), 0, st,
                                a);
             break;
         case SKYHIP_MATERIAL_DEFAULT1_WORLEY:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY, false, kCount, false>), grid, dim3(256), 0,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_DEFAULT1_WORLEY, false, kCount, false, false>), grid, dim3(256), 0,
                                st, a);
             break;
         case SKYHIP_MATERIAL_MINIMAL:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_MINIMAL, false, kCount, false>), grid, dim3(256), 0, st,
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_MINIMAL, false, kCount, false, false>), grid, dim3(256), 0, st,
                                a);
             break;
         case SKYHIP_MATERIAL_VOXEL:
-            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL, false, kCount, false>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL, false, kCount, false, false>), grid, dim3(256), 0, st, a);
             break;
         default:
+            if constexpr (kBricks && !kCount) {
+                if (a.cells && a.brick <= kRefillMaxBrick) {
+                    // One persistent launch per frame, its tile counter cleared by a kernel before it.
+                    for (int f = 0; f < a.frames; ++f) {
+                        PtArgs af = a;
+                        af.first_frame = a.first_frame + uint32_t(f);
+                        af.frames = 1;
+                        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, st, a.queue);
+                        hipLaunchKernelGGL((path_trace_refill_kernel<SKYHIP_MATERIAL_VOXEL_F16, true, true, false>),
+                                           dim3(cu_count * kRefillBlocksPerCu), dim3(256), 0, st, af);
+                    }
+                    break;
+                }
+            }
             if (kBricks && a.cells)
-                hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount, kBricks>), grid,
-                                   dim3(256), 0, st, a);
+                hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount, kBricks, kBricks>),
+                                   grid, dim3(256), 0, st, a);
             else
-                hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount, false>), grid,
-                                   dim3(256), 0, st, a);
+                hipLaunchKernelGGL((path_trace_kernel<SKYHIP_MATERIAL_VOXEL_F16, kBricks, kCount, false, kBricks>),
+                                   grid, dim3(256), 0, st, a);
             break;
     }
 }
@@ -771,16 +998,19 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
     a.mask = rendered_mask;
     a.counters = work_counters;
     if (region[2] == region[0] || region[3] == region[1] || frames == 0) return SKYHIP_OK;
+    if (!ctx->pt_queue) SKYHIP_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->pt_queue), 256));
+    a.queue = ctx->pt_queue;
+    const int cus = std::max(ctx->cu_count, 1);
     if (bricks) {
         if (work_counters)
-            launch_path_trace<true, true>(ctx->stream, a);
+            launch_path_trace<true, true>(ctx->stream, a, cus);
         else
-            launch_path_trace<true, false>(ctx->stream, a);
+            launch_path_trace<true, false>(ctx->stream, a, cus);
     } else {
         if (work_counters)
-            launch_path_trace<false, true>(ctx->stream, a);
+            launch_path_trace<false, true>(ctx->stream, a, cus);
         else
-            launch_path_trace<false, false>(ctx->stream, a);
+            launch_path_trace<false, false>(ctx->stream, a, cus);
     }
     SKYHIP_LAUNCH_CHECK("path_trace_kernel");
     return SKYHIP_OK;

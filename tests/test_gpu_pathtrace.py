@@ -120,6 +120,45 @@ def test_cell_corner_copy_and_identical_samples(ctx):
     assert np.array_equal(a, b)
 
 
+def test_refill_kernel_matches_per_pixel_kernel(ctx):
+    """With 8^3 bricks and the cell-corner copy bound, brick-majorant frames run on the persistent
+    kernel that refills finished lanes after every bounce, with nested tracking loops; without
+    the copy, on the one-lane-per-pixel kernel with flat tracking loops.  A three-frame call over
+    a region off the origin whose sides are not multiples of the 8x8 tiles gives bit-identical
+    accumulators and the same rendered mask, and pixels outside the region stay untouched."""
+    w = cases.small_grid_workload()
+    w["brick"] = 8
+    refill = PathTraceScene(ctx, 30, 22, workload=w, bricks=True)
+    assert refill.inputs.cell_corners and refill.grid.brick == 8
+    plain_grid = VoxelGrid(ctx, refill.grid.dims, brick=8, cells=False)
+    plain_grid.upload(refill.grid.download())
+    w2 = cases.small_grid_workload()
+    w2["brick"] = 8
+    per_pixel = PathTraceScene(ctx, 30, 22, workload=w2, grid=plain_grid, bricks=True)
+    assert not per_pixel.inputs.cell_corners
+    region = (3, 5, 27, 19)
+    out = []
+    for scene in (refill, per_pixel):
+        n = scene.width * scene.height
+        seed = np.full((n, 4), 0.25, np.float32)
+        acc = ctx.upload(seed)
+        mask = ctx.upload(np.zeros(n, np.uint8))
+        path_trace(ctx, scene.atmosphere.buffer, scene.renderer.buffer, scene.common, scene.material,
+                   scene.tracer.params, scene.inputs, region, 7, 3, acc, mask, scene.width, scene.height)
+        out.append((acc.download(np.float32, (scene.height, scene.width, 4)),
+                    mask.download(np.uint8, (scene.height, scene.width))))
+        acc.free()
+        mask.free()
+    (a, ma), (b, mb) = out
+    inside = np.zeros((22, 30), bool)
+    inside[5:19, 3:27] = True
+    print(f"refill vs per-pixel: identical {np.array_equal(a, b)}, "
+          f"{np.mean(a[inside][:, 3] < 3.25):.3f} of region pixels scattered in some frame")
+    assert np.array_equal(a, b) and np.array_equal(ma, mb)
+    assert np.all(ma[inside] == 1) and np.all(ma[~inside] == 0)
+    assert np.all(a[~inside] == 0.25)
+
+
 def test_single_majorant_samples_match_oracle(ctx):
     w = cases.small_grid_workload()
     scene = PathTraceScene(ctx, 32, 32, workload=w, bricks=False)
