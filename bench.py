@@ -61,7 +61,8 @@ FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
     "cloud_raymarch_kernel<5,false>": (36115.9, 25697.5),
-    "path_trace_kernel<4,true,false,true>": (3.9300e6, 9.8996e4),
+    "path_trace_kernel<4,true,false,true,true>": (4.1901e6, 1.2513e5),
+    "path_trace_refill_kernel<4,true,true,false>": (3.5157e6, 9.346e4),
 }
 
 
@@ -214,8 +215,9 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
     # Algorithmic bytes per frame: 8 FP16 texels per trilinear density lookup, one FP32 majorant
     # per brick segment, and the RGBA32F accumulator read and written once per pixel.
     alg_bytes = counts["lookups"] * 16 + counts["segments"] * 4 + n * 32
-    traffic = profiled_traffic("path_trace_kernel<4,true,false,true>")
-    roofline = {"bound": "hbm", "unit": "GB/s", "kernel": "path_trace_kernel<4,true,false,true>",
+    alg8 = counts8["lookups"] * 16 + counts8["segments"] * 4 + n * 32
+    traffic = profiled_traffic("path_trace_kernel<4,true,false,true,true>")
+    roofline = {"bound": "hbm", "unit": "GB/s", "kernel": "path_trace_kernel<4,true,false,true,true>",
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "achieved": round(alg_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                 "frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -232,7 +234,10 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
         "glookups_per_s": round(counts["lookups"] / (ms * 1e-3) / 1e9, 3),
         "gvertices_per_s": round(counts["vertices"] / (ms * 1e-3) / 1e9, 4),
         "brick8": {"ms_per_frame": round(ms8, 4), "frames_timed": frames8,
-                   "density_lookups_per_frame": counts8["lookups"], "majorant_segments_per_frame": counts8["segments"]},
+                   "density_lookups_per_frame": counts8["lookups"], "majorant_segments_per_frame": counts8["segments"],
+                   "kernel": "path_trace_refill_kernel<4,true,true,false>", "algorithmic_bytes_per_launch": alg8,
+                   "hbm_frac": round(alg8 / (ms8 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "traffic": profiled_traffic("path_trace_refill_kernel<4,true,true,false>")},
         "single_majorant_ms_per_frame": round(ref_ms, 3),
         "single_majorant_density_lookups_per_frame": ref_counts["lookups"],
         "brick_speedup_vs_single_majorant": round(ref_ms / ms, 2),
