@@ -284,6 +284,18 @@ def test_raymarch_noise_cloud_map_and_reference_blue_noise(ctx, blue_noise_64):
     _compare(scene, 5e-4, "Material1 NoiseGen cloud map, blue-noise fixture, frame 7", l2_tol=1e-2)
 
 
+@pytest.mark.parametrize("shadow_steps", [0.5, 20.0])
+def test_raymarch_light_step_counts_parity(ctx, shadow_steps):
+    """Light marches outside the wave-cooperative form: shadow steps below 1 (SampleShadow's
+    counter makes no step, so the sun is unshadowed) and shadow steps 20 (19 light steps from
+    the shader's FP32 counter, more than the 16 the staging holds, so each lit lane marches
+    toward the sun itself)."""
+    w = presets.cloud_workload()
+    w["cloud"].shadow_steps = shadow_steps
+    scene = CloudScene(ctx, 480, 270, workload=w)
+    _compare(scene, 5e-4, f"Material1 480x270, shadow steps {shadow_steps}")
+
+
 def test_raymarch_minimal_material_parity(ctx):
     scene = CloudScene(ctx, 320, 180, material=VolumetricCloudMinimalMaterial(density=3.0))
     _compare(scene, 1e-4, "Minimal material")
