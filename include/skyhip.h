@@ -861,10 +861,12 @@ int skyhip_voxel_grid_cells(SkyhipContext* ctx, const uint16_t* grid, int32_t di
  *   work_counters: NULL, or device uint64[5] accumulating samples, density lookups,
  *                  majorant segments, path vertices (volume and ground events) and shadow rays
  *                  (a counting kernel variant; leave NULL when timing).
- * With brick majorants of at most 8 texels and the cell-corner copy bound, each frame is one
- * persistent launch whose waves take 8x8-pixel tiles from a counter; the first call on a
- * context allocates that counter (256 bytes, freed by skyhip_destroy).  Calls on one context
- * run in stream order and share it. */
+ * With brick majorants, each call first writes the bricks' inverse majorants (1 / (majorant *
+ * uDensity), 0 for an empty brick) into a context workspace of one float per brick, and with
+ * bricks of at most 8 texels and the cell-corner copy bound, each frame is one persistent
+ * launch whose waves take 8x8-pixel tiles from a context counter.  The first call that needs
+ * them allocates these (the workspace grows to the largest brick grid traced; skyhip_destroy
+ * frees both).  Calls on one context run in stream order and share them. */
 int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmosphereBufferData* atmosphere,
                             const SkyhipAtmosphereRenderBufferData* render, const SkyhipCloudCommonBufferData* common,
                             const SkyhipCloudMaterial* material, const SkyhipPathTracingParameters* params,

@@ -59,8 +59,11 @@ struct SkyhipContext {
     int froxel_w = 0, froxel_h = 0, froxel_d = 0;
     // The raymarch's dual-lobe phase (the reference's constant lobes by default).
     SkyhipCloudPhase phase = {-0.15f, 2.16f, 0.85f, 0.0f};
-    // The persistent path tracer's tile counter (device memory, allocated on first use).
+    // The persistent path tracer's tile counter and the brick walks' inverse majorants (device
+    // memory, allocated on first use; the latter grows to the largest brick grid traced).
     unsigned* pt_queue = nullptr;
+    float* pt_inv_majorants = nullptr;
+    size_t pt_inv_majorants_count = 0;
 };
 
 namespace skyhip {

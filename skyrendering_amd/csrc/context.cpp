@@ -234,6 +234,7 @@ extern "C" int skyhip_destroy(SkyhipContext* ctx) {
     if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
     if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
     if (ctx->pt_queue) (void)hipFree(ctx->pt_queue);
+    if (ctx->pt_inv_majorants) (void)hipFree(ctx->pt_inv_majorants);
     if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -55,8 +55,11 @@ struct PtArgs {
     const float4* ap_l;
     const float4* ap_t;
     int aw, ah, ad;
-    // Brick majorants: texel-space X = pos * tex_scale + tex_bias per axis (z on the flat height)
+    // Brick majorants: texel-space X = pos * tex_scale + tex_bias per axis (z on the flat height).
+    // The walks read inv_majorants[b] = 1 / (majorants[b] * majorant_scale), 0 for an empty brick
+    // (inv_majorant_kernel, before every launch).
     const float* majorants;
+    const float* inv_majorants;
     int nb[3];
     int brick;        // brick edge B in texels (a power of two)
     float inv_brick;  // 1 / B, exact
@@ -237,10 +240,9 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
         int it = 0;
         for (int seg = 0; seg < kMaxSegments; ++seg) {
             const float seg_end = w.segment_end();
-            const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+            const float inv_mu = a.inv_majorants[w.brick(a)];
             if (kCount) wc.segments++;
-            if (mu > 0.0f) {
-                const float inv_mu = 1.0f / mu;
+            if (inv_mu > 0.0f) {
                 for (; it < kMaxTrackingSteps; ++it) {
                     float tt = t + exp_sample(rng.next()) * inv_mu;
                     if (!(tt < seg_end)) break;
@@ -264,13 +266,12 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
     // most collisions there, but every iteration carries both bodies.  Same draws in the same
     // order as the nested form (a draw past the brick is discarded), so the same samples.
     float seg_end = w.segment_end();
-    float mu = a.majorants[w.brick(a)] * a.majorant_scale;
-    float inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+    float inv_mu = a.inv_majorants[w.brick(a)];
     if (kCount) wc.segments++;
     int it = 0, seg = 0;
     for (;;) {
         bool cross = true;
-        if (mu > 0.0f && it < kMaxTrackingSteps) {
+        if (inv_mu > 0.0f && it < kMaxTrackingSteps) {
             const float tt = t + exp_sample(rng.next()) * inv_mu;
             if (tt < seg_end) {
                 cross = false;
@@ -282,13 +283,12 @@ __device__ bool delta_track(const PtArgs& a, Rng& rng, F3 o, F3 d, float t0, flo
             }
         }
         if (cross) {
-            if (mu > 0.0f && it >= kMaxTrackingSteps) return false;
+            if (inv_mu > 0.0f && it >= kMaxTrackingSteps) return false;
             if (!(seg_end < w.t_end) || ++seg >= kMaxSegments) return false;
             t = seg_end;
             w.advance(a);
             seg_end = w.segment_end();
-            mu = a.majorants[w.brick(a)] * a.majorant_scale;
-            inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+            inv_mu = a.inv_majorants[w.brick(a)];
             if (kCount) wc.segments++;
         }
     }
@@ -325,10 +325,9 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
         int it = 0;
         for (int seg = 0; seg < kMaxSegments; ++seg) {
             const float seg_end = w.segment_end();
-            const float mu = a.majorants[w.brick(a)] * a.majorant_scale;
+            const float inv_mu = a.inv_majorants[w.brick(a)];
             if (kCount) wc.segments++;
-            if (mu > 0.0f) {
-                const float inv_mu = 1.0f / mu;
+            if (inv_mu > 0.0f) {
                 while (it < kMaxTrackingSteps) {
                     const float tt = t + exp_sample(rng.next()) * inv_mu;
                     if (!(tt < seg_end)) break;
@@ -358,13 +357,12 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
         return clamp01(tr);
     }
     float seg_end = w.segment_end();
-    float mu = a.majorants[w.brick(a)] * a.majorant_scale;
-    float inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+    float inv_mu = a.inv_majorants[w.brick(a)];
     if (kCount) wc.segments++;
     int it = 0, seg = 0;
     for (;;) {
         bool cross = true;
-        if (mu > 0.0f && it < kMaxTrackingSteps) {
+        if (inv_mu > 0.0f && it < kMaxTrackingSteps) {
             const float tt = t + exp_sample(rng.next()) * inv_mu;
             if (tt < seg_end) {
                 cross = false;
@@ -387,13 +385,12 @@ __device__ float ratio_track(const PtArgs& a, Rng rng, F3 o, F3 d, PtCount& wc) 
             }
         }
         if (cross) {
-            if (mu > 0.0f && (it >= kMaxTrackingSteps || tr == 0.0f)) break;
+            if (inv_mu > 0.0f && (it >= kMaxTrackingSteps || tr == 0.0f)) break;
             if (!(seg_end < w.t_end) || ++seg >= kMaxSegments) break;
             t = seg_end;
             w.advance(a);
             seg_end = w.segment_end();
-            mu = a.majorants[w.brick(a)] * a.majorant_scale;
-            inv_mu = mu > 0.0f ? 1.0f / mu : 0.0f;
+            inv_mu = a.inv_majorants[w.brick(a)];
             if (kCount) wc.segments++;
         }
     }
@@ -766,6 +763,16 @@ __global__ __launch_bounds__(256) void cells_kernel(const uint16_t* __restrict__
     out[c] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
+// The walks' per-brick inverse majorants: the same float operations the walk would do per brick
+// crossing (mu = majorant * scale, then 1 / mu when mu > 0), done once per brick.
+__global__ __launch_bounds__(256) void inv_majorant_kernel(const float* __restrict__ majorants, float scale,
+                                                           float* __restrict__ out, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float mu = majorants[i] * scale;
+    out[i] = mu > 0.0f ? 1.0f / mu : 0.0f;
+}
+
 // Workgroups of path_trace_refill_kernel per CU: 5 of 4 waves, its occupancy at the 5-wave cap.
 constexpr int kRefillBlocksPerCu = 5;
 // Brick-majorant frames with the cell-corner copy run on one of two forms, chosen by the brick
@@ -1000,6 +1007,23 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
     if (region[2] == region[0] || region[3] == region[1] || frames == 0) return SKYHIP_OK;
     if (!ctx->pt_queue) SKYHIP_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->pt_queue), 256));
     a.queue = ctx->pt_queue;
+    if (bricks) {
+        const size_t n = size_t(a.nb[0]) * a.nb[1] * a.nb[2];
+        if (ctx->pt_inv_majorants_count < n) {
+            if (ctx->pt_inv_majorants) {
+                SKYHIP_HIP(hipStreamSynchronize(ctx->stream));
+                SKYHIP_HIP(hipFree(ctx->pt_inv_majorants));
+                ctx->pt_inv_majorants = nullptr;
+                ctx->pt_inv_majorants_count = 0;
+            }
+            SKYHIP_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->pt_inv_majorants), n * sizeof(float)));
+            ctx->pt_inv_majorants_count = n;
+        }
+        hipLaunchKernelGGL(inv_majorant_kernel, dim3(div_up((long long)n, 256)), dim3(256), 0, ctx->stream,
+                           a.majorants, a.majorant_scale, ctx->pt_inv_majorants, int(n));
+        SKYHIP_LAUNCH_CHECK("inv_majorant_kernel");
+        a.inv_majorants = ctx->pt_inv_majorants;
+    }
     const int cus = std::max(ctx->cu_count, 1);
     if (bricks) {
         if (work_counters)
