@@ -114,7 +114,17 @@ struct Rng {
 };
 
 // InfiniteTransmittanceIS numerator: -log(1 - xi).
-__device__ __forceinline__ float exp_sample(float xi) { return -__logf(1.0f - xi); }
+// -ln(1 - xi) for Random01's xi in [0, 1 - 2^-32]: 1 - xi is a normal float in [2^-32, 1], so
+// the hardware log2 (v_log_f32) needs none of the library logf's denormal and infinity handling.
+// ln = log2 * ln2, with ln2 split into a float kLn2Hi just below it and the remainder kLn2Lo,
+// and the product's rounding error recovered with an FMA, so the result keeps FP32 precision.
+__device__ __forceinline__ float exp_sample(float xi) {
+    constexpr float kLn2Hi = 0x1.62e42ep-1f;     // 0.693147122...
+    constexpr float kLn2Lo = 0x1.efa39ep-25f;    // ln 2 - kLn2Hi = 5.77e-8
+    const float l2 = __builtin_amdgcn_logf(1.0f - xi);
+    const float hi = l2 * kLn2Hi;
+    return -(hi + (fmaf(l2, kLn2Hi, -hi) + l2 * kLn2Lo));
+}
 
 // CloudRegionIntersect (:57-75) on the slab +-half_width x [hb, ht].
 __device__ __forceinline__ void region(const PtArgs& a, F3 o, F3 d, float& t0, float& t1) {
@@ -994,6 +1004,13 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
         a.cells = reinterpret_cast<const uint4*>(in->cell_corners);
         if (a.cells && (reinterpret_cast<uintptr_t>(a.cells) & 15) != 0)
             return fail(SKYHIP_ERR_INVALID_ARGUMENT, "cell_corners must be 16-byte aligned");
+        if (a.cells) {  // the lookups' cell index uses 24-bit multiplies (grid_cells_trilinear_texel)
+            const long long cx = material->voxel_grid_dim[0] + 1LL, cy = material->voxel_grid_dim[1] + 1LL,
+                            cz = material->voxel_grid_dim[2] + 1LL;
+            if (cy * cz >= (1LL << 24) || cx * cy * cz > (1LL << 31))
+                return fail(SKYHIP_ERR_INVALID_SIZE,
+                            "cell_corners needs (dim_y + 1)(dim_z + 1) < 2^24 and at most 2^31 cells");
+        }
         if (!(vb.uDensity >= 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "voxel density < 0");
     }
     for (int k = 0; k < 4; ++k) a.region[k] = region[k];
