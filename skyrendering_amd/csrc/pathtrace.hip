@@ -114,16 +114,22 @@ struct Rng {
 };
 
 // InfiniteTransmittanceIS numerator: -log(1 - xi).
-// -ln(1 - xi) for Random01's xi in [0, 1 - 2^-32]: 1 - xi is a normal float in [2^-32, 1], so
-// the hardware log2 (v_log_f32) needs none of the library logf's denormal and infinity handling.
-// ln = log2 * ln2, with ln2 split into a float kLn2Hi just below it and the remainder kLn2Lo,
-// and the product's rounding error recovered with an FMA, so the result keeps FP32 precision.
+// -ln(1 - xi) for Random01's xi = seed * 2^-32.  The float conversion rounds seeds from
+// 2^32 - 128 up to xi = 1, where the shader's log(1 - xi) is -inf and the step +inf (it leaves
+// the segment); that case is returned as +inf explicitly, because the product below would turn
+// -inf into NaN (a NaN step never leaves the single-majorant loops before their safety cap).
+// Otherwise 1 - xi is a normal float in [2^-32, 1], so the hardware log2 (v_log_f32) needs none
+// of the library logf's denormal and infinity handling.  ln = log2 * ln2, with ln2 split into a
+// float kLn2Hi just below it and the remainder kLn2Lo, and the product's rounding error
+// recovered with an FMA, so the result keeps FP32 precision.
 __device__ __forceinline__ float exp_sample(float xi) {
     constexpr float kLn2Hi = 0x1.62e42ep-1f;     // 0.693147122...
     constexpr float kLn2Lo = 0x1.efa39ep-25f;    // ln 2 - kLn2Hi = 5.77e-8
-    const float l2 = __builtin_amdgcn_logf(1.0f - xi);
+    const float x = 1.0f - xi;
+    const float l2 = __builtin_amdgcn_logf(fmaxf(x, 0x1p-32f));
     const float hi = l2 * kLn2Hi;
-    return -(hi + (fmaf(l2, kLn2Hi, -hi) + l2 * kLn2Lo));
+    const float r = -(hi + (fmaf(l2, kLn2Hi, -hi) + l2 * kLn2Lo));
+    return x > 0.0f ? r : __builtin_inff();
 }
 
 // CloudRegionIntersect (:57-75) on the slab +-half_width x [hb, ht].
