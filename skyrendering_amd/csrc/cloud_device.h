@@ -204,11 +204,8 @@ __device__ __forceinline__ float grid_cells_trilinear_texel(const uint4* __restr
     const int ix = floor_to_int(x), iy = floor_to_int(y), iz = floor_to_int(z);
     const float ax = x - floorf(x), ay = y - floorf(y), az = z - floorf(z);
     const bool inside = ix >= -1 && iy >= -1 && iz >= -1 && ix < dim[0] && iy < dim[1] && iz < dim[2];
-    // 24-bit multiplies (full rate): exact because the caller guarantees (dim[1] + 1) (dim[2] + 1)
-    // < 2^24 and a copy of at most 2^31 cells.
-    const uint32_t cell = inside ? __umul24(__umul24(uint32_t(iz + 1), uint32_t(dim[1] + 1)) + uint32_t(iy + 1),
-                                            uint32_t(dim[0] + 1)) +
-                                       uint32_t(ix + 1)
+    const uint32_t cell = inside ? (uint32_t(iz + 1) * uint32_t(dim[1] + 1) + uint32_t(iy + 1)) *
+                                           uint32_t(dim[0] + 1) + uint32_t(ix + 1)
                                  : 0u;
     const uint4 q = cells[cell];
     const uint32_t w[4] = {q.x, q.y, q.z, q.w};

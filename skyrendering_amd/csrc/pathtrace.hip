@@ -1010,12 +1010,11 @@ extern "C" int skyhip_cloud_path_trace(SkyhipContext* ctx, const SkyhipAtmospher
         a.cells = reinterpret_cast<const uint4*>(in->cell_corners);
         if (a.cells && (reinterpret_cast<uintptr_t>(a.cells) & 15) != 0)
             return fail(SKYHIP_ERR_INVALID_ARGUMENT, "cell_corners must be 16-byte aligned");
-        if (a.cells) {  // the lookups' cell index uses 24-bit multiplies (grid_cells_trilinear_texel)
-            const long long cx = material->voxel_grid_dim[0] + 1LL, cy = material->voxel_grid_dim[1] + 1LL,
-                            cz = material->voxel_grid_dim[2] + 1LL;
-            if (cy * cz >= (1LL << 24) || cx * cy * cz > (1LL << 31))
-                return fail(SKYHIP_ERR_INVALID_SIZE,
-                            "cell_corners needs (dim_y + 1)(dim_z + 1) < 2^24 and at most 2^31 cells");
+        if (a.cells) {  // a copy has at most 2^31 cells (skyhip_voxel_grid_cells); the lookups index in 32 bits
+            const long long cells = (material->voxel_grid_dim[0] + 1LL) * (material->voxel_grid_dim[1] + 1LL) *
+                                    (material->voxel_grid_dim[2] + 1LL);
+            if (cells > (1LL << 31))
+                return fail(SKYHIP_ERR_INVALID_SIZE, "cell_corners: a grid of more than 2^31 cells has no copy");
         }
         if (!(vb.uDensity >= 0.0f)) return fail(SKYHIP_ERR_INVALID_ARGUMENT, "voxel density < 0");
     }

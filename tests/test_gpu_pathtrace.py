@@ -304,11 +304,12 @@ def test_counting_variant_and_rejections(ctx):
     minimal = cases.minimal_material(1.0)
     assert L.skyhip_cloud_path_trace(*args(ok, 1, scene.tracer.params, minimal, scene.inputs)) == \
         _abi.ERR_INVALID_ARGUMENT  # brick majorants need a VOXEL_F16 material
-    # The cell-corner lookups index with 24-bit multiplies: (dim_y + 1)(dim_z + 1) must stay below
-    # 2^24.  Zero frames, so that nothing would be launched even if the check let it through.
+    # A grid of more than 2^31 cells has no cell-corner copy (skyhip_voxel_grid_cells refuses it):
+    # bound with one, the call is refused.  Zero frames, so that nothing would be launched even if
+    # the check let it through.
     tall = _abi.CloudMaterial()
     C.memmove(C.byref(tall), C.byref(scene.material), C.sizeof(tall))
-    tall.voxel_grid_dim[0], tall.voxel_grid_dim[1], tall.voxel_grid_dim[2] = 2, 4096, 4096
+    tall.voxel_grid_dim[0], tall.voxel_grid_dim[1], tall.voxel_grid_dim[2] = 2048, 2048, 2048
     assert scene.inputs.cell_corners
     assert L.skyhip_cloud_path_trace(*args(ok, 0, scene.tracer.params, tall, scene.inputs)) == _abi.ERR_INVALID_SIZE
     assert b"cell_corners" in L.skyhip_last_error()
