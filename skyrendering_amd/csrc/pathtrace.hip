@@ -793,11 +793,12 @@ __global__ __launch_bounds__(256) void inv_majorant_kernel(const float* __restri
 constexpr int kRefillBlocksPerCu = 5;
 // Brick-majorant frames with the cell-corner copy run on one of two forms, chosen by the brick
 // edge (same samples either way).  Measured on the 1024^2, 512^3-grid workload (ms per frame,
-// one box, per-pixel kernel / refill kernel, nested / flat tracking loops): 16^3 bricks 8.64-8.70
-// per-pixel nested, 7.96-8.01 per-pixel flat, 8.46-8.52 refill nested, 8.09-8.18 refill flat;
-// 8^3 bricks 9.23-9.47, 9.29, 8.59-8.67, 9.37-9.51.  Small bricks make crossings outnumber
-// lookups (240 M against 82 M per frame at 8^3), which the flat loop pays for in every
-// iteration; there the nested loops with lanes refilled per bounce win.
+// one box): per-pixel kernel with flat tracking loops 7.30-7.36 at 16^3 and 8.49-8.50 at 8^3;
+// refill kernel with nested loops 8.43-8.48 at 8^3; refill kernel with flat loops 8.18-8.24 at
+// 16^3 and 9.61-9.76 at 8^3.  Before the inverse majorants and the cheaper draws the per-pixel
+// flat kernel was 9.29 at 8^3 against the nested refill kernel's 8.59-8.67: small bricks make
+// crossings outnumber lookups (240 M against 82 M per frame at 8^3), and the flat loop carries
+// the crossing body in every iteration.
 constexpr int kRefillMaxBrick = 8;
 
 template <bool kBricks, bool kCount>
