@@ -605,10 +605,11 @@ __device__ __forceinline__ float4 add_sample(const PtArgs& a, float u, float v, 
     return acc;
 }
 
-// Five waves per SIMD (96 VGPRs; the compiler spills the rest of the 154 the path state would
-// take to scratch): more waves hide more of the tracking's fetch latency.  Measured on the
-// 1024^2 brick-majorant workload: 3 waves (no cap) 14.2 ms per frame, 4 waves 12.0, 5 waves
-// 11.2, 6 waves 11.4, 7 and 8 waves 12.0-12.2; every variant gives the same image.
+// Five waves per SIMD (96 VGPRs; the compiler spills the rest of the path state to scratch):
+// more waves hide more of the tracking's fetch latency.  Measured on the 1024^2 brick-majorant
+// workload with the nested loops: 3 waves (no cap) 14.2 ms per frame, 4 waves 12.0, 5 waves
+// 11.2, 6 waves 11.4, 7 and 8 waves 12.0-12.2; with the flat loops and cell-corner lookups at
+// 16^3: 4 waves 7.81-7.88, 5 waves 7.27-7.28, 6 waves 9.45.  Every variant gives the same image.
 template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void path_trace_kernel(PtArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
