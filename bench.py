@@ -61,8 +61,8 @@ FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
     "cloud_raymarch_kernel<5,false>": (36115.9, 25697.5),
-    "path_trace_kernel<4,true,false,true,true>": (4.1875e6, 1.1704e5),
-    "path_trace_refill_kernel<4,true,true,false>": (3.4820e6, 6.712e4),
+    "path_trace_kernel<4,true,false,true,true>": (4.1817e6, 1.1709e5),
+    "path_trace_refill_kernel<4,true,true,false>": (3.4974e6, 6.710e4),
 }
 
 
