@@ -230,11 +230,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
                 }
                 wave_lds_sync();
                 const int L = a.light_steps, total = m * L;
+                const float inv_L = 1.0f / float(L);
                 float* const s_contrib = s_contrib_all + wave * 64 * L;
                 for (int base = 0; base < total; base += 64) {
                     const int item = base + lane;
                     if (item < total) {
-                        const int r = item / L, ls = item - r * L;
+                        // item / L for item < 64 * 16: exact through the float reciprocal (the
+                        // quotient's distance from the next integer, >= 0.5 / L, is far above
+                        // the rounding error), where the integer division takes ~15 instructions.
+                        const int r = int((float(item) + 0.5f) * inv_L), ls = item - r * L;
                         const F3 o = {s_origin[wave][3 * r], s_origin[wave][3 * r + 1], s_origin[wave][3 * r + 2]};
                         const F3 sp = o + sun_vector * a.light_offset[ls];
                         s_contrib[item] =
