@@ -42,12 +42,12 @@ HBM_PEAK_GBS = 8000.0
 # profiles/lut_batch_256x8.md): SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP32_TRANS count
 # operations per wave instruction (an FMA counts 2), so the lane operations are 64 times that,
 # scaled by the measured lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), then
-# divided by the executed sample-steps (raymarch: 7.314e9 over 35,111,907) or integrator steps
+# divided by the executed sample-steps (raymarch: 7.3665e9 over 35,111,907) or integrator steps
 # (LUT set, counted on the one-lane-per-texel kernels so that the split kernels' repeated ray
-# set-up is not counted as work: 1.0386e9 over 4,914,688; batch: 7.185e11 over 3,024,532,480)
+# set-up is not counted as work: 1.0386e9 over 4,914,688; batch: 7.184e11 over 3,024,532,480)
 # of the same launches.
 RAYMARCH_KERNEL = "cloud_raymarch_kernel<5,false>"
-RAYMARCH_FP32_FLOP_PER_STEP = 208.3
+RAYMARCH_FP32_FLOP_PER_STEP = 209.8
 LUT_SET_FLOP_PER_STEP = 211.3
 BATCH_FLOP_PER_STEP = 237.5
 
@@ -60,7 +60,7 @@ BATCH_FLOP_PER_STEP = 237.5
 FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
-    "cloud_raymarch_kernel<5,false>": (36115.9, 25697.5),
+    "cloud_raymarch_kernel<5,false>": (36040.0, 25686.0),
     "path_trace_kernel<4,true,false,true,true>": (4.1817e6, 1.1709e5),
     "path_trace_refill_kernel<4,true,true,false>": (3.4974e6, 6.710e4),
 }
