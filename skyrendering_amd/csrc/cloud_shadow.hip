@@ -8,7 +8,8 @@
 //  - shadow_froxel_kernel: per froxel column (viewport / 12), the running mean of the
 //    blurred map's transmittance along the view ray over 128 slices, stored as R16 UNORM
 //    (VolumetricCloudShadowFroxel.comp with SampleCloudShadowTransmittance,
-//    VolumetricCloudShadowInterface.glsl:4-8).
+//    VolumetricCloudShadowInterface.glsl:4-8); the slices' terms are evaluated in parallel and
+//    summed per column in slice order.
 // The arithmetic is FP32 as the shaders have it, except the shadow map's per-texel geometry
 // (FP64, see shadow_map_kernel).
 #include "cloud_device.h"
@@ -167,25 +168,49 @@ struct FroxelArgs {
     int w, h, d;
 };
 
+// A workgroup takes kFroxelCols consecutive columns of one froxel row.  The slices' terms are
+// independent of each other, so its 256 threads evaluate them in parallel, kFroxelChunk slices
+// at a time, into LDS; then one thread per column forms the running mean in slice order.  The
+// slice distances come from a table that one thread fills with the shader's running
+// t += step_size, so every term, sum and output is the one-thread-per-column loop's.
+constexpr int kFroxelCols = 32;
+constexpr int kFroxelChunk = 128;
 __global__ __launch_bounds__(256) void shadow_froxel_kernel(FroxelArgs a) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.w * a.h) return;
-    const int x = i % a.w, y = i / a.w;
+    __shared__ float s_t[kFroxelChunk];
+    __shared__ float s_term[kFroxelChunk][kFroxelCols];
+    const int tiles_x = (a.w + kFroxelCols - 1) / kFroxelCols;
+    const int y = int(blockIdx.x) / tiles_x, x = (int(blockIdx.x) % tiles_x) * kFroxelCols + int(threadIdx.x) % kFroxelCols;
+    const int c = int(threadIdx.x) % kFroxelCols, g = int(threadIdx.x) / kFroxelCols;  // g in [0, 256 / kFroxelCols)
+    const bool column = x < a.w;  // y < a.h by the grid
     const float u = (float(x) + 0.5f) / float(a.w), v = (float(y) + 0.5f) / float(a.h);
     const F3 frag = projective_mul(a.inv_mvp, F3{u * 2.0f - 1.0f, v * 2.0f - 1.0f, 0.0f});
     const float step_size = a.max_distance / float(a.d);
     F3 dir = frag - a.cam;
     dir = dir * (1.0f / sqrtf(dot(dir, dir)));
-    float t = 0.5f * step_size, sum = 0.0f;
-    for (int z = 0; z < a.d; ++z, t += step_size) {
-        const F3 pos = a.cam + dir * t;
-        const F3 ndc = projective_mul(a.light_vp, pos);
-        // SampleCloudShadowTransmittance (VolumetricCloudShadowInterface.glsl:4-8)
-        const float2 dt = shadow_map_bilinear(a.map, a.mw, a.mh, ndc.x * 0.5f + 0.5f, ndc.y * 0.5f + 0.5f);
-        const float s = clamp01((dt.x - ndc.z) * 2.0f);
-        sum += dt.y * (1.0f - s) + 1.0f * s;
-        const float vis = sum / float(z + 1);
-        a.out[((long long)z * a.h + y) * a.w + x] = uint16_t(rintf(clamp01(vis) * 65535.0f));  // R16 UNORM
+    float t = 0.5f * step_size, sum = 0.0f;  // t: thread 0's running distance; sum: the column threads'
+    for (int z0 = 0; z0 < a.d; z0 += kFroxelChunk) {
+        const int n = min(kFroxelChunk, a.d - z0);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < n; ++k, t += step_size) s_t[k] = t;
+        __syncthreads();
+        if (column)
+            for (int k = g; k < n; k += 256 / kFroxelCols) {
+                const F3 pos = a.cam + dir * s_t[k];
+                const F3 ndc = projective_mul(a.light_vp, pos);
+                // SampleCloudShadowTransmittance (VolumetricCloudShadowInterface.glsl:4-8)
+                const float2 dt = shadow_map_bilinear(a.map, a.mw, a.mh, ndc.x * 0.5f + 0.5f, ndc.y * 0.5f + 0.5f);
+                const float s = clamp01((dt.x - ndc.z) * 2.0f);
+                s_term[k][c] = dt.y * (1.0f - s) + 1.0f * s;
+            }
+        __syncthreads();
+        if (column && g == 0)
+            for (int k = 0; k < n; ++k) {
+                sum += s_term[k][c];
+                const int z = z0 + k;
+                const float vis = sum / float(z + 1);
+                a.out[((long long)z * a.h + y) * a.w + x] = uint16_t(rintf(clamp01(vis) * 65535.0f));  // R16 UNORM
+            }
+        __syncthreads();  // the next chunk overwrites the tables
     }
 }
 
@@ -278,8 +303,8 @@ extern "C" int skyhip_cloud_shadow_froxel(SkyhipContext* ctx, const SkyhipCloudC
     a.w = width;
     a.h = height;
     a.d = depth;
-    hipLaunchKernelGGL(shadow_froxel_kernel, dim3(div_up((long long)width * height, 256)), dim3(256), 0, ctx->stream,
-                       a);
+    hipLaunchKernelGGL(shadow_froxel_kernel, dim3(unsigned(div_up(width, kFroxelCols) * height)), dim3(256), 0,
+                       ctx->stream, a);
     SKYHIP_LAUNCH_CHECK("shadow_froxel_kernel");
     return SKYHIP_OK;
 }
