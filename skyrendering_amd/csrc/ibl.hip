@@ -8,7 +8,7 @@
 //  - sh9_kernel: the 9 spherical-harmonic radiance coefficients (EnvRadianceSH.comp), one
 //    workgroup of 1024 stratified directions per coefficient, summed by the shader's LDS tree;
 //  - prefilter_kernel: the GGX-prefiltered radiance cube, one level per roughness
-//    (PrefilterRadiance.comp, IBL.cpp:25-45).
+//    (PrefilterRadiance.comp, IBL.cpp:25-45), a texel's samples spread over a group of lanes.
 // Each pass is tiny (at most 98,304 texels); the sequence is bound by launch latency.
 #include "sky_env.h"
 
@@ -122,13 +122,20 @@ __device__ inline void onb_frame(F3 N, F3& t0, F3& t1) {
     t1 = {b, s + N.y * N.y * aa, -N.y};
 }
 
-// PrefilterRadiance.comp for level `level` (width wl) of the output cube at roughness r.
+// PrefilterRadiance.comp for level `level` (width wl) of the output cube at roughness r.  Each
+// texel's samples are spread over a group of `group` lanes (a power of two >= the sample count n,
+// at most 64; 1 at roughness 0, where n = 1): lane k evaluates sample k into LDS, and the group's
+// first lane adds the terms in sample order, so the sums are the one-thread-per-texel loop's.
+// The rough levels have few texels (384 at 8^2 x 6) and up to 64 samples each, which one thread
+// per texel left as a chain of dependent cube fetches.
 __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restrict__ env, int ew, int elevels,
                                                         ushort4* __restrict__ out, int pw, int level,
-                                                        float roughness) {
+                                                        float roughness, int group) {
+    __shared__ float4 s_term[256];
     const int wl = max(pw >> level, 1);
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= 6 * wl * wl) return;
+    const int j = int(threadIdx.x) / group, k = int(threadIdx.x) % group;
+    const int i = int(blockIdx.x) * (256 / group) + j;
+    const bool texel = i < 6 * wl * wl;
     const int face = i / (wl * wl), y = (i / wl) % wl, x = i % wl;
     const F3 R = cube_texel_dir(face, x, y, wl);
     const F3 N = R, V = R;
@@ -136,15 +143,14 @@ __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restric
     const float p = powf(roughness, 0.3f);
     const uint32_t n = uint32_t(1.0f * (1.0f - p) + 64.0f * p);  // uint(mix(1, 64, pow(roughness, 0.3)))
     const float inv_sa_texel = (6.0f * float(wl) * float(wl)) / (4.0f * kPiF);
-    F3 t0, t1;
-    onb_frame(N, t0, t1);
-    F3 color = f3(0.0f);
-    float total = 0.0f;
-    for (uint32_t k = 0; k < n; ++k) {
+    float4 term = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // (sample * NoL, NoL), NoL = 0: no contribution
+    if (texel && uint32_t(k) < n) {
+        F3 t0, t1;
+        onb_frame(N, t0, t1);
         // Hammersley (shaders/Base/Noise.glsl:113-117) with no random offset
-        const float q = float(k) / float(n);
+        const float q = float(uint32_t(k)) / float(n);
         const float e1 = q - floorf(q);
-        const float e2 = float(reverse_bits32(k)) * 2.3283064365386963e-10f;
+        const float e2 = float(reverse_bits32(uint32_t(k))) * 2.3283064365386963e-10f;
         // ImportanceSampleGGX (shaders/Base/BRDF.glsl:48-67)
         const float phi = 2.0f * kPiF * e1;
         const float cos_t = sqrtf((1.0f - e2) / (1.0f + (a2 - 1.0f) * e2));
@@ -159,8 +165,20 @@ __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restric
             const float pdf = fmaxf(D * NoH / (4.0f * HoV), 0.0001f);
             const float sa_sample = 1.0f / fmaxf(float(n) * pdf, 0.00001f);
             const float mip = roughness == 0.0f ? 0.0f : 0.5f * log2f(sa_sample * inv_sa_texel) + 2.5f;
-            color += cube_sample_lod(env, ew, elevels, L, mip) * NoL;
-            total += NoL;
+            const F3 c = cube_sample_lod(env, ew, elevels, L, mip) * NoL;
+            term = make_float4(c.x, c.y, c.z, NoL);
+        }
+    }
+    s_term[threadIdx.x] = term;
+    __syncthreads();
+    if (!texel || k != 0) return;
+    F3 color = f3(0.0f);
+    float total = 0.0f;
+    for (uint32_t m = 0; m < n; ++m) {
+        const float4 t = s_term[int(threadIdx.x) + int(m)];
+        if (t.w > 0.0f) {
+            color += F3{t.x, t.y, t.z};
+            total += t.w;
         }
     }
     out[cube_level_offset(pw, level) + i] = to_rgba16f(color * (1.0f / total), 0.0f);
@@ -229,8 +247,9 @@ extern "C" int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment
     for (int l = 0; l < roughness_count; ++l) {
         const int wl = prefiltered_width >> l;
         const float roughness = roughness_count > 1 ? float(l) / float(roughness_count - 1) : 0.0f;
-        hipLaunchKernelGGL(prefilter_kernel, dim3(div_up(6LL * wl * wl, 256)), dim3(256), 0, ctx->stream, env,
-                           environment_width, elevels, out, prefiltered_width, l, roughness);
+        const int group = roughness > 0.0f ? 64 : 1;  // at most 64 samples per texel; 1 at roughness 0
+        hipLaunchKernelGGL(prefilter_kernel, dim3(div_up(6LL * wl * wl, 256 / group)), dim3(256), 0, ctx->stream,
+                           env, environment_width, elevels, out, prefiltered_width, l, roughness, group);
     }
     SKYHIP_LAUNCH_CHECK("prefilter_kernel");
     return SKYHIP_OK;
