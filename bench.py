@@ -50,6 +50,13 @@ RAYMARCH_KERNEL = "cloud_raymarch_kernel<5,false>"
 RAYMARCH_FP32_FLOP_PER_STEP = 209.8
 LUT_SET_FLOP_PER_STEP = 211.3
 BATCH_FLOP_PER_STEP = 237.5
+# The cloud shadow frame (map, two blur passes, froxel) and the environment update (cube, mips,
+# SH9, five prefiltered levels) of this bench's workloads: FP32 operations (the SQ counters as
+# above) and calibrated HBM bytes, summed over their launches (profiles/cloud_shadow_environment.md).
+SHADOW_FP32_FLOP_PER_FRAME = 4.459e8
+SHADOW_HBM_BYTES_PER_FRAME = 61.0e6
+ENV_FP32_FLOP_PER_UPDATE = 1.031e9
+ENV_HBM_BYTES_PER_UPDATE = 9.85e6
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
@@ -279,7 +286,15 @@ def bench_environment(ctx, reps=20):
     out = {"workload": f"environment cube {w}^2 x 6 RGBA16F + {env.levels - 1} mip levels, SH9 (9 x 1024 samples), "
                        f"prefiltered cube {pw}^2 x {n_r} roughness levels (GGX, 1-64 samples), Earth, sun 45 deg",
            "ms_per_update": round(ms, 4), "env_texels": 6 * w * w, "prefilter_samples": prefilter_samples,
-           "bound": "launch latency (14 dependent launches of at most 98,304 threads)"}
+           "bound": "launch latency (14 dependent launches of at most 98,304 threads)",
+           "roofline": {"bound": "valu", "unit": "TFLOP/s", "fp32_flop_per_update": ENV_FP32_FLOP_PER_UPDATE,
+                        "achieved": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12, 3),
+                        "peak": FP32_PEAK_TFLOPS,
+                        "frac": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                        "traffic": int(ENV_HBM_BYTES_PER_UPDATE),
+                        "traffic_gbs": round(ENV_HBM_BYTES_PER_UPDATE / (ms * 1e-3) / 1e9, 1),
+                        "note": "latency-bound chain of 14 launches (the prefilter's five levels take 79 µs of it); "
+                                "FP32 operations and HBM bytes measured per update with rocprofv3 counters"}}
     import oracle
     sv_l, sv_t = rend.sky_view()
     T = atm.transmittance()
@@ -314,7 +329,15 @@ def bench_cloud_shadow(ctx, scene, reps=20):
     fw, fh, fd = sh.froxel_size
     out = {"workload": f"cloud shadow map {mw}x{mh} (Material1 + Worley volume, temporal), 2 blur passes, shadow froxel "
                        f"{fw}x{fh}x{fd}", "ms_per_frame": round(ms, 4),
-           "bound": "latency (4 launches; the map's 262k threads march 6-12 material samples each)"}
+           "bound": "latency (4 dependent launches; the map's 262k threads march 6-12 material samples each)",
+           "roofline": {"bound": "valu", "unit": "TFLOP/s", "fp32_flop_per_frame": SHADOW_FP32_FLOP_PER_FRAME,
+                        "achieved": round(SHADOW_FP32_FLOP_PER_FRAME / (ms * 1e-3) / 1e12, 3),
+                        "peak": FP32_PEAK_TFLOPS,
+                        "frac": round(SHADOW_FP32_FLOP_PER_FRAME / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                        "traffic": int(SHADOW_HBM_BYTES_PER_FRAME),
+                        "traffic_gbs": round(SHADOW_HBM_BYTES_PER_FRAME / (ms * 1e-3) / 1e9, 1),
+                        "note": "latency-bound chain of four launches; FP32 operations and HBM bytes measured "
+                                "per frame with rocprofv3 counters"}}
     import oracle
     import ctypes as C
     from skyrendering_amd import _abi
