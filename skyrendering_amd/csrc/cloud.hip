@@ -104,9 +104,9 @@ struct RayState {
     }
 };
 
-// Six waves per SIMD: the register budget that allows it (80 VGPRs, one spill) ran 2.7 % faster
-// than the unconstrained allocation (95 VGPRs, five waves); seven and eight waves spilled 17 and
-// 38 registers and were slower.  Every variant gives the same image.
+// Six waves per SIMD (79 VGPRs, no spill).  On the config-2 frame: five waves 0.493-0.495 ms,
+// six 0.461-0.464 ms, seven (13 VGPRs spilled) 0.466 ms; eight spilled more and was slower.
+// Every variant gives the same image.
 template <int kMat, bool kCount>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void cloud_raymarch_kernel(CloudArgs a) {
     __shared__ float s_origin[4][64 * 3];
