@@ -122,19 +122,32 @@ __device__ inline void onb_frame(F3 N, F3& t0, F3& t1) {
     t1 = {b, s + N.y * N.y * aa, -N.y};
 }
 
+constexpr int kMaxPrefilterLevels = 13;  // roughness_count <= log2(4096) + 1
+
 // PrefilterRadiance.comp for level `level` (width wl) of the output cube at roughness r.  Each
 // texel's samples are spread over a group of `group` lanes (a power of two >= the sample count n,
 // at most 64; 1 at roughness 0, where n = 1): lane k evaluates sample k into LDS, and the group's
 // first lane adds the terms in sample order, so the sums are the one-thread-per-texel loop's.
 // The rough levels have few texels (384 at 8^2 x 6) and up to 64 samples each, which one thread
 // per texel left as a chain of dependent cube fetches.
+// All levels run in one launch: workgroups [block_start[l], block_start[l + 1]) build level l.
+struct PrefilterLevels {
+    int count;
+    int block_start[kMaxPrefilterLevels + 1];
+    float roughness[kMaxPrefilterLevels];
+    int group[kMaxPrefilterLevels];
+};
+
 __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restrict__ env, int ew, int elevels,
-                                                        ushort4* __restrict__ out, int pw, int level,
-                                                        float roughness, int group) {
+                                                        ushort4* __restrict__ out, int pw, PrefilterLevels lv) {
     __shared__ float4 s_term[256];
+    int level = 0;
+    while (level + 1 < lv.count && int(blockIdx.x) >= lv.block_start[level + 1]) ++level;
+    const float roughness = lv.roughness[level];
+    const int group = lv.group[level];
     const int wl = max(pw >> level, 1);
     const int j = int(threadIdx.x) / group, k = int(threadIdx.x) % group;
-    const int i = int(blockIdx.x) * (256 / group) + j;
+    const int i = (int(blockIdx.x) - lv.block_start[level]) * (256 / group) + j;
     const bool texel = i < 6 * wl * wl;
     const int face = i / (wl * wl), y = (i / wl) % wl, x = i % wl;
     const F3 R = cube_texel_dir(face, x, y, wl);
@@ -244,13 +257,19 @@ extern "C" int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment
                        reinterpret_cast<float4*>(out_sh9_rgba));
     SKYHIP_LAUNCH_CHECK("sh9_kernel");
     ushort4* out = reinterpret_cast<ushort4*>(out_prefiltered_rgba16f);
+    PrefilterLevels lv{};
+    lv.count = roughness_count;
+    long long blocks = 0;
     for (int l = 0; l < roughness_count; ++l) {
         const int wl = prefiltered_width >> l;
-        const float roughness = roughness_count > 1 ? float(l) / float(roughness_count - 1) : 0.0f;
-        const int group = roughness > 0.0f ? 64 : 1;  // at most 64 samples per texel; 1 at roughness 0
-        hipLaunchKernelGGL(prefilter_kernel, dim3(div_up(6LL * wl * wl, 256 / group)), dim3(256), 0, ctx->stream,
-                           env, environment_width, elevels, out, prefiltered_width, l, roughness, group);
+        lv.roughness[l] = roughness_count > 1 ? float(l) / float(roughness_count - 1) : 0.0f;
+        lv.group[l] = lv.roughness[l] > 0.0f ? 64 : 1;  // at most 64 samples per texel; 1 at roughness 0
+        lv.block_start[l] = int(blocks);
+        blocks += div_up(6LL * wl * wl, 256 / lv.group[l]);
     }
+    lv.block_start[roughness_count] = int(blocks);
+    hipLaunchKernelGGL(prefilter_kernel, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream, env, environment_width,
+                       elevels, out, prefiltered_width, lv);
     SKYHIP_LAUNCH_CHECK("prefilter_kernel");
     return SKYHIP_OK;
 }
