@@ -3,8 +3,9 @@
 //  - env_luminance_kernel: one thread per cube texel, the sky-view LUT pair looked up along
 //    the texel's direction plus the lit ground (AtmosphereRenderer.glsl:248-275), stored as
 //    RGBA16F (a = 0);
-//  - cube_mip_kernel: the cube's mip chain, a 2x2 box per level (glGenerateTextureMipmap,
-//    AtmosphereRenderer.cpp:242; the filter is the implementation's, here a box);
+//  - cube_mip_kernel, cube_mip_tail_kernel: the cube's mip chain, a 2x2 box per level
+//    (glGenerateTextureMipmap, AtmosphereRenderer.cpp:242; the filter is the implementation's,
+//    here a box), the levels of at most 64^2 texels a face in one launch;
 //  - sh9_kernel: the 9 spherical-harmonic radiance coefficients (EnvRadianceSH.comp), one
 //    workgroup of 1024 stratified directions per coefficient, summed by the shader's LDS tree;
 //  - prefilter_kernel: the GGX-prefiltered radiance cube, one level per roughness
@@ -66,6 +67,45 @@ __global__ __launch_bounds__(256) void cube_mip_kernel(ushort4* cube, int w, int
     o.z = __half_as_ushort(__float2half_rn(((h2f(a.z) + h2f(b.z)) + (h2f(c.z) + h2f(d.z))) * 0.25f));
     o.w = __half_as_ushort(__float2half_rn(((h2f(a.w) + h2f(b.w)) + (h2f(c.w) + h2f(d.w))) * 0.25f));
     cube[cube_level_offset(w, level) + (long long)face * wl * wl + y * wl + x] = o;
+}
+
+// The mean of a 2x2 parent block per channel (cube_mip_kernel's arithmetic).
+__device__ inline ushort4 box4(ushort4 a, ushort4 b, ushort4 c, ushort4 d) {
+    ushort4 o;
+    o.x = __half_as_ushort(__float2half_rn(((h2f(a.x) + h2f(b.x)) + (h2f(c.x) + h2f(d.x))) * 0.25f));
+    o.y = __half_as_ushort(__float2half_rn(((h2f(a.y) + h2f(b.y)) + (h2f(c.y) + h2f(d.y))) * 0.25f));
+    o.z = __half_as_ushort(__float2half_rn(((h2f(a.z) + h2f(b.z)) + (h2f(c.z) + h2f(d.z))) * 0.25f));
+    o.w = __half_as_ushort(__float2half_rn(((h2f(a.w) + h2f(b.w)) + (h2f(c.w) + h2f(d.w))) * 0.25f));
+    return o;
+}
+
+// Levels first_level .. levels - 1 of one face (blockIdx.x) in one workgroup, for the levels of at
+// most kMipTailWidth^2 texels a face: the first from the parent level in global memory, each
+// later one from the previous level kept in LDS, every level also stored to the cube.  The same
+// arithmetic as cube_mip_kernel level by level, without a launch per level.
+constexpr int kMipTailWidth = 64;
+__global__ __launch_bounds__(1024) void cube_mip_tail_kernel(ushort4* cube, int w, int first_level, int levels) {
+    __shared__ ushort4 s_a[kMipTailWidth * kMipTailWidth];
+    __shared__ ushort4 s_b[(kMipTailWidth / 2) * (kMipTailWidth / 2)];
+    const int face = blockIdx.x;
+    for (int l = first_level; l < levels; ++l) {
+        const int wl = max(w >> l, 1), wp = max(w >> (l - 1), 1);
+        const bool from_global = l == first_level;
+        const ushort4* gsrc = cube + cube_level_offset(w, l - 1) + (long long)face * wp * wp;
+        const ushort4* lsrc = ((l - first_level) & 1) ? s_a : s_b;  // the previous level's buffer
+        ushort4* ldst = ((l - first_level) & 1) ? s_b : s_a;
+        ushort4* gdst = cube + cube_level_offset(w, l) + (long long)face * wl * wl;
+        for (int t = threadIdx.x; t < wl * wl; t += 1024) {
+            const int y = t / wl, x = t % wl;
+            const int x0 = min(2 * x, wp - 1), x1 = min(2 * x + 1, wp - 1), y0 = min(2 * y, wp - 1),
+                      y1 = min(2 * y + 1, wp - 1);
+            const ushort4* src = from_global ? gsrc : lsrc;
+            const ushort4 o = box4(src[y0 * wp + x0], src[y0 * wp + x1], src[y1 * wp + x0], src[y1 * wp + x1]);
+            ldst[t] = o;
+            gdst[t] = o;
+        }
+        __syncthreads();
+    }
 }
 
 // EnvRadianceSH.comp: coefficient blockIdx.x from 1024 directions on a 32 x 32 stratified
@@ -230,12 +270,17 @@ extern "C" int skyhip_environment_luminance(SkyhipContext* ctx, const SkyhipAtmo
                        width);
     SKYHIP_LAUNCH_CHECK("env_luminance_kernel");
     const int levels = log2i(width) + 1;
-    for (int l = 1; l < levels; ++l) {
+    int l = 1;
+    for (; l < levels && (width >> l) > kMipTailWidth; ++l) {  // levels wider than the tail's buffer
         const int wl = width >> l;
         hipLaunchKernelGGL(cube_mip_kernel, dim3(div_up(6LL * wl * wl, 256)), dim3(256), 0, ctx->stream, cube, width,
                            l);
     }
     SKYHIP_LAUNCH_CHECK("cube_mip_kernel");
+    if (l < levels) {
+        hipLaunchKernelGGL(cube_mip_tail_kernel, dim3(6), dim3(1024), 0, ctx->stream, cube, width, l, levels);
+        SKYHIP_LAUNCH_CHECK("cube_mip_tail_kernel");
+    }
     return SKYHIP_OK;
 }
 

@@ -20,7 +20,7 @@ from skyrendering_amd.ibl import IBL, EnvironmentLuminance, cube_texels
 pytestmark = pytest.mark.gpu
 
 
-def _scene(ctx, theta=45.0):
+def _scene(ctx, theta=45.0, width=None):
     p = presets.earth_100km()
     atm = Atmosphere(ctx)
     atm.update_luts(p)
@@ -29,7 +29,7 @@ def _scene(ctx, theta=45.0):
                                     view_projection=cam.view_projection())
     rend = AtmosphereRenderer(ctx, AtmosphereRenderInitParameters(sky_view_lut_size=(192, 108)))
     rend.render(atm, p, rp, aerial_perspective=False)
-    env = EnvironmentLuminance(ctx)
+    env = EnvironmentLuminance(ctx) if width is None else EnvironmentLuminance(ctx, width)
     env.update(atm, rend)
     return atm, rend, env
 
@@ -41,6 +41,17 @@ def _ulp_or_rel(got_bits, ref_bits, rel=1e-3):
     err = np.abs(g - r)
     ok = (err <= ulp) | (err <= rel * np.abs(r))
     return ok, err / np.maximum(np.abs(r), 1e-6)
+
+
+@pytest.mark.parametrize("width", [256, 4])
+def test_cube_mip_chain_bit_exact(ctx, width):
+    """Mip levels wider than 64 texels are built one launch per level, the rest of the chain in
+    one launch per face from levels held in LDS: a 256^2 cube takes both paths (level 1 per
+    level, levels 2-8 in the tail), a 4^2 cube only the tail; both bit-exact against the
+    oracle's 2x2 box applied level by level to the GPU's level 0."""
+    _, _, env = _scene(ctx, width=width)
+    gpu = env.texture.download(np.uint16, (cube_texels(width, env.levels), 4))
+    assert np.array_equal(oracle.cube_mips(gpu, width), gpu)
 
 
 def test_environment_cube_matches_oracle(ctx):
