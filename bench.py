@@ -56,7 +56,7 @@ BATCH_FLOP_PER_STEP = 237.5
 SHADOW_FP32_FLOP_PER_FRAME = 4.459e8
 SHADOW_HBM_BYTES_PER_FRAME = 61.0e6
 ENV_FP32_FLOP_PER_UPDATE = 1.031e9
-ENV_HBM_BYTES_PER_UPDATE = 9.85e6
+ENV_HBM_BYTES_PER_UPDATE = 8.62e6
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
@@ -286,14 +286,14 @@ def bench_environment(ctx, reps=20):
     out = {"workload": f"environment cube {w}^2 x 6 RGBA16F + {env.levels - 1} mip levels, SH9 (9 x 1024 samples), "
                        f"prefiltered cube {pw}^2 x {n_r} roughness levels (GGX, 1-64 samples), Earth, sun 45 deg",
            "ms_per_update": round(ms, 4), "env_texels": 6 * w * w, "prefilter_samples": prefilter_samples,
-           "bound": "launch latency (14 dependent launches of at most 98,304 threads)",
+           "bound": "latency (4 dependent launches: cube, mip chain, SH9, the five prefiltered levels)",
            "roofline": {"bound": "valu", "unit": "TFLOP/s", "fp32_flop_per_update": ENV_FP32_FLOP_PER_UPDATE,
                         "achieved": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12, 3),
                         "peak": FP32_PEAK_TFLOPS,
                         "frac": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                         "traffic": int(ENV_HBM_BYTES_PER_UPDATE),
                         "traffic_gbs": round(ENV_HBM_BYTES_PER_UPDATE / (ms * 1e-3) / 1e9, 1),
-                        "note": "latency-bound chain of 14 launches (the prefilter's five levels take 79 µs of it); "
+                        "note": "latency-bound chain of 4 launches (the prefilter's five levels take 55 µs of it); "
                                 "FP32 operations and HBM bytes measured per update with rocprofv3 counters"}}
     import oracle
     sv_l, sv_t = rend.sky_view()
