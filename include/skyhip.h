@@ -310,16 +310,17 @@ typedef struct SkyhipCloudMaterial {
      * fixed to the reference's: REPEAT + LINEAR mag + NEAREST_MIPMAP_NEAREST min
      * for the default materials (VolumetricCloudDefaultMaterial.cpp:112-116),
      * CLAMP_TO_BORDER(0) for the voxel material (VolumetricCloudVoxelMaterial.cpp:30-37). */
-    const uint8_t* cloud_map;    /* RG8 2D, cloud_map_size^2 */
+    const uint8_t* cloud_map;    /* RG8 2D, cloud_map_size^2 (a power of two <= 16384) */
     int32_t cloud_map_size;
     int32_t cloud_map_levels;
-    const uint8_t* detail;       /* R8 3D, detail_size^3 */
+    const uint8_t* detail;       /* R8 3D, detail_size^3 (a power of two <= 1024) */
     int32_t detail_size;
     int32_t detail_levels;
-    const uint8_t* displacement; /* RGBA8 2D, displacement_size^2 (DEFAULT0 only) */
+    const uint8_t* displacement; /* RGBA8 2D, displacement_size^2 (a power of two <= 16384; DEFAULT0 only) */
     int32_t displacement_size;
     int32_t displacement_levels;
-    const uint8_t* voxel_texture; /* R8 3D, voxel_dim[0] x voxel_dim[1] x voxel_dim[2] (VOXEL only) */
+    const uint8_t* voxel_texture; /* R8 3D, voxel_dim[0] x voxel_dim[1] x voxel_dim[2], each in [1, 4096],
+                                   * with more than one level at most 2^32 texels over the levels (VOXEL only) */
     int32_t voxel_dim[3];
     int32_t voxel_levels;
     const uint16_t* voxel_grid;   /* FP16 3D, x fastest, voxel_grid_dim[0..2] (VOXEL_F16 only) */

@@ -437,7 +437,8 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
             make_tex3d(d.detail, m->detail, dim, m->detail_levels);
             d.detail.cells = reinterpret_cast<const uint2*>(m->detail_cells);
             if (m->kind == SKYHIP_MATERIAL_DEFAULT0) {
-                if (!m->displacement || !pow2(m->displacement_size) || m->displacement_levels < 1 ||
+                if (!m->displacement || !pow2(m->displacement_size) || m->displacement_size > 16384 ||
+                    m->displacement_levels < 1 ||
                     m->displacement_levels > skyhip_mip_levels(m->displacement_size, m->displacement_size, 1))
                     return fail(SKYHIP_ERR_INVALID_SIZE, "bad displacement texture");
                 make_tex2d(d.displacement, m->displacement, m->displacement_size, m->displacement_levels, 4);
@@ -456,6 +457,14 @@ int build_material(const SkyhipCloudMaterial* m, MaterialDev& d) {
             if (m->voxel_levels < 1 ||
                 m->voxel_levels > skyhip_mip_levels(m->voxel_dim[0], m->voxel_dim[1], m->voxel_dim[2]))
                 return fail(SKYHIP_ERR_INVALID_SIZE, "bad voxel mip level count");
+            {  // the levels' offsets are 32-bit (level 0 is indexed in 64 bits)
+                unsigned long long texels = 0;
+                for (int l = 0; l < m->voxel_levels; ++l)
+                    texels += (unsigned long long)std::max(m->voxel_dim[0] >> l, 1) * std::max(m->voxel_dim[1] >> l, 1) *
+                              std::max(m->voxel_dim[2] >> l, 1);
+                if (m->voxel_levels > 1 && texels > 0xffffffffull)
+                    return fail(SKYHIP_ERR_INVALID_SIZE, "voxel mip chain of more than 2^32 texels");
+            }
             make_tex3d(d.voxel_tex, m->voxel_texture, m->voxel_dim, m->voxel_levels);
             return SKYHIP_OK;
         case SKYHIP_MATERIAL_VOXEL_F16:

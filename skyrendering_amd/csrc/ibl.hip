@@ -294,7 +294,7 @@ extern "C" int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment
         return fail(SKYHIP_ERR_INVALID_SIZE, "environment width must be a power of two <= 4096");
     if (!is_pow2(prefiltered_width) || prefiltered_width > 4096)
         return fail(SKYHIP_ERR_INVALID_SIZE, "prefiltered width must be a power of two <= 4096");
-    if (roughness_count < 1 || (1 << (roughness_count - 1)) > prefiltered_width)
+    if (roughness_count < 1 || roughness_count > kMaxPrefilterLevels || (1 << (roughness_count - 1)) > prefiltered_width)
         return fail(SKYHIP_ERR_INVALID_SIZE, "roughness_count outside [1, log2(prefiltered_width) + 1]");
     const ushort4* env = reinterpret_cast<const ushort4*>(environment_cube_rgba16f);
     const int elevels = log2i(environment_width) + 1;

@@ -323,6 +323,26 @@ def test_raymarch_rejects_bad_input(ctx):
                                  scene.blue_noise.ptr, 128, None, None, scene.out_rgba.ptr, scene.out_dist.ptr, 64, 36,
                                  None, None)
     assert rc == _abi.ERR_INVALID_ARGUMENT
+    # Texture sizes whose packed mip offsets (32-bit) would overflow are refused before any
+    # launch: a DEFAULT0 displacement map past 16384^2, a mipmapped voxel volume of more than
+    # 2^32 texels.  The pointers are never read.
+    pos = ctx.upload(np.zeros((4, 3), np.float32))
+    out = ctx.alloc(16)
+    disp = _abi.CloudMaterial()
+    C.memmove(C.byref(disp), C.byref(scene.mat_block), C.sizeof(disp))
+    disp.kind = _abi.MATERIAL_DEFAULT0
+    disp.displacement = scene.mat_block.cloud_map
+    disp.displacement_size, disp.displacement_levels = 32768, 1
+    vox = _abi.CloudMaterial()
+    C.memmove(C.byref(vox), C.byref(scene.mat_block), C.sizeof(vox))
+    vox.kind = _abi.MATERIAL_VOXEL
+    vox.voxel_texture = scene.mat_block.cloud_map
+    vox.voxel_dim[0], vox.voxel_dim[1], vox.voxel_dim[2], vox.voxel_levels = 4096, 4096, 512, 2
+    for m, what in ((disp, b"displacement"), (vox, b"voxel mip chain")):
+        rc = L.skyhip_cloud_sample_sigma_t(ctx.handle, C.byref(scene.common), C.byref(m), pos.ptr, 4, out.ptr)
+        assert rc == _abi.ERR_INVALID_SIZE and what in L.skyhip_last_error(), (what, rc)
+    pos.free()
+    out.free()
 
 
 @pytest.mark.parametrize("lod_bias,config2", [(-30.0, False), (None, False), (-30.0, True), (None, True)])

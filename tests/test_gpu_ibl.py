@@ -87,3 +87,18 @@ def test_sh9_and_prefiltered_radiance_match_oracle(ctx):
     print(f"prefiltered: {ok.mean():.6f} within 1 ulp / 1e-3, max rel {rel.max():.2e}")
     assert ok.mean() >= 0.999 and rel.max() < 2e-2
     assert np.array_equal(pre[:6 * w * w], cube[:6 * w * w])  # roughness 0: the source texels
+
+
+def test_ibl_precompute_rejects_bad_roughness_counts(ctx):
+    """roughness_count must lie in [1, log2(prefiltered_width) + 1] (13 at 4096); counts past
+    that, including ones whose 1 << (count - 1) would overflow, are refused before any launch."""
+    from skyrendering_amd import _abi
+    _, _, env = _scene(ctx)
+    ibl = IBL(ctx)
+    L = _abi.lib()
+    for width, count in ((4096, 0), (4096, 14), (4096, 40), (128, 9), (128, -3)):
+        rc = L.skyhip_ibl_precompute(ctx.handle, env.texture.ptr, env.width, ibl.env_radiance_sh_buffer.ptr,
+                                     ibl.prefiltered_radiance_texture.ptr, width, count)
+        assert rc == _abi.ERR_INVALID_SIZE, (width, count, rc)
+        assert b"roughness_count" in L.skyhip_last_error()
+    ibl.precompute(env)  # the context is still usable
