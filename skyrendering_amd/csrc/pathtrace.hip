@@ -605,11 +605,13 @@ __device__ __forceinline__ float4 add_sample(const PtArgs& a, float u, float v, 
     return acc;
 }
 
-// Five waves per SIMD (96 VGPRs; the compiler spills the rest of the path state to scratch):
-// more waves hide more of the tracking's fetch latency.  Measured on the 1024^2 brick-majorant
-// workload with the nested loops: 3 waves (no cap) 14.2 ms per frame, 4 waves 12.0, 5 waves
-// 11.2, 6 waves 11.4, 7 and 8 waves 12.0-12.2; with the flat loops and cell-corner lookups at
-// 16^3: 4 waves 7.81-7.88, 5 waves 7.27-7.28, 6 waves 9.45.  Every variant gives the same image.
+// Five waves per SIMD (96 VGPRs; 3 VGPRs of the path state spill to scratch): more waves hide
+// more of the tracking's fetch latency.  Measured on the 1024^2 brick-majorant workload with the
+// nested loops: 3 waves (no cap) 14.2 ms per frame, 4 waves 12.0, 5 waves 11.2, 6 waves 11.4, 7
+// and 8 waves 12.0-12.2; with the flat loops and cell-corner lookups at 16^3: 4 waves 7.81-7.88,
+// 5 waves 7.27-7.28 (24 VGPRs spilled), 6 waves 9.45; with the accumulator read after the path:
+// 5 waves 7.26-7.27 (3 spilled), 6 waves 7.51-7.58 (21 spilled).  Every variant gives the same
+// image.
 template <int kMat, bool kBricks, bool kCount, bool kCells, bool kFlat>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void path_trace_kernel(PtArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -618,11 +620,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
     PtCount wc;
     const bool active = x < a.region[2] && y < a.region[3];
     if (active) {
-        const float u = (float(x) + 0.5f) / float(a.width), v = (float(y) + 0.5f) / float(a.height);
-        float frag_dist;
-        const F3 view = unproject(a.inv_mvp, u * 2.0f - 1.0f, v * 2.0f - 1.0f, 1.0f, a.cam, &frag_dist);
-        const long long pix = (long long)y * a.width + x;
-        float4 acc = a.accum[pix];
+        F3 view;
+        {
+            const float u = (float(x) + 0.5f) / float(a.width), v = (float(y) + 0.5f) / float(a.height);
+            float frag_dist;
+            view = unproject(a.inv_mvp, u * 2.0f - 1.0f, v * 2.0f - 1.0f, 1.0f, a.cam, &frag_dist);
+        }
         Rng rng;
         rng.pcg = a.prng;
         const uint32_t seed_xy = rng.hash(rng.hash(uint32_t(x)) + uint32_t(y));
@@ -631,10 +634,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
             bool hs;
             float st;
             F3 L = trace<kMat, kBricks, kCount, kCells, kFlat>(rng, view, hs, st, wc);
-            acc = add_sample(a, u, v, view.z, L, hs, st, acc);
+            // The accumulator and the pixel centre are read again after the path (one read and
+            // one write per frame, the same additions in frame order), so that they are not
+            // held through the tracking loops.
+            const PtArgs& ak = kernel_args();
+            const float u = (float(x) + 0.5f) / float(ak.width), v = (float(y) + 0.5f) / float(ak.height);
+            const long long pix = (long long)y * ak.width + x;
+            ak.accum[pix] = add_sample(ak, u, v, view.z, L, hs, st, ak.accum[pix]);
         }
-        a.accum[pix] = acc;
-        if (a.mask) a.mask[pix] = 1;
+        if (a.mask) a.mask[(long long)y * a.width + x] = 1;
     }
     if (kCount) {
         unsigned long long v[5] = {active ? (unsigned long long)a.frames : 0ull, wc.lookups, wc.segments,
