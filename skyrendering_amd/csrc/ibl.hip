@@ -165,9 +165,10 @@ __device__ inline void onb_frame(F3 N, F3& t0, F3& t1) {
 constexpr int kMaxPrefilterLevels = 13;  // roughness_count <= log2(4096) + 1
 
 // PrefilterRadiance.comp for level `level` (width wl) of the output cube at roughness r.  Each
-// texel's samples are spread over a group of `group` lanes (a power of two >= the sample count n,
-// at most 64; 1 at roughness 0, where n = 1): lane k evaluates sample k into LDS, and the group's
-// first lane adds the terms in sample order, so the sums are the one-thread-per-texel loop's.
+// texel's samples are spread over a group of `group` = n lanes (the level's sample count, at most
+// 64; 1 at roughness 0), 256 / n texels to a workgroup: lane k evaluates sample k into LDS, and
+// the group's first lane adds the terms in sample order, so the sums are the
+// one-thread-per-texel loop's.
 // The rough levels have few texels (384 at 8^2 x 6) and up to 64 samples each, which one thread
 // per texel left as a chain of dependent cube fetches.
 // All levels run in one launch: workgroups [block_start[l], block_start[l + 1]) build level l.
@@ -175,7 +176,7 @@ struct PrefilterLevels {
     int count;
     int block_start[kMaxPrefilterLevels + 1];
     float roughness[kMaxPrefilterLevels];
-    int group[kMaxPrefilterLevels];
+    int group[kMaxPrefilterLevels];  // lanes per texel (the sample count)
 };
 
 __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restrict__ env, int ew, int elevels,
@@ -186,15 +187,16 @@ __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restric
     const float roughness = lv.roughness[level];
     const int group = lv.group[level];
     const int wl = max(pw >> level, 1);
-    const int j = int(threadIdx.x) / group, k = int(threadIdx.x) % group;
-    const int i = (int(blockIdx.x) - lv.block_start[level]) * (256 / group) + j;
-    const bool texel = i < 6 * wl * wl;
+    const int per_block = 256 / group;
+    const int j = int(threadIdx.x) / group, k = int(threadIdx.x) - j * group;
+    const int i = (int(blockIdx.x) - lv.block_start[level]) * per_block + j;
+    const bool texel = j < per_block && i < 6 * wl * wl;
     const int face = i / (wl * wl), y = (i / wl) % wl, x = i % wl;
     const F3 R = cube_texel_dir(face, x, y, wl);
     const F3 N = R, V = R;
     const float a = roughness * roughness, a2 = a * a;
-    const float p = powf(roughness, 0.3f);
-    const uint32_t n = uint32_t(1.0f * (1.0f - p) + 64.0f * p);  // uint(mix(1, 64, pow(roughness, 0.3)))
+    // uint(mix(1, 64, pow(roughness, 0.3))), evaluated on the host with the group sizes
+    const uint32_t n = uint32_t(group);
     const float inv_sa_texel = (6.0f * float(wl) * float(wl)) / (4.0f * kPiF);
     float4 term = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // (sample * NoL, NoL), NoL = 0: no contribution
     if (texel && uint32_t(k) < n) {
@@ -308,7 +310,11 @@ extern "C" int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment
     for (int l = 0; l < roughness_count; ++l) {
         const int wl = prefiltered_width >> l;
         lv.roughness[l] = roughness_count > 1 ? float(l) / float(roughness_count - 1) : 0.0f;
-        lv.group[l] = lv.roughness[l] > 0.0f ? 64 : 1;  // at most 64 samples per texel; 1 at roughness 0
+        // The level's sample count uint(mix(1, 64, pow(roughness, 0.3))) in FP32 (1 at roughness 0,
+        // 64 at 1; in between, for every level of up to 13, at least 0.011 from an integer, so no
+        // powf rounding moves it), which the kernel uses as its group size and its sample count.
+        const float p = std::pow(lv.roughness[l], 0.3f);
+        lv.group[l] = std::min(std::max(int(uint32_t(1.0f * (1.0f - p) + 64.0f * p)), 1), 64);
         lv.block_start[l] = int(blocks);
         blocks += div_up(6LL * wl * wl, 256 / lv.group[l]);
     }
