@@ -55,8 +55,8 @@ BATCH_FLOP_PER_STEP = 237.5
 # above) and calibrated HBM bytes, summed over their launches (profiles/cloud_shadow_environment.md).
 SHADOW_FP32_FLOP_PER_FRAME = 4.459e8
 SHADOW_HBM_BYTES_PER_FRAME = 61.0e6
-ENV_FP32_FLOP_PER_UPDATE = 1.031e9
-ENV_HBM_BYTES_PER_UPDATE = 8.62e6
+ENV_FP32_FLOP_PER_UPDATE = 7.10e8
+ENV_HBM_BYTES_PER_UPDATE = 8.66e6
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
@@ -293,7 +293,7 @@ def bench_environment(ctx, reps=20):
                         "frac": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                         "traffic": int(ENV_HBM_BYTES_PER_UPDATE),
                         "traffic_gbs": round(ENV_HBM_BYTES_PER_UPDATE / (ms * 1e-3) / 1e9, 1),
-                        "note": "latency-bound chain of 4 launches (the prefilter's five levels take 55 µs of it); "
+                        "note": "latency-bound chain of 4 launches (the prefilter's five levels take 40 µs of it); "
                                 "FP32 operations and HBM bytes measured per update with rocprofv3 counters"}}
     import oracle
     sv_l, sv_t = rend.sky_view()
