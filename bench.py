@@ -286,14 +286,14 @@ def bench_environment(ctx, reps=20):
     out = {"workload": f"environment cube {w}^2 x 6 RGBA16F + {env.levels - 1} mip levels, SH9 (9 x 1024 samples), "
                        f"prefiltered cube {pw}^2 x {n_r} roughness levels (GGX, 1-64 samples), Earth, sun 45 deg",
            "ms_per_update": round(ms, 4), "env_texels": 6 * w * w, "prefilter_samples": prefilter_samples,
-           "bound": "latency (4 dependent launches: cube, mip chain, SH9, the five prefiltered levels)",
+           "bound": "latency (3 dependent launches: cube, mip chain, SH9 with the five prefiltered levels)",
            "roofline": {"bound": "valu", "unit": "TFLOP/s", "fp32_flop_per_update": ENV_FP32_FLOP_PER_UPDATE,
                         "achieved": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12, 3),
                         "peak": FP32_PEAK_TFLOPS,
                         "frac": round(ENV_FP32_FLOP_PER_UPDATE / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                         "traffic": int(ENV_HBM_BYTES_PER_UPDATE),
                         "traffic_gbs": round(ENV_HBM_BYTES_PER_UPDATE / (ms * 1e-3) / 1e9, 1),
-                        "note": "latency-bound chain of 4 launches (the prefilter's five levels take 40 µs of it); "
+                        "note": "latency-bound chain of 3 launches (SH9 and the prefilter's five levels take 40 µs of it); "
                                 "FP32 operations and HBM bytes measured per update with rocprofv3 counters"}}
     import oracle
     sv_l, sv_t = rend.sky_view()

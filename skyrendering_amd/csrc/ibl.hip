@@ -6,9 +6,9 @@
 //  - cube_mip_kernel, cube_mip_tail_kernel: the cube's mip chain, a 2x2 box per level
 //    (glGenerateTextureMipmap, AtmosphereRenderer.cpp:242; the filter is the implementation's,
 //    here a box), the levels of at most 64^2 texels a face in one launch;
-//  - sh9_kernel: the 9 spherical-harmonic radiance coefficients (EnvRadianceSH.comp), one
-//    workgroup of 1024 stratified directions per coefficient, summed by the shader's LDS tree;
-//  - prefilter_kernel: the GGX-prefiltered radiance cube, one level per roughness
+//  - prefilter_kernel: the 9 spherical-harmonic radiance coefficients (EnvRadianceSH.comp), one
+//    workgroup per coefficient over 1024 stratified directions summed by the shader's halving
+//    tree, and in the same launch the GGX-prefiltered radiance cube, one level per roughness
 //    (PrefilterRadiance.comp, IBL.cpp:25-45), a texel's samples spread over a group of lanes.
 // Each pass is tiny (at most 98,304 texels); the sequence is bound by launch latency.
 #include "sky_env.h"
@@ -108,13 +108,13 @@ __global__ __launch_bounds__(1024) void cube_mip_tail_kernel(ushort4* cube, int 
     }
 }
 
-// EnvRadianceSH.comp: coefficient blockIdx.x from 1024 directions on a 32 x 32 stratified
-// grid (uniform in cos theta and phi, y up), each sample weighted 4 pi / 1024, summed by
-// the same halving tree over LDS.
-__global__ __launch_bounds__(1024) void sh9_kernel(const ushort4* __restrict__ cube, int w, int levels,
-                                                   float4* __restrict__ out) {
-    __shared__ F3 s[1024];
-    const int index = blockIdx.x, li = threadIdx.x;
+// EnvRadianceSH.comp: coefficient `index` from 1024 directions on a 32 x 32 stratified grid
+// (uniform in cos theta and phi, y up), each sample weighted 4 pi / 1024, summed by the shader's
+// halving tree over the 1024 values (at half = 512, 256, ..., 1: s[i] += s[i + half]).  A
+// 256-thread workgroup holds samples t, t + 256, t + 512 and t + 768 in thread t and forms the
+// tree's first two levels in registers, (s_t + s_(t+512)) + (s_(t+256) + s_(t+768)), the rest
+// over LDS: the same additions in the same order.
+__device__ inline F3 sh9_sample(const ushort4* __restrict__ cube, int w, int levels, int index, int li) {
     const float unit_theta = (0.5f + float(li >> 5)) / 32.0f;
     const float unit_phi = (0.5f + float(li & 31)) / 32.0f;
     const float cos_theta = 1.0f - 2.0f * unit_theta;
@@ -135,14 +135,22 @@ __global__ __launch_bounds__(1024) void sh9_kernel(const ushort4* __restrict__ c
         case 7: c = Y2n * dir.x * dir.z; break;
         default: c = Y22 * (dir.x * dir.x - dir.y * dir.y); break;
     }
-    s[li] = radiance * (c * (4.0f * kPiF / 1024.0f));
+    return radiance * (c * (4.0f * kPiF / 1024.0f));
+}
+__device__ inline void sh9_block(const ushort4* __restrict__ cube, int w, int levels, int index,
+                                 float4* __restrict__ out, F3* s) {
+    const int t = threadIdx.x;
+    const F3 lo = sh9_sample(cube, w, levels, index, t) + sh9_sample(cube, w, levels, index, t + 512);
+    const F3 hi = sh9_sample(cube, w, levels, index, t + 256) + sh9_sample(cube, w, levels, index, t + 768);
+    s[t] = lo + hi;
     __syncthreads();
-    for (int half = 512; half >= 1; half >>= 1) {
-        if (li < half) s[li] = s[li] + s[li + half];
+    for (int half = 128; half >= 1; half >>= 1) {
+        if (t < half) s[t] = s[t] + s[t + half];
         __syncthreads();
     }
-    if (li == 0) out[index] = make_float4(s[0].x, s[0].y, s[0].z, 0.0f);
+    if (t == 0) out[index] = make_float4(s[0].x, s[0].y, s[0].z, 0.0f);
 }
+constexpr int kSh9Blocks = 9;
 
 __device__ inline uint32_t reverse_bits32(uint32_t b) {  // ReverseBits32 (shaders/Base/Noise.glsl:104-111)
     b = (b << 16) | (b >> 16);
@@ -171,7 +179,9 @@ constexpr int kMaxPrefilterLevels = 13;  // roughness_count <= log2(4096) + 1
 // one-thread-per-texel loop's.
 // The rough levels have few texels (384 at 8^2 x 6) and up to 64 samples each, which one thread
 // per texel left as a chain of dependent cube fetches.
-// All levels run in one launch: workgroups [block_start[l], block_start[l + 1]) build level l.
+// All levels run in one launch, after the kSh9Blocks workgroups that compute the SH9
+// coefficients (sh9_block; both read the environment cube only): workgroups kSh9Blocks +
+// [block_start[l], block_start[l + 1]) build level l.
 struct PrefilterLevels {
     int count;
     int block_start[kMaxPrefilterLevels + 1];
@@ -180,16 +190,23 @@ struct PrefilterLevels {
 };
 
 __global__ __launch_bounds__(256) void prefilter_kernel(const ushort4* __restrict__ env, int ew, int elevels,
-                                                        ushort4* __restrict__ out, int pw, PrefilterLevels lv) {
+                                                        float4* __restrict__ sh9_out, ushort4* __restrict__ out,
+                                                        int pw, PrefilterLevels lv) {
     __shared__ float4 s_term[256];
+    __shared__ F3 s_sh[256];
+    if (int(blockIdx.x) < kSh9Blocks) {
+        sh9_block(env, ew, elevels, int(blockIdx.x), sh9_out, s_sh);
+        return;
+    }
+    const int block = int(blockIdx.x) - kSh9Blocks;
     int level = 0;
-    while (level + 1 < lv.count && int(blockIdx.x) >= lv.block_start[level + 1]) ++level;
+    while (level + 1 < lv.count && block >= lv.block_start[level + 1]) ++level;
     const float roughness = lv.roughness[level];
     const int group = lv.group[level];
     const int wl = max(pw >> level, 1);
     const int per_block = 256 / group;
     const int j = int(threadIdx.x) / group, k = int(threadIdx.x) - j * group;
-    const int i = (int(blockIdx.x) - lv.block_start[level]) * per_block + j;
+    const int i = (block - lv.block_start[level]) * per_block + j;
     const bool texel = j < per_block && i < 6 * wl * wl;
     const int face = i / (wl * wl), y = (i / wl) % wl, x = i % wl;
     const F3 R = cube_texel_dir(face, x, y, wl);
@@ -300,9 +317,6 @@ extern "C" int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment
         return fail(SKYHIP_ERR_INVALID_SIZE, "roughness_count outside [1, log2(prefiltered_width) + 1]");
     const ushort4* env = reinterpret_cast<const ushort4*>(environment_cube_rgba16f);
     const int elevels = log2i(environment_width) + 1;
-    hipLaunchKernelGGL(sh9_kernel, dim3(9), dim3(1024), 0, ctx->stream, env, environment_width, elevels,
-                       reinterpret_cast<float4*>(out_sh9_rgba));
-    SKYHIP_LAUNCH_CHECK("sh9_kernel");
     ushort4* out = reinterpret_cast<ushort4*>(out_prefiltered_rgba16f);
     PrefilterLevels lv{};
     lv.count = roughness_count;
@@ -319,8 +333,8 @@ extern "C" int skyhip_ibl_precompute(SkyhipContext* ctx, const void* environment
         blocks += div_up(6LL * wl * wl, 256 / lv.group[l]);
     }
     lv.block_start[roughness_count] = int(blocks);
-    hipLaunchKernelGGL(prefilter_kernel, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream, env, environment_width,
-                       elevels, out, prefiltered_width, lv);
+    hipLaunchKernelGGL(prefilter_kernel, dim3(unsigned(kSh9Blocks + blocks)), dim3(256), 0, ctx->stream, env,
+                       environment_width, elevels, reinterpret_cast<float4*>(out_sh9_rgba), out, prefiltered_width, lv);
     SKYHIP_LAUNCH_CHECK("prefilter_kernel");
     return SKYHIP_OK;
 }
