@@ -534,7 +534,8 @@ int skyhip_environment_luminance(SkyhipContext* ctx, const SkyhipAtmosphereBuffe
  * coefficients of the environment cube (EnvRadianceSH.comp; out_sh9_rgba: device vec4[9],
  * a = 0) and the GGX-prefiltered radiance cube (PrefilterRadiance.comp) with roughness_count
  * levels, level l at roughness l / (roughness_count - 1) and width prefiltered_width >> l
- * (the reference: 128 and 5).  environment_cube: the full mip chain of
+ * (the reference: 128 and 5); widths are powers of two <= 4096 and roughness_count lies in
+ * [1, log2(prefiltered_width) + 1], else SKYHIP_ERR_INVALID_SIZE.  environment_cube: the full mip chain of
  * skyhip_environment_luminance; out_prefiltered: skyhip_cube_texels(prefiltered_width,
  * roughness_count) texels.  Lookups filter bilinearly within a face and linearly between
  * levels (the anisotropy sampler at an explicit level of detail). */
