@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void aerial_perspective_kernel(
 // match the per-pass kernels bit for bit, does not use these.
 // ---------------------------------------------------------------------------
 constexpr int kSetTransmittanceParts = 4;
-constexpr int kSetSkyViewParts = 2;
+constexpr int kSetSkyViewParts = 3;
 constexpr int kSetAerialParts = 8;
 
 template <class S, int kParts>
@@ -168,7 +168,8 @@ __global__ __launch_bounds__(64 * kSetMsParts) void set_multiscattering_kernel(S
 }
 
 // Blocks [0, sv_blocks) build the sky view, kSetSkyViewParts lanes per texel; the rest the
-// aerial perspective, kSetAerialParts lanes per froxel (no blue-noise dither, as the set's
+// aerial perspective, kSetAerialParts lanes per froxel; a wave holds 64 / parts whole groups (a
+// lane left over idles) (no blue-noise dither, as the set's
 // per-pass calls).
 __global__ __launch_bounds__(256) void set_sky_aerial_kernel(
     SkyhipAtmosphereBufferData b, SkyhipAtmosphereRenderBufferData rbd, const float4* __restrict__ T, int tw, int th,
@@ -178,11 +179,12 @@ __global__ __launch_bounds__(256) void set_sky_aerial_kernel(
     const RenderDev R = make_render(rbd);
     F3 L, Tr;
     if (int(blockIdx.x) < sv_blocks) {
-        constexpr int P = kSetSkyViewParts;
-        const int g = blockIdx.x * 256 + threadIdx.x;
-        const int part = g % P;
-        const bool valid = g / P < sw * sh;
-        const int t = valid ? g / P : 0;  // every lane of a group reaches the shuffles
+        constexpr int P = kSetSkyViewParts, kPerWave = 64 / P;  // groups of P lanes within a wave
+        const int g = blockIdx.x * 256 + threadIdx.x, lane = g % 64, slot = lane / P;
+        const int part = lane % P;
+        const int texel = (g / 64) * kPerWave + slot;
+        const bool valid = slot < kPerWave && texel < sw * sh;
+        const int t = valid ? texel : 0;  // every lane of a wave reaches the shuffles
         const SkyViewRay ray = sky_view_ray(A, R, nullptr, 1, t % sw, t / sw, sw, sh);
         scattered_luminance_split<P>(A, T, tw, th, MS, mw, mh, ray.s, ray.dist, R.sv_steps, ray.start_i,
                                      ray.dist > 0.0f, part, L, Tr);
@@ -192,12 +194,13 @@ __global__ __launch_bounds__(256) void set_sky_aerial_kernel(
         }
         return;
     }
-    constexpr int P = kSetAerialParts;
+    constexpr int P = kSetAerialParts, kPerWave = 64 / P;
     const long long g = (long long)(blockIdx.x - sv_blocks) * 256 + threadIdx.x;
     const int n_texels = aw * ah * ad;
-    const int part = int(g % P);
-    const bool valid = g / P < n_texels;
-    const int t = valid ? int(g / P) : 0;
+    const int lane = int(g % 64), slot = lane / P, part = lane % P;
+    const long long texel = (g / 64) * kPerWave + slot;
+    const bool valid = slot < kPerWave && texel < n_texels;
+    const int t = valid ? int(texel) : 0;
     const AerialRay ray = aerial_ray(A, R, rbd.inv_view_projection, nullptr, 1, t % aw, (t / aw) % ah, t / (aw * ah),
                                      aw, ah, ad);
     scattered_luminance_split<P>(A, T, tw, th, MS, mw, mh, ray.s, ray.marching_distance, R.ap_steps, ray.start_i,
@@ -397,8 +400,10 @@ extern "C" int skyhip_atmosphere_lut_set(SkyhipContext* ctx, const SkyhipAtmosph
                        reinterpret_cast<const float4*>(transmittance_rgba), tw, th,
                        reinterpret_cast<float4*>(multiscattering_rgba), mw, mh);
     SKYHIP_LAUNCH_CHECK("set_multiscattering_kernel");
-    const int sv_blocks = int(div_up((long long)sky_view_width * sky_view_height * kSetSkyViewParts, 256));
-    const int ap_blocks = int(div_up((long long)ap_width * ap_height * ap_depth * kSetAerialParts, 256));
+    // Whole waves of 64 / parts groups each (a group never straddles two waves).
+    const int sv_blocks = int(div_up(div_up((long long)sky_view_width * sky_view_height, 64 / kSetSkyViewParts) * 64, 256));
+    const int ap_blocks =
+        int(div_up(div_up((long long)ap_width * ap_height * ap_depth, 64 / kSetAerialParts) * 64, 256));
     hipLaunchKernelGGL(set_sky_aerial_kernel, dim3(sv_blocks + ap_blocks), dim3(256), 0, st, *atmosphere, *render,
                        reinterpret_cast<const float4*>(transmittance_rgba), tw, th,
                        reinterpret_cast<const float4*>(multiscattering_rgba), mw, mh,
