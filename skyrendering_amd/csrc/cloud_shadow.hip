@@ -172,7 +172,8 @@ struct FroxelArgs {
 // independent of each other, so its 256 threads evaluate them in parallel, kFroxelChunk slices
 // at a time, into LDS; then one thread per column forms the running mean in slice order.  The
 // slice distances come from a table that one thread fills with the shader's running
-// t += step_size, so every term, sum and output is the one-thread-per-column loop's.
+// t += step_size, and the column's thread leaves the running sums in the table for all threads to
+// divide and store, so every term, sum and output is the one-thread-per-column loop's.
 constexpr int kFroxelCols = 32;
 constexpr int kFroxelChunk = 128;
 __global__ __launch_bounds__(256) void shadow_froxel_kernel(FroxelArgs a) {
@@ -203,12 +204,17 @@ __global__ __launch_bounds__(256) void shadow_froxel_kernel(FroxelArgs a) {
                 s_term[k][c] = dt.y * (1.0f - s) + 1.0f * s;
             }
         __syncthreads();
-        if (column && g == 0)
+        if (column && g == 0)  // the running sum in slice order, left in place of the terms
             for (int k = 0; k < n; ++k) {
                 sum += s_term[k][c];
+                s_term[k][c] = sum;
+            }
+        __syncthreads();
+        if (column)  // the means and their R16 UNORM stores, all threads
+            for (int k = g; k < n; k += 256 / kFroxelCols) {
                 const int z = z0 + k;
-                const float vis = sum / float(z + 1);
-                a.out[((long long)z * a.h + y) * a.w + x] = uint16_t(rintf(clamp01(vis) * 65535.0f));  // R16 UNORM
+                const float vis = s_term[k][c] / float(z + 1);
+                a.out[((long long)z * a.h + y) * a.w + x] = uint16_t(rintf(clamp01(vis) * 65535.0f));
             }
         __syncthreads();  // the next chunk overwrites the tables
     }
