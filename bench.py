@@ -329,14 +329,14 @@ def bench_cloud_shadow(ctx, scene, reps=20):
     fw, fh, fd = sh.froxel_size
     out = {"workload": f"cloud shadow map {mw}x{mh} (Material1 + Worley volume, temporal), 2 blur passes, shadow froxel "
                        f"{fw}x{fh}x{fd}", "ms_per_frame": round(ms, 4),
-           "bound": "latency (4 dependent launches; the map's 262k threads march 6-12 material samples each)",
+           "bound": "latency (3 dependent launches: map, both blur passes, froxel; the map's 262k threads march 6-12 material samples each)",
            "roofline": {"bound": "valu", "unit": "TFLOP/s", "fp32_flop_per_frame": SHADOW_FP32_FLOP_PER_FRAME,
                         "achieved": round(SHADOW_FP32_FLOP_PER_FRAME / (ms * 1e-3) / 1e12, 3),
                         "peak": FP32_PEAK_TFLOPS,
                         "frac": round(SHADOW_FP32_FLOP_PER_FRAME / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                         "traffic": int(SHADOW_HBM_BYTES_PER_FRAME),
                         "traffic_gbs": round(SHADOW_HBM_BYTES_PER_FRAME / (ms * 1e-3) / 1e9, 1),
-                        "note": "latency-bound chain of four launches; FP32 operations and HBM bytes measured "
+                        "note": "latency-bound chain of three launches; FP32 operations and HBM bytes measured "
                                 "per frame with rocprofv3 counters"}}
     import oracle
     import ctypes as C

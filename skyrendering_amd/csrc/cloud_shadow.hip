@@ -4,7 +4,7 @@
 //    SampleSigmaT, stored as (mid-shell depth, transmittance) and blended 1:4 into the
 //    previous frame's map reprojected (VolumetricCloudShadowMap.comp);
 //  - shadow_blur_kernel: the separable 9-tap Gaussian, x then y, clamped at the edges
-//    (VolumetricCloudShadowMapBlur.comp);
+//    (VolumetricCloudShadowMapBlur.comp), both passes in one launch;
 //  - shadow_froxel_kernel: per froxel column (viewport / 12), the running mean of the
 //    blurred map's transmittance along the view ray over 128 slices, stored as R16 UNORM
 //    (VolumetricCloudShadowFroxel.comp with SampleCloudShadowTransmittance,
@@ -131,31 +131,48 @@ __global__ __launch_bounds__(256) void shadow_map_kernel(ShadowMapArgs a) {
     a.out[i] = res;
 }
 
-// One pass of the separable Gaussian (weights of VolumetricCloudShadowMapBlur.comp:10).
-__global__ __launch_bounds__(256) void shadow_blur_kernel(const float2* __restrict__ in, float2* __restrict__ out,
-                                                          int w, int h, int axis) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= w * h) return;
-    const int x = i % w, y = i / w;
+// Both passes of the separable 9-tap Gaussian (weights of VolumetricCloudShadowMapBlur.comp:10),
+// x then y, clamped at the edges, in one launch.  A 16x16 tile's workgroup computes the x pass
+// for its rows and the 4 rows above and below them (rows past the map's edge clamped to it, as
+// the y pass's taps are), keeps those in LDS, stores its own rows of them as the x-pass map
+// `mid`, and runs the y pass from LDS.  Each value is the two-launch form's sum, the same
+// multiply-adds in the same order.
+constexpr int kBlurTile = 16, kBlurHalo = 4;
+__global__ __launch_bounds__(256) void shadow_blur_kernel(const float2* __restrict__ in, float2* __restrict__ mid,
+                                                          float2* __restrict__ out, int w, int h) {
+    __shared__ float2 s_x[kBlurTile + 2 * kBlurHalo][kBlurTile];
     const float wt[5] = {0.227027f, 0.1945946f, 0.1216216f, 0.054054f, 0.016216f};
-    float2 c = in[i];
+    const int x0 = int(blockIdx.x) * kBlurTile, y0 = int(blockIdx.y) * kBlurTile;
+    for (int i = threadIdx.x; i < (kBlurTile + 2 * kBlurHalo) * kBlurTile; i += 256) {
+        const int ly = i / kBlurTile, lx = i % kBlurTile;
+        const int x = x0 + lx, yr = y0 - kBlurHalo + ly, y = min(max(yr, 0), h - 1);
+        if (x >= w) continue;
+        const float2 c = in[y * w + x];
+        float rx = c.x * wt[0], ry = c.y * wt[0];
+        for (int k = 1; k < 5; ++k) {
+            const float2 a = in[y * w + max(x - k, 0)], b = in[y * w + min(x + k, w - 1)];
+            rx += a.x * wt[k];
+            ry += a.y * wt[k];
+            rx += b.x * wt[k];
+            ry += b.y * wt[k];
+        }
+        s_x[ly][lx] = make_float2(rx, ry);
+        if (ly >= kBlurHalo && ly < kBlurHalo + kBlurTile && yr < h) mid[yr * w + x] = make_float2(rx, ry);
+    }
+    __syncthreads();
+    const int lx = int(threadIdx.x) % kBlurTile, ly = int(threadIdx.x) / kBlurTile;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const float2 c = s_x[ly + kBlurHalo][lx];
     float rx = c.x * wt[0], ry = c.y * wt[0];
     for (int k = 1; k < 5; ++k) {
-        int i1, i2;
-        if (axis == 0) {
-            i1 = y * w + max(x - k, 0);
-            i2 = y * w + min(x + k, w - 1);
-        } else {
-            i1 = max(y - k, 0) * w + x;
-            i2 = min(y + k, h - 1) * w + x;
-        }
-        const float2 a = in[i1], b = in[i2];
+        const float2 a = s_x[ly + kBlurHalo - k][lx], b = s_x[ly + kBlurHalo + k][lx];
         rx += a.x * wt[k];
         ry += a.y * wt[k];
         rx += b.x * wt[k];
         ry += b.y * wt[k];
     }
-    out[i] = make_float2(rx, ry);
+    out[y * w + x] = make_float2(rx, ry);
 }
 
 struct FroxelArgs {
@@ -275,13 +292,9 @@ extern "C" int skyhip_cloud_shadow_map(SkyhipContext* ctx, const SkyhipCloudComm
         default: launch_shadow_map<SKYHIP_MATERIAL_VOXEL_F16>(ctx->stream, a); break;
     }
     SKYHIP_LAUNCH_CHECK("shadow_map_kernel");
-    const unsigned blocks = div_up((long long)width * height, 256);
-    hipLaunchKernelGGL(shadow_blur_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
-                       reinterpret_cast<const float2*>(out_raw_rg32f), reinterpret_cast<float2*>(scratch_rg32f), width,
-                       height, 0);
-    hipLaunchKernelGGL(shadow_blur_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
-                       reinterpret_cast<const float2*>(scratch_rg32f), reinterpret_cast<float2*>(out_blurred_rg32f),
-                       width, height, 1);
+    hipLaunchKernelGGL(shadow_blur_kernel, dim3(div_up(width, kBlurTile), div_up(height, kBlurTile)), dim3(256), 0,
+                       ctx->stream, reinterpret_cast<const float2*>(out_raw_rg32f), reinterpret_cast<float2*>(scratch_rg32f),
+                       reinterpret_cast<float2*>(out_blurred_rg32f), width, height);
     SKYHIP_LAUNCH_CHECK("shadow_blur_kernel");
     return SKYHIP_OK;
 }
