@@ -53,10 +53,10 @@ BATCH_FLOP_PER_STEP = 237.5
 # The cloud shadow frame (map, two blur passes, froxel) and the environment update (cube, mips,
 # SH9, five prefiltered levels) of this bench's workloads: FP32 operations (the SQ counters as
 # above) and calibrated HBM bytes, summed over their launches (profiles/cloud_shadow_environment.md).
-SHADOW_FP32_FLOP_PER_FRAME = 4.459e8
-SHADOW_HBM_BYTES_PER_FRAME = 61.0e6
+SHADOW_FP32_FLOP_PER_FRAME = 4.472e8
+SHADOW_HBM_BYTES_PER_FRAME = 56.6e6
 ENV_FP32_FLOP_PER_UPDATE = 7.10e8
-ENV_HBM_BYTES_PER_UPDATE = 8.66e6
+ENV_HBM_BYTES_PER_UPDATE = 8.34e6
 
 # HBM traffic per launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this
 # bench (KiB per dispatch), each divided by what the same counter reads per KiB moved by a
