@@ -68,7 +68,7 @@ FETCH_COUNTS_PER_KIB = 0.5
 WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
     "cloud_raymarch_kernel<5,false>": (36040.0, 25686.0),
-    "path_trace_kernel<4,true,false,true,true>": (4.1582e6, 2.697e4),
+    "path_trace_kernel<4,true,false,true,true>": (4.5146e6, 2.8026e4),
     "path_trace_refill_kernel<4,true,true,false>": (3.5035e6, 6.771e4),
 }
 
