@@ -69,7 +69,7 @@ WRITE_COUNTS_PER_KIB = 1.0
 PROFILED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) in KiB per dispatch, mean over the profiled launches
     "cloud_raymarch_kernel<5,false>": (36040.0, 25686.0),
     "path_trace_kernel<4,true,false,true,true>": (4.5146e6, 2.8026e4),
-    "path_trace_refill_kernel<4,true,true,false>": (3.5035e6, 6.771e4),
+    "path_trace_kernel<4,true,false,true,true> at 8^3 bricks": (3.4176e6, 2.8265e4),
 }
 
 
@@ -242,9 +242,9 @@ def bench_pathtrace(ctx, frames, reference_frames=1):
         "gvertices_per_s": round(counts["vertices"] / (ms * 1e-3) / 1e9, 4),
         "brick8": {"ms_per_frame": round(ms8, 4), "frames_timed": frames8,
                    "density_lookups_per_frame": counts8["lookups"], "majorant_segments_per_frame": counts8["segments"],
-                   "kernel": "path_trace_refill_kernel<4,true,true,false>", "algorithmic_bytes_per_launch": alg8,
+                   "kernel": "path_trace_kernel<4,true,false,true,true>", "algorithmic_bytes_per_launch": alg8,
                    "hbm_frac": round(alg8 / (ms8 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                   "traffic": profiled_traffic("path_trace_refill_kernel<4,true,true,false>")},
+                   "traffic": profiled_traffic("path_trace_kernel<4,true,false,true,true> at 8^3 bricks")},
         "single_majorant_ms_per_frame": round(ref_ms, 3),
         "single_majorant_density_lookups_per_frame": ref_counts["lookups"],
         "brick_speedup_vs_single_majorant": round(ref_ms / ms, 2),

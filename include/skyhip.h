@@ -865,7 +865,7 @@ int skyhip_voxel_grid_cells(SkyhipContext* ctx, const uint16_t* grid, int32_t di
  *                  (a counting kernel variant; leave NULL when timing).
  * With brick majorants, each call first writes the bricks' inverse majorants (1 / (majorant *
  * uDensity), 0 for an empty brick) into a context workspace of one float per brick, and with
- * bricks of at most 8 texels and the cell-corner copy bound, each frame is one persistent
+ * bricks of 4 texels and the cell-corner copy bound, each frame is one persistent
  * launch whose waves take 8x8-pixel tiles from a context counter.  The first call that needs
  * them allocates these (the workspace grows to the largest brick grid traced; skyhip_destroy
  * frees both).  Calls on one context run in stream order and share them. */

@@ -801,14 +801,15 @@ __global__ __launch_bounds__(256) void inv_majorant_kernel(const float* __restri
 // Workgroups of path_trace_refill_kernel per CU: 5 of 4 waves, its occupancy at the 5-wave cap.
 constexpr int kRefillBlocksPerCu = 5;
 // Brick-majorant frames with the cell-corner copy run on one of two forms, chosen by the brick
-// edge (same samples either way).  Measured on the 1024^2, 512^3-grid workload (ms per frame,
-// one box): per-pixel kernel with flat tracking loops 7.30-7.36 at 16^3 and 8.49-8.50 at 8^3;
-// refill kernel with nested loops 8.43-8.48 at 8^3; refill kernel with flat loops 8.18-8.24 at
-// 16^3 and 9.61-9.76 at 8^3.  Before the inverse majorants and the cheaper draws the per-pixel
-// flat kernel was 9.29 at 8^3 against the nested refill kernel's 8.59-8.67: small bricks make
-// crossings outnumber lookups (240 M against 82 M per frame at 8^3), and the flat loop carries
-// the crossing body in every iteration.
-constexpr int kRefillMaxBrick = 8;
+// edge (same samples either way): the refill kernel for bricks of 4 texels, the per-pixel kernel
+// above.  Measured on the 1024^2, 512^3-grid workload (ms per frame, one box): with one wave per
+// 8x8-pixel workgroup the per-pixel kernel with flat tracking loops takes 6.65-6.66 at 16^3 and
+// 7.74-7.77 at 8^3, against 8.38-8.48 for the refill kernel with nested loops at 8^3.  With
+// 16x16-pixel workgroups the two forms were level at 8^3 (per-pixel flat 8.49-8.50, refill
+// nested 8.43-8.48; refill flat 9.61-9.76, and 8.18-8.24 at 16^3): small bricks make crossings
+// outnumber lookups (240 M against 82 M per frame at 8^3), and the flat loop carries the crossing
+// body in every iteration.
+constexpr int kRefillMaxBrick = 4;
 
 template <bool kBricks, bool kCount>
 static void launch_path_trace(hipStream_t st, const PtArgs& a, int cu_count) {

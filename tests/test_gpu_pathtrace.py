@@ -121,19 +121,19 @@ def test_cell_corner_copy_and_identical_samples(ctx):
 
 
 def test_refill_kernel_matches_per_pixel_kernel(ctx):
-    """With 8^3 bricks and the cell-corner copy bound, brick-majorant frames run on the persistent
+    """With 4^3 bricks and the cell-corner copy bound, brick-majorant frames run on the persistent
     kernel that refills finished lanes after every bounce, with nested tracking loops; without
     the copy, on the one-lane-per-pixel kernel with flat tracking loops.  A three-frame call over
     a region off the origin whose sides are not multiples of the 8x8 tiles gives bit-identical
     accumulators and the same rendered mask, and pixels outside the region stay untouched."""
     w = cases.small_grid_workload()
-    w["brick"] = 8
+    w["brick"] = 4
     refill = PathTraceScene(ctx, 30, 22, workload=w, bricks=True)
-    assert refill.inputs.cell_corners and refill.grid.brick == 8
-    plain_grid = VoxelGrid(ctx, refill.grid.dims, brick=8, cells=False)
+    assert refill.inputs.cell_corners and refill.grid.brick == 4
+    plain_grid = VoxelGrid(ctx, refill.grid.dims, brick=4, cells=False)
     plain_grid.upload(refill.grid.download())
     w2 = cases.small_grid_workload()
-    w2["brick"] = 8
+    w2["brick"] = 4
     per_pixel = PathTraceScene(ctx, 30, 22, workload=w2, grid=plain_grid, bricks=True)
     assert not per_pixel.inputs.cell_corners
     region = (3, 5, 27, 19)
